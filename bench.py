@@ -1,0 +1,138 @@
+"""Benchmark: Eclat on 1,000,000 transactions x 1,000 items (one MI355X).
+
+A step is one pass of the support-counting path over the resident input:
+the vertical store is built from the COO pairs already in HBM, then the
+device search (level-1 counts, level-2 INT8 Gram on the matrix cores,
+levels >= 3 on the support kernel with on-device candidate generation) runs
+to completion and the pattern forest is copied back.  ``value`` is itemsets
+per second of that step.  The JSON line also carries the support kernel's
+roofline (algorithmic bitset bytes per launch / mean launch time, from HIP
+events on the library's stream) and a CPU baseline: the oracle's OpenMP
+vertical Eclat on the same workload, which doubles as the parity check.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "itemsets/s + bitset GB/s (Eclat 1M×1k), batched vs plain; serve p99 latency"
+HBM_PEAK_GBS = 8000.0
+MINSUP = 1000  # 0.001 x 1,000,000
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(each GPU would mine its own prefix classes; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from cupami_amd import _hip, datagen
+
+    t = time.perf_counter()
+    db = datagen.eclat_1m_1k()
+    labels, cols = datagen.label_columns(db.items)
+    n, items, nnz = db.n_transactions, len(labels), db.rows.size
+    log(f"bench: generated {n} x {items} ({nnz} pairs) in {time.perf_counter() - t:.1f}s")
+
+    dev = torch.device("cuda:0")
+    d_rows = torch.from_numpy(db.rows.view(np.int64)).to(dev)
+    d_cols = torch.from_numpy(cols.view(np.int32)).to(dev)
+    torch.cuda.synchronize()
+    ctx = _hip.context(0)
+
+    def step():
+        bits = ctx.build_bits_device(d_rows.data_ptr(), d_cols.data_ptr(), nnz, n, items)
+        try:
+            return ctx.eclat_mine(bits, MINSUP, use_gram=True)
+        finally:
+            bits.close()
+
+    for i in range(args.warmup):
+        pats = step()
+        log(f"bench: warmup {i} -> {len(pats)} patterns")
+
+    ctx.reset_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pats = step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stats = ctx.stats()
+    n_pat = len(pats)
+    ms_per_step = elapsed / args.steps * 1e3
+    value = n_pat / (elapsed / args.steps)
+    log(f"bench: {args.steps} steps, {ms_per_step:.2f} ms/step, {n_pat} patterns")
+
+    launches = max(stats["launches"], 1)
+    avg_ms = stats["kernel_ms"] / launches
+    bytes_per_launch = stats["bytes"] / launches
+    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9 if avg_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_support_count", "avg_launch_ms": round(avg_ms, 4),
+                "bytes_per_launch": bytes_per_launch, "launches_per_step": launches / args.steps}
+    gram_ms = stats["gram_ms"] / max(stats["gram_launches"], 1)
+    gram_ops = stats["gram_ops"] / max(stats["gram_launches"], 1)
+    gram = {"kernel": "k_gram_i8", "avg_launch_ms": round(gram_ms, 4),
+            "tops": round(gram_ops / (gram_ms / 1e3) / 1e12, 2) if gram_ms > 0 else 0.0,
+            "peak_tops": 5000.0}
+
+    cpu = None
+    parity = None
+    if not args.no_cpu_baseline:
+        import oracle
+        host = oracle.bits_build(db.rows, cols, n, items)
+        threads = oracle.threads()
+        t = time.perf_counter()
+        expect = oracle.eclat(host, MINSUP)
+        cpu_s = time.perf_counter() - t
+        parity = pats.to_dict() == expect
+        cpu = {"value": round(len(expect) / cpu_s, 1), "unit": "itemsets/s", "cores": threads,
+               "kind": "port",
+               "sample": f"whole workload, one run ({len(expect)} itemsets in {cpu_s:.2f}s): "
+                         "oracle/cpm_oracle.c vertical Eclat, OpenMP over prefix classes"}
+        log(f"bench: cpu baseline {cpu_s:.2f}s on {threads} threads, parity={parity}")
+        if not parity:
+            log("bench: PARITY FAILURE against the oracle")
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "itemsets/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (numpy PCG64 seed 202)",
+        "config": {"workload": "eclat_1m_1k", "transactions": n, "items": items,
+                   "lambda": 10, "zipf_s": 1.1, "minsup": MINSUP, "patterns": n_pat,
+                   "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "gram": gram,
+        "levels_per_step": stats["levels"], "parity": parity,
+    }
+    print(json.dumps(line), flush=True)
+    if parity is False:
+        sys.exit(1)
+
+
+if __name__ == "__main__":
+    main()

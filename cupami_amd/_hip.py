@@ -1,0 +1,342 @@
+"""ctypes boundary to ``libcupami_hip.so`` (declared in ``include/cupami_hip.h``).
+
+Status codes map to the reference's exception types (``cuApriori.py:458-471``,
+``_opencl_bitsets.py:92-135``): invalid arguments raise ``ValueError``, HIP
+failures and missing devices ``RuntimeError``, device allocation failures
+``MemoryError``.  A missing or unloadable library raises ``ImportError`` --
+there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("CUPAMI_HIP_LIB", _HERE / "libcupami_hip.so"))
+
+OK, ERR_INVALID, ERR_RUNTIME, ERR_NOMEM, ERR_NODEVICE = 0, -1, -2, -3, -4
+NO_PARENT = 0xFFFFFFFF
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); every symbol include/cupami_hip.h declares
+SIGNATURES = {
+    "cpm_version": (ctypes.c_int, []),
+    "cpm_last_error": (ctypes.c_char_p, []),
+    "cpm_device_count": (ctypes.c_int, []),
+    "cpm_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "cpm_ctx_destroy": (ctypes.c_int, [_vp]),
+    "cpm_ctx_synchronize": (ctypes.c_int, [_vp]),
+    "cpm_ctx_stream": (_vp, [_vp]),
+    "cpm_bits_build": (ctypes.c_int, [_vp, _u64p, _u32p, ctypes.c_uint64, ctypes.c_uint64,
+                                      ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    "cpm_bits_build_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint64,
+                                             ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    "cpm_bits_destroy": (ctypes.c_int, [_vp]),
+    "cpm_bits_shape": (ctypes.c_int, [_vp, _u64p, _u32p, _u64p, _u64p]),
+    "cpm_bits_download": (ctypes.c_int, [_vp, _vp, _u64p]),
+    "cpm_count_supports": (ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint64, ctypes.c_uint32,
+                                          _u64p]),
+    "cpm_intersect_extensions": (ctypes.c_int, [_vp, _u64p, _u64p, ctypes.c_uint32,
+                                                ctypes.c_uint64, ctypes.c_uint64, _u64p, _u64p,
+                                                _u32p, _u32p]),
+    "cpm_pair_gram_i8": (ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint32, _u32p]),
+    "cpm_periodic_eval": (ctypes.c_int, [_vp, _vp, _i64p, ctypes.c_int64, _u32p, ctypes.c_uint64,
+                                         ctypes.c_uint32, _u64p, _i64p]),
+    "cpm_eclat_mine": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_int,
+                                      ctypes.POINTER(_vp)]),
+    "cpm_pfeclat_mine": (ctypes.c_int, [_vp, _vp, _i64p, ctypes.c_int64, ctypes.c_uint64,
+                                        ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "cpm_patterns_count": (ctypes.c_int, [_vp, _u64p]),
+    "cpm_patterns_download": (ctypes.c_int, [_vp, _u32p, _u32p, _u64p, _i64p]),
+    "cpm_patterns_destroy": (ctypes.c_int, [_vp]),
+    "cpm_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
+    "cpm_stats_reset": (ctypes.c_int, [_vp]),
+    "cpm_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+}
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def load():
+    """Load the library (ImportError when it is missing or unloadable)."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise ImportError(
+                    f"GPU mining requires the HIP library {LIB_PATH}; build it with "
+                    "`python -m cupami_amd.build` (hipcc, gfx950)")
+            try:
+                lib = ctypes.CDLL(str(LIB_PATH))
+            except OSError as error:
+                raise ImportError(f"cannot load {LIB_PATH}: {error}") from error
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def check(status):
+    if status == OK:
+        return
+    message = load().cpm_last_error().decode("utf-8", "replace")
+    if status == ERR_INVALID:
+        raise ValueError(message)
+    if status == ERR_NOMEM:
+        raise MemoryError(message)
+    raise RuntimeError(message)
+
+
+def _p(array, ptype):
+    return array.ctypes.data_as(ptype)
+
+
+class Context:
+    """One device and its stream (``cpm_ctx``)."""
+
+    def __init__(self, device=0):
+        lib = load()
+        handle = _vp()
+        status = lib.cpm_ctx_create(int(device), ctypes.byref(handle))
+        if status == ERR_NODEVICE:
+            raise RuntimeError("GPU mining requires a HIP device; " +
+                               lib.cpm_last_error().decode())
+        check(status)
+        self.handle = handle
+        self.device = int(device)
+
+    def close(self):
+        if self.handle:
+            load().cpm_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(load().cpm_ctx_synchronize(self.handle))
+
+    def stream(self):
+        return load().cpm_ctx_stream(self.handle)
+
+    def stats(self):
+        out = (ctypes.c_double * 8)()
+        check(load().cpm_stats(self.handle, out))
+        keys = ("launches", "kernel_ms", "bytes", "candidates", "gram_launches", "gram_ms",
+                "gram_ops", "levels")
+        return dict(zip(keys, list(out)))
+
+    def reset_stats(self):
+        check(load().cpm_stats_reset(self.handle))
+
+    def set_timing(self, on):
+        check(load().cpm_set_timing(self.handle, 1 if on else 0))
+
+    # ---- store -------------------------------------------------------------
+    def build_bits(self, rows, cols, n_transactions, n_items):
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        cols = np.ascontiguousarray(cols, dtype=np.uint32)
+        handle = _vp()
+        check(load().cpm_bits_build(self.handle, _p(rows, _u64p), _p(cols, _u32p), rows.size,
+                                    int(n_transactions), int(n_items), ctypes.byref(handle)))
+        return Bits(self, handle)
+
+    def build_bits_device(self, d_rows, d_cols, nnz, n_transactions, n_items):
+        """rows/cols given as device addresses (ints)."""
+        handle = _vp()
+        check(load().cpm_bits_build_device(self.handle, _vp(d_rows), _vp(d_cols), int(nnz),
+                                           int(n_transactions), int(n_items),
+                                           ctypes.byref(handle)))
+        return Bits(self, handle)
+
+    # ---- operators -----------------------------------------------------------
+    def count_supports(self, bits, candidates):
+        cand = np.ascontiguousarray(candidates, dtype=np.uint32)
+        if cand.ndim != 2:
+            raise ValueError("candidates must be a 2-D array of item columns")
+        out = np.zeros(cand.shape[0], dtype=np.uint64)
+        if cand.shape[0]:
+            check(load().cpm_count_supports(self.handle, bits.handle, _p(cand, _u32p),
+                                            cand.shape[0], cand.shape[1], _p(out, _u64p)))
+        return out
+
+    def intersect_extensions(self, prefix, extensions, keep_min=0, want_rows=True):
+        prefix = np.ascontiguousarray(prefix, dtype=np.uint64)
+        ext = np.ascontiguousarray(extensions, dtype=np.uint64)
+        k = ext.shape[0]
+        words = prefix.shape[0]
+        if ext.ndim != 2 or ext.shape[1] != words:
+            raise ValueError("extensions must be (k, words) with the prefix's word count")
+        sup = np.zeros(k, dtype=np.uint64)
+        rows = np.zeros((k, words), dtype=np.uint64) if want_rows else None
+        idx = np.zeros(max(k, 1), dtype=np.uint32)
+        nk = ctypes.c_uint32(0)
+        if k:
+            check(load().cpm_intersect_extensions(
+                self.handle, _p(prefix, _u64p), _p(ext, _u64p), k, words, int(keep_min),
+                _p(sup, _u64p), _p(rows, _u64p) if want_rows else None, _p(idx, _u32p),
+                ctypes.byref(nk)))
+        n = nk.value
+        return sup, (rows[:n] if want_rows else None), idx[:n]
+
+    def pair_gram(self, bits, cols):
+        cols = np.ascontiguousarray(cols, dtype=np.uint32)
+        n = cols.size
+        out = np.zeros((n, n), dtype=np.uint32)
+        if n:
+            check(load().cpm_pair_gram_i8(self.handle, bits.handle, _p(cols, _u32p), n,
+                                          _p(out, _u32p)))
+        return out
+
+    def periodic_eval(self, bits, timestamps, end, candidates):
+        cand = np.ascontiguousarray(candidates, dtype=np.uint32)
+        ts = np.ascontiguousarray(timestamps, dtype=np.int64)
+        if ts.size == 0:
+            ts = np.zeros(1, dtype=np.int64)
+        sup = np.zeros(cand.shape[0], dtype=np.uint64)
+        per = np.zeros(cand.shape[0], dtype=np.int64)
+        if cand.shape[0]:
+            check(load().cpm_periodic_eval(self.handle, bits.handle, _p(ts, _i64p), int(end),
+                                           _p(cand, _u32p), cand.shape[0], cand.shape[1],
+                                           _p(sup, _u64p), _p(per, _i64p)))
+        return sup, per
+
+    # ---- searches -------------------------------------------------------------
+    def eclat_mine(self, bits, minsup, use_gram=True):
+        handle = _vp()
+        check(load().cpm_eclat_mine(self.handle, bits.handle, clamp_u64(minsup),
+                                    1 if use_gram else 0, ctypes.byref(handle)))
+        return Patterns(handle)
+
+    def pfeclat_mine(self, bits, timestamps, end, minsup, maxper_floor):
+        ts = np.ascontiguousarray(timestamps, dtype=np.int64)
+        if ts.size == 0:
+            ts = np.zeros(1, dtype=np.int64)
+        handle = _vp()
+        check(load().cpm_pfeclat_mine(self.handle, bits.handle, _p(ts, _i64p), int(end),
+                                      clamp_u64(minsup), int(maxper_floor), ctypes.byref(handle)))
+        return Patterns(handle, periodic=True)
+
+
+def clamp_u64(value):
+    value = int(value)
+    return 0 if value < 0 else min(value, 2**64 - 1)
+
+
+class Bits:
+    """Device vertical store (``cpm_bits``)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    def close(self):
+        if self.handle:
+            load().cpm_bits_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self):
+        n = ctypes.c_uint64()
+        i = ctypes.c_uint32()
+        w = ctypes.c_uint64()
+        s = ctypes.c_uint64()
+        check(load().cpm_bits_shape(self.handle, ctypes.byref(n), ctypes.byref(i),
+                                    ctypes.byref(w), ctypes.byref(s)))
+        return n.value, i.value, w.value, s.value
+
+    def download(self):
+        _, items, words, _ = self.shape()
+        out = np.zeros((items, words), dtype=np.uint64)
+        if out.size:
+            check(load().cpm_bits_download(self.ctx.handle, self.handle, _p(out, _u64p)))
+        return out
+
+
+class Patterns:
+    """Pattern forest returned by a search (``cpm_patterns``)."""
+
+    def __init__(self, handle, periodic=False):
+        lib = load()
+        n = ctypes.c_uint64()
+        try:
+            check(lib.cpm_patterns_count(handle, ctypes.byref(n)))
+            self.n = n.value
+            self.parent = np.zeros(self.n, dtype=np.uint32)
+            self.item = np.zeros(self.n, dtype=np.uint32)
+            self.support = np.zeros(self.n, dtype=np.uint64)
+            self.period = np.zeros(self.n, dtype=np.int64) if periodic else None
+            if self.n:
+                check(lib.cpm_patterns_download(
+                    handle, _p(self.parent, _u32p), _p(self.item, _u32p),
+                    _p(self.support, _u64p), _p(self.period, _i64p) if periodic else None))
+        finally:
+            lib.cpm_patterns_destroy(handle)
+        self.periodic = periodic
+
+    def __len__(self):
+        return self.n
+
+    def itemsets(self):
+        """Column tuples, one per pattern (parents precede children)."""
+        out = [None] * self.n
+        parent = self.parent.tolist()
+        item = self.item.tolist()
+        for i in range(self.n):
+            p = parent[i]
+            out[i] = (item[i],) if p == NO_PARENT else out[p] + (item[i],)
+        return out
+
+    def to_dict(self, labels=None):
+        """``{itemset: support}`` (or ``(support, period)``), keyed by column tuples
+        or, with ``labels``, by label tuples."""
+        sets = self.itemsets()
+        if labels is not None:
+            lab = list(labels)
+            sets = [tuple(lab[c] for c in s) for s in sets]
+        sup = self.support.tolist()
+        if self.periodic:
+            per = self.period.tolist()
+            return {s: (int(a), int(b)) for s, a, b in zip(sets, sup, per)}
+        return {s: int(a) for s, a in zip(sets, sup)}
+
+
+_contexts = {}
+_ctx_lock = threading.Lock()
+
+
+def device_count():
+    try:
+        return load().cpm_device_count()
+    except ImportError:
+        return 0
+
+
+def context(device=0):
+    """Shared per-device context (created on first use)."""
+    with _ctx_lock:
+        ctx = _contexts.get(device)
+        if ctx is None:
+            ctx = Context(device)
+            _contexts[device] = ctx
+        return ctx

@@ -1,0 +1,480 @@
+// Context, errors, device buffers, the vertical store and the operator-level
+// entry points (count_supports, intersect_extensions, periodic_eval).
+#include "cpm_internal.h"
+#include "support_kernels.h"
+
+#include <string.h>
+#include <algorithm>
+
+namespace cpm {
+
+static thread_local std::string g_error;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+void clear_error() { g_error.clear(); }
+
+int DBuf::ensure(size_t bytes) {
+    if (bytes <= cap && p) return CPM_OK;
+    release();
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        cap = 0;
+        (void)hipGetLastError();
+        return fail(CPM_ERR_NOMEM, "device allocation of %zu bytes failed: %s", want,
+                    hipGetErrorString(e));
+    }
+    cap = want;
+    return CPM_OK;
+}
+
+void DBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+int HBuf::ensure(size_t bytes) {
+    if (bytes <= cap && p) return CPM_OK;
+    release();
+    size_t want = std::max<size_t>(bytes, 4096);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        p = nullptr;
+        cap = 0;
+        (void)hipGetLastError();
+        return fail(CPM_ERR_NOMEM, "pinned host allocation of %zu bytes failed: %s", want,
+                    hipGetErrorString(e));
+    }
+    cap = want;
+    return CPM_OK;
+}
+
+void HBuf::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+int bind(cpm_ctx *ctx) {
+    if (!ctx) return fail(CPM_ERR_INVALID, "null context");
+    CPM_HIP(hipSetDevice(ctx->device));
+    return CPM_OK;
+}
+
+// ---- store build ------------------------------------------------------------
+
+// One lane per COO pair: OR the bit into its 64-bit word.  Pairs that fall
+// outside the store raise *bad instead of writing.
+__global__ __launch_bounds__(256) void k_bits_scatter(const uint64_t *__restrict__ rows,
+                                                      const uint32_t *__restrict__ cols,
+                                                      uint64_t nnz, uint64_t n_trans,
+                                                      uint32_t n_items, uint64_t stride,
+                                                      unsigned long long *store,
+                                                      unsigned int *bad) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < nnz; i += step) {
+        uint64_t r = rows[i];
+        uint32_t c = cols[i];
+        if (r >= n_trans || c >= n_items) {
+            atomicOr(bad, 1u);
+            continue;
+        }
+        atomicOr(&store[(uint64_t)c * stride + (r >> 6)], 1ull << (r & 63));
+    }
+}
+
+}  // namespace cpm
+
+using namespace cpm;
+
+extern "C" {
+
+int cpm_version(void) { return 10000; }
+
+const char *cpm_last_error(void) { return cpm::g_error.c_str(); }
+
+int cpm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int cpm_ctx_create(int device, cpm_ctx **out) {
+    clear_error();
+    if (!out) return fail(CPM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int n = cpm_device_count();
+    if (n <= 0) return fail(CPM_ERR_NODEVICE, "no HIP device was found");
+    if (device < 0 || device >= n)
+        return fail(CPM_ERR_INVALID, "device %d out of range (found %d)", device, n);
+    CPM_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    CPM_HIP(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(CPM_ERR_NODEVICE, "device %d is %s; this library is built for gfx950", device,
+                    prop.gcnArchName);
+    cpm_ctx *ctx = new cpm_ctx();
+    ctx->device = device;
+    ctx->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev_stop);
+    if (e != hipSuccess) {
+        cpm_ctx_destroy(ctx);
+        return fail(CPM_ERR_RUNTIME, "stream/event creation failed: %s", hipGetErrorString(e));
+    }
+    *out = ctx;
+    return CPM_OK;
+}
+
+int cpm_ctx_destroy(cpm_ctx *ctx) {
+    if (!ctx) return CPM_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &b : ctx->scratch) b.release();
+    ctx->host_scratch.release();
+    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CPM_OK;
+}
+
+int cpm_ctx_synchronize(cpm_ctx *ctx) {
+    CPM_CHECK(bind(ctx));
+    CPM_HIP(hipStreamSynchronize(ctx->stream));
+    return CPM_OK;
+}
+
+void *cpm_ctx_stream(cpm_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+static int bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits **out) {
+    cpm_bits *b = new cpm_bits();
+    b->ctx = ctx;
+    b->n_transactions = n_trans;
+    b->n_items = n_items;
+    b->words = (n_trans + 63) / 64;
+    b->stride = row_stride(b->words);
+    size_t bytes = (size_t)n_items * b->stride * sizeof(uint64_t);
+    int s = b->store.ensure(bytes ? bytes : 8);
+    if (s != CPM_OK) {
+        delete b;
+        return s;
+    }
+    hipError_t e = hipMemsetAsync(b->store.p, 0, bytes ? bytes : 8, ctx->stream);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(CPM_ERR_RUNTIME, "hipMemsetAsync failed: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return CPM_OK;
+}
+
+int cpm_bits_build_device(cpm_ctx *ctx, const uint64_t *d_rows, const uint32_t *d_cols,
+                          uint64_t nnz, uint64_t n_trans, uint32_t n_items, cpm_bits **out) {
+    clear_error();
+    CPM_CHECK(bind(ctx));
+    if (!out) return fail(CPM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (nnz && (!d_rows || !d_cols)) return fail(CPM_ERR_INVALID, "rows/cols are NULL");
+    if (n_trans >= (1ull << 40)) return fail(CPM_ERR_INVALID, "too many transactions");
+    cpm_bits *b = nullptr;
+    CPM_CHECK(bits_alloc(ctx, n_trans, n_items, &b));
+    if (nnz) {
+        int s = ctx->scratch[5].ensure(sizeof(unsigned int));
+        if (s != CPM_OK) { cpm_bits_destroy(b); return s; }
+        unsigned int *bad = ctx->scratch[5].as<unsigned int>();
+        hipError_t e = hipMemsetAsync(bad, 0, sizeof(unsigned int), ctx->stream);
+        if (e == hipSuccess) {
+            uint64_t blocks = std::min<uint64_t>((nnz + 255) / 256, (uint64_t)ctx->cu_count * 32);
+            hipLaunchKernelGGL(k_bits_scatter, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                               d_rows, d_cols, nnz, n_trans, n_items, b->stride,
+                               b->store.as<unsigned long long>(), bad);
+            e = hipGetLastError();
+        }
+        unsigned int hbad = 0;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            cpm_bits_destroy(b);
+            return fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(e));
+        }
+        if (hbad) {
+            cpm_bits_destroy(b);
+            return fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
+                        (unsigned long long)n_trans, n_items);
+        }
+    }
+    *out = b;
+    return CPM_OK;
+}
+
+int cpm_bits_build(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, uint64_t nnz,
+                   uint64_t n_trans, uint32_t n_items, cpm_bits **out) {
+    clear_error();
+    CPM_CHECK(bind(ctx));
+    if (nnz && (!rows || !cols)) return fail(CPM_ERR_INVALID, "rows/cols are NULL");
+    DBuf &dr = ctx->scratch[0], &dc = ctx->scratch[1];
+    CPM_CHECK(dr.ensure(nnz * sizeof(uint64_t)));
+    CPM_CHECK(dc.ensure(nnz * sizeof(uint32_t)));
+    if (nnz) {
+        CPM_HIP(hipMemcpyAsync(dr.p, rows, nnz * sizeof(uint64_t), hipMemcpyHostToDevice,
+                               ctx->stream));
+        CPM_HIP(hipMemcpyAsync(dc.p, cols, nnz * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               ctx->stream));
+    }
+    return cpm_bits_build_device(ctx, dr.as<uint64_t>(), dc.as<uint32_t>(), nnz, n_trans,
+                                 n_items, out);
+}
+
+int cpm_bits_destroy(cpm_bits *b) {
+    if (!b) return CPM_OK;
+    if (b->ctx) {
+        (void)hipSetDevice(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+    }
+    delete b;
+    return CPM_OK;
+}
+
+int cpm_bits_shape(const cpm_bits *b, uint64_t *n_trans, uint32_t *n_items, uint64_t *words,
+                   uint64_t *stride) {
+    if (!b) return fail(CPM_ERR_INVALID, "null store");
+    if (n_trans) *n_trans = b->n_transactions;
+    if (n_items) *n_items = b->n_items;
+    if (words) *words = b->words;
+    if (stride) *stride = b->stride;
+    return CPM_OK;
+}
+
+int cpm_bits_download(cpm_ctx *ctx, const cpm_bits *b, uint64_t *host) {
+    clear_error();
+    CPM_CHECK(bind(ctx));
+    if (!b || !host) return fail(CPM_ERR_INVALID, "null argument");
+    if (b->words && b->n_items)
+        CPM_HIP(hipMemcpy2DAsync(host, b->words * 8, b->store.p, b->stride * 8, b->words * 8,
+                                 b->n_items, hipMemcpyDeviceToHost, ctx->stream));
+    CPM_HIP(hipStreamSynchronize(ctx->stream));
+    return CPM_OK;
+}
+
+// ---- operators ----------------------------------------------------------------
+
+int cpm_count_supports(cpm_ctx *ctx, const cpm_bits *b, const uint32_t *cand, uint64_t n_cand,
+                       uint32_t width, uint64_t *supports) {
+    clear_error();
+    CPM_CHECK(bind(ctx));
+    if (!b) return fail(CPM_ERR_INVALID, "null store");
+    if (n_cand == 0) return CPM_OK;
+    if (!cand || !supports) return fail(CPM_ERR_INVALID, "null candidates/supports");
+    if (width == 0) return fail(CPM_ERR_INVALID, "candidate width must be >= 1");
+    if (n_cand >= (1ull << 31)) return fail(CPM_ERR_INVALID, "too many candidates in one call");
+    for (uint64_t i = 0; i < n_cand * width; ++i)
+        if (cand[i] >= b->n_items)
+            return fail(CPM_ERR_INVALID, "candidate column %u >= %u items", cand[i], b->n_items);
+    // One task per candidate: prefix = its first width-1 columns, one extension.
+    const uint32_t d = width - 1;
+    std::vector<uint32_t> prefix((size_t)n_cand * (d ? d : 1)), off(n_cand + 1), ext(n_cand);
+    for (uint64_t c = 0; c < n_cand; ++c) {
+        for (uint32_t j = 0; j < d; ++j) prefix[c * d + j] = cand[c * width + j];
+        ext[c] = cand[c * width + d];
+        off[c] = (uint32_t)c;
+    }
+    off[n_cand] = (uint32_t)n_cand;
+    DBuf &dp = ctx->scratch[0], &doff = ctx->scratch[1], &dext = ctx->scratch[2],
+         &dsup = ctx->scratch[3];
+    CPM_CHECK(dp.ensure(prefix.size() * 4));
+    CPM_CHECK(doff.ensure(off.size() * 4));
+    CPM_CHECK(dext.ensure(ext.size() * 4));
+    CPM_CHECK(dsup.ensure(n_cand * 4));
+    uint32_t nt = (uint32_t)n_cand;
+    CPM_HIP(hipMemcpyAsync(dp.p, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    CPM_HIP(hipMemcpyAsync(doff.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    CPM_HIP(hipMemcpyAsync(dext.p, ext.data(), ext.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    CPM_HIP(hipMemsetAsync(dsup.p, 0, n_cand * 4, ctx->stream));
+    TaskView tv;
+    tv.prefix = dp.as<uint32_t>();
+    tv.d = d;
+    tv.ext_off = doff.as<uint32_t>();
+    tv.ext = dext.as<uint32_t>();
+    tv.grp_off = doff.as<uint32_t>();  // one extension per task -> one group per task
+    tv.n_tasks = nt;
+    tv.fcol = nullptr;
+    tv.G = 1;
+    CPM_CHECK(launch_support_count(ctx, b, tv, n_cand, dsup.as<uint32_t>()));
+    std::vector<uint32_t> s32(n_cand);
+    CPM_HIP(hipMemcpyAsync(s32.data(), dsup.p, n_cand * 4, hipMemcpyDeviceToHost, ctx->stream));
+    CPM_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t c = 0; c < n_cand; ++c) supports[c] = s32[c];
+    return CPM_OK;
+}
+
+int cpm_intersect_extensions(cpm_ctx *ctx, const uint64_t *prefix, const uint64_t *ext, uint32_t k,
+                             uint64_t words, uint64_t keep_min, uint64_t *supports,
+                             uint64_t *kept_rows, uint32_t *kept_index, uint32_t *n_kept) {
+    clear_error();
+    CPM_CHECK(bind(ctx));
+    if (n_kept) *n_kept = 0;
+    if (k == 0) return CPM_OK;
+    if (!supports || (words && (!prefix || !ext))) return fail(CPM_ERR_INVALID, "null argument");
+    const uint64_t stride = row_stride(words);
+    if (words) {
+        DBuf &dpre = ctx->scratch[0], &dext = ctx->scratch[1], &dout = ctx->scratch[2],
+             &dsup = ctx->scratch[3];
+        CPM_CHECK(dpre.ensure(stride * 8));
+        CPM_CHECK(dext.ensure((size_t)k * stride * 8));
+        CPM_CHECK(dsup.ensure((size_t)k * 8));
+        if (kept_rows) CPM_CHECK(dout.ensure((size_t)k * stride * 8));
+        CPM_HIP(hipMemsetAsync(dpre.p, 0, stride * 8, ctx->stream));
+        CPM_HIP(hipMemcpyAsync(dpre.p, prefix, words * 8, hipMemcpyHostToDevice, ctx->stream));
+        CPM_HIP(hipMemsetAsync(dext.p, 0, (size_t)k * stride * 8, ctx->stream));
+        CPM_HIP(hipMemcpy2DAsync(dext.p, stride * 8, ext, words * 8, words * 8, k,
+                                 hipMemcpyHostToDevice, ctx->stream));
+        CPM_CHECK(launch_intersect(ctx, dpre.as<uint64_t>(), dext.as<uint64_t>(), k, stride,
+                                   dsup.as<unsigned long long>(),
+                                   kept_rows ? dout.as<uint64_t>() : nullptr));
+        CPM_HIP(hipMemcpyAsync(supports, dsup.p, (size_t)k * 8, hipMemcpyDeviceToHost,
+                               ctx->stream));
+        CPM_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+        for (uint32_t j = 0; j < k; ++j) supports[j] = 0;
+    }
+    // keep step (cuEclat.py:234-242): children at or above keep_min, in order
+    uint32_t nk = 0;
+    for (uint32_t j = 0; j < k; ++j) {
+        if (supports[j] < keep_min) continue;
+        if (kept_index) kept_index[nk] = j;
+        if (kept_rows && words)
+            CPM_HIP(hipMemcpyAsync(kept_rows + (uint64_t)nk * words,
+                                   ctx->scratch[2].as<uint64_t>() + (uint64_t)j * stride, words * 8,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        ++nk;
+    }
+    CPM_HIP(hipStreamSynchronize(ctx->stream));
+    if (n_kept) *n_kept = nk;
+    return CPM_OK;
+}
+
+int cpm_periodic_eval(cpm_ctx *ctx, const cpm_bits *b, const int64_t *timestamps, int64_t end,
+                      const uint32_t *cand, uint64_t n_cand, uint32_t width, uint64_t *supports,
+                      int64_t *periods) {
+    clear_error();
+    CPM_CHECK(bind(ctx));
+    if (!b) return fail(CPM_ERR_INVALID, "null store");
+    if (n_cand == 0) return CPM_OK;
+    if (!cand || !supports || !periods) return fail(CPM_ERR_INVALID, "null argument");
+    if (b->n_transactions && !timestamps) return fail(CPM_ERR_INVALID, "null timestamps");
+    if (width == 0) return fail(CPM_ERR_INVALID, "candidate width must be >= 1");
+    if (n_cand >= (1ull << 31)) return fail(CPM_ERR_INVALID, "too many candidates in one call");
+    for (uint64_t i = 0; i < n_cand * width; ++i)
+        if (cand[i] >= b->n_items)
+            return fail(CPM_ERR_INVALID, "candidate column %u >= %u items", cand[i], b->n_items);
+    const uint32_t d = width - 1;
+    std::vector<uint32_t> prefix((size_t)n_cand * (d ? d : 1)), off(n_cand + 1), ext(n_cand);
+    for (uint64_t c = 0; c < n_cand; ++c) {
+        for (uint32_t j = 0; j < d; ++j) prefix[c * d + j] = cand[c * width + j];
+        ext[c] = cand[c * width + d];
+        off[c] = (uint32_t)c;
+    }
+    off[n_cand] = (uint32_t)n_cand;
+    DBuf tsb, parts, per;
+    DBuf &dp = ctx->scratch[0], &doff = ctx->scratch[1], &dext = ctx->scratch[2],
+         &dsup = ctx->scratch[3];
+    const uint64_t n = b->n_transactions;
+    CPM_CHECK(tsb.ensure((n ? n : 1) * 8));
+    CPM_CHECK(dp.ensure(prefix.size() * 4));
+    CPM_CHECK(doff.ensure(off.size() * 4));
+    CPM_CHECK(dext.ensure(ext.size() * 4));
+    CPM_CHECK(dsup.ensure(n_cand * 4));
+    CPM_CHECK(per.ensure(n_cand * 8));
+    const uint32_t n_chunks = (uint32_t)((b->stride + kTileWords - 1) / kTileWords);
+    CPM_CHECK(parts.ensure((size_t)n_cand * n_chunks * sizeof(PeriodPart)));
+    uint32_t nt = (uint32_t)n_cand;
+    if (n) CPM_HIP(hipMemcpyAsync(tsb.p, timestamps, n * 8, hipMemcpyHostToDevice, ctx->stream));
+    CPM_HIP(hipMemcpyAsync(dp.p, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    CPM_HIP(hipMemcpyAsync(doff.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    CPM_HIP(hipMemcpyAsync(dext.p, ext.data(), ext.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    TaskView tv;
+    tv.prefix = dp.as<uint32_t>();
+    tv.d = d;
+    tv.ext_off = doff.as<uint32_t>();
+    tv.ext = dext.as<uint32_t>();
+    tv.grp_off = doff.as<uint32_t>();
+    tv.n_tasks = nt;
+    tv.fcol = nullptr;
+    tv.G = 1;
+    CPM_CHECK(launch_support_periodic(ctx, b, tv, n_cand, tsb.as<int64_t>(), end,
+                                      parts.as<PeriodPart>(), dsup.as<uint32_t>(),
+                                      per.as<int64_t>(), dext.as<uint32_t>() /*unused*/, 0));
+    std::vector<uint32_t> s32(n_cand);
+    CPM_HIP(hipMemcpyAsync(s32.data(), dsup.p, n_cand * 4, hipMemcpyDeviceToHost, ctx->stream));
+    CPM_HIP(hipMemcpyAsync(periods, per.p, n_cand * 8, hipMemcpyDeviceToHost, ctx->stream));
+    CPM_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t c = 0; c < n_cand; ++c) supports[c] = s32[c];
+    return CPM_OK;
+}
+
+// ---- patterns -----------------------------------------------------------------
+
+int cpm_patterns_count(const cpm_patterns *p, uint64_t *n) {
+    if (!p || !n) return fail(CPM_ERR_INVALID, "null argument");
+    *n = p->n;
+    return CPM_OK;
+}
+
+int cpm_patterns_download(const cpm_patterns *p, uint32_t *parent, uint32_t *item,
+                          uint64_t *support, int64_t *period) {
+    if (!p) return fail(CPM_ERR_INVALID, "null patterns");
+    if (p->n == 0) return CPM_OK;
+    if (parent) memcpy(parent, p->parent.data(), p->n * 4);
+    if (item) memcpy(item, p->item.data(), p->n * 4);
+    if (support) memcpy(support, p->support.data(), p->n * 8);
+    if (period) {
+        if (p->periodic) memcpy(period, p->period.data(), p->n * 8);
+        else memset(period, 0, p->n * 8);
+    }
+    return CPM_OK;
+}
+
+int cpm_patterns_destroy(cpm_patterns *p) {
+    delete p;
+    return CPM_OK;
+}
+
+// ---- measurement ----------------------------------------------------------------
+
+int cpm_stats(cpm_ctx *ctx, double *out8) {
+    if (!ctx || !out8) return fail(CPM_ERR_INVALID, "null argument");
+    memcpy(out8, ctx->stats, sizeof(ctx->stats));
+    return CPM_OK;
+}
+
+int cpm_stats_reset(cpm_ctx *ctx) {
+    if (!ctx) return fail(CPM_ERR_INVALID, "null context");
+    memset(ctx->stats, 0, sizeof(ctx->stats));
+    return CPM_OK;
+}
+
+int cpm_set_timing(cpm_ctx *ctx, int on) {
+    if (!ctx) return fail(CPM_ERR_INVALID, "null context");
+    ctx->timing = on != 0;
+    return CPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+// Internal definitions shared by the libcupami_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string>
+#include <vector>
+
+#include "../../include/cupami_hip.h"
+
+namespace cpm {
+
+// ---- errors -----------------------------------------------------------------
+
+int fail(int code, const char *fmt, ...);
+void clear_error();
+
+#define CPM_HIP(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return ::cpm::fail(e_ == hipErrorOutOfMemory ? CPM_ERR_NOMEM : CPM_ERR_RUNTIME, \
+                               "%s failed: %s", #expr, hipGetErrorString(e_));          \
+    } while (0)
+
+#define CPM_CHECK(expr)                \
+    do {                               \
+        int s_ = (expr);               \
+        if (s_ != CPM_OK) return s_;   \
+    } while (0)
+
+// ---- device buffers -----------------------------------------------------------
+
+// Grow-only device allocation owned by a context (or a handle).
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes);       // keeps contents only if no growth is needed
+    void release();
+    ~DBuf() { release(); }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Pinned host staging buffer.
+struct HBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes);
+    void release();
+    ~HBuf() { release(); }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Row stride of the vertical store, in 64-bit words: rows are padded with
+// zero words to a multiple of 64 words (512 bytes).
+inline uint64_t row_stride(uint64_t words) { return (words + 63) / 64 * 64; }
+
+// Words handled by one workgroup of the support kernels (256 lanes x 8 words).
+constexpr uint32_t kTileWords = 2048;
+constexpr uint32_t kBlock = 256;
+
+}  // namespace cpm
+
+struct cpm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 256;
+    bool timing = true;
+    double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    // scratch used by the operator entry points
+    cpm::DBuf scratch[6];
+    cpm::HBuf host_scratch;
+};
+
+struct cpm_bits {
+    cpm_ctx *ctx = nullptr;
+    uint64_t n_transactions = 0;
+    uint32_t n_items = 0;
+    uint64_t words = 0;    // ceil(N/64)
+    uint64_t stride = 0;   // padded words per row
+    cpm::DBuf store;       // n_items * stride uint64
+};
+
+struct cpm_patterns {
+    uint64_t n = 0;
+    bool periodic = false;
+    std::vector<uint32_t> parent, item;
+    std::vector<uint64_t> support;
+    std::vector<int64_t> period;
+};
+
+namespace cpm {
+
+// Make the context's device current for the calling thread.
+int bind(cpm_ctx *ctx);
+
+// Event-timed launch bracket: accumulates milliseconds into stats[slot].
+struct Timed {
+    cpm_ctx *ctx;
+    int slot;
+    bool on;
+    Timed(cpm_ctx *c, int s) : ctx(c), slot(s), on(c->timing) {
+        if (on) (void)hipEventRecord(ctx->ev_start, ctx->stream);
+    }
+    int finish() {
+        if (!on) return CPM_OK;
+        CPM_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+        CPM_HIP(hipEventSynchronize(ctx->ev_stop));
+        float ms = 0.f;
+        CPM_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+        ctx->stats[slot] += ms;
+        return CPM_OK;
+    }
+};
+
+// ---- launchers implemented in the kernel translation units --------------------
+
+struct TaskLevel;  // miner.hip
+
+// gram.hip: out[a*ld + b] = |row(cols[a]) & row(cols[b])| for a <= b (int32),
+// computed for the upper block triangle; cols are device store columns.
+int launch_gram_i8(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t *d_cols, uint32_t n,
+                   int32_t *d_out, uint32_t ld);
+
+}  // namespace cpm

@@ -1,0 +1,127 @@
+"""Seeded synthetic transaction databases for the support-counting workloads.
+
+Every generator draws from ``numpy.random.Generator(PCG64(seed))`` and returns
+a COO pair list: ``rows`` (uint64 transaction index) and ``items`` (uint32 item
+id, 1-based like the reference generators, ``TransactionalDatabase.py:104``).
+
+Item popularity is Zipf over a seeded permutation of the ids
+(``p(rank r) ~ r**-s``; ``s = 0`` is uniform).  A basket of length ``L`` is a
+weighted sample *without replacement*: items are drawn one at a time from the
+popularity distribution restricted to the items not yet chosen.  That is done
+by rejection (redraw on a repeat), which has exactly the same law as successive
+sampling (Plackett-Luce / Efraimidis-Spirakis), without touching all items per
+basket.  ``L = max(1, Poisson(lam))``, capped at the item count.
+"""
+
+from __future__ import annotations
+
+import hashlib
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class Database:
+    rows: np.ndarray          # uint64[nnz], transaction index, non-decreasing
+    items: np.ndarray         # uint32[nnz], item id in 1..n_items
+    n_transactions: int
+    n_items: int
+    timestamps: np.ndarray | None = None   # int64[n_transactions], temporal only
+
+    def sha256(self) -> str:
+        h = hashlib.sha256()
+        h.update(np.ascontiguousarray(self.rows, dtype="<u8").tobytes())
+        h.update(np.ascontiguousarray(self.items, dtype="<u4").tobytes())
+        if self.timestamps is not None:
+            h.update(np.ascontiguousarray(self.timestamps, dtype="<i8").tobytes())
+        return h.hexdigest()
+
+    def baskets(self):
+        """Python baskets (lists of int ids), for small databases only."""
+        out = [[] for _ in range(self.n_transactions)]
+        for r, i in zip(self.rows.tolist(), self.items.tolist()):
+            out[r].append(i)
+        return out
+
+
+def _sample_baskets(rng, n, n_items, lam, zipf_s, chunk=200_000):
+    ranks = np.arange(1, n_items + 1, dtype=np.float64)
+    weights = ranks ** (-float(zipf_s))
+    cdf = np.cumsum(weights / weights.sum())
+    perm = rng.permutation(n_items).astype(np.uint32) + 1      # rank -> item id
+    lengths = np.minimum(np.maximum(1, rng.poisson(lam, n)), n_items).astype(np.int64)
+    row_parts, item_parts = [], []
+    for start in range(0, n, chunk):
+        stop = min(n, start + chunk)
+        m = stop - start
+        need = lengths[start:stop]
+        width = int(need.max())
+        chosen = np.zeros((m, n_items), dtype=bool)
+        out = np.zeros((m, width), dtype=np.int64)
+        count = np.zeros(m, dtype=np.int64)
+        active = np.arange(m)
+        while active.size:
+            draw = np.searchsorted(cdf, rng.random(active.size), side="right")
+            np.minimum(draw, n_items - 1, out=draw)
+            fresh = ~chosen[active, draw]
+            a, d = active[fresh], draw[fresh]
+            chosen[a, d] = True
+            out[a, count[a]] = d
+            count[a] += 1
+            active = active[count[active] < need[active]]
+        mask = np.arange(width)[None, :] < need[:, None]
+        rr = np.broadcast_to(np.arange(start, stop, dtype=np.uint64)[:, None], (m, width))
+        row_parts.append(rr[mask])
+        item_parts.append(perm[out[mask]])
+    rows = np.concatenate(row_parts) if row_parts else np.zeros(0, np.uint64)
+    items = np.concatenate(item_parts) if item_parts else np.zeros(0, np.uint32)
+    return rows.astype(np.uint64), items.astype(np.uint32)
+
+
+def transactional(n, n_items, lam, zipf_s, seed) -> Database:
+    """Binary baskets: ``n`` transactions, Poisson(lam) unique items each."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rows, items = _sample_baskets(rng, n, n_items, lam, zipf_s)
+    return Database(rows, items, n, n_items)
+
+
+def temporal(n, n_items, lam, zipf_s, seed, gap_low=1, gap_high=3) -> Database:
+    """Temporal baskets with strictly increasing timestamps (gaps uniform in
+    ``gap_low..gap_high``)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ts = np.cumsum(rng.integers(gap_low, gap_high + 1, n, dtype=np.int64))
+    rows, items = _sample_baskets(rng, n, n_items, lam, zipf_s)
+    return Database(rows, items, n, n_items, ts.astype(np.int64))
+
+
+# Named workloads, with the seeds the measurement plan fixes.
+def small_uniform(seed=101):
+    """10,000 baskets over 50 items, Poisson(6), uniform popularity."""
+    return transactional(10_000, 50, 6.0, 0.0, seed)
+
+
+def eclat_1m_1k(seed=202):
+    """1,000,000 baskets over 1,000 items, Poisson(10), Zipf s=1.1."""
+    return transactional(1_000_000, 1000, 10.0, 1.1, seed)
+
+
+def temporal_2m_500(seed=404):
+    """2,000,000 timestamped baskets over 500 items, Poisson(8), Zipf s=1.05,
+    timestamp gaps uniform in 1..3."""
+    return temporal(2_000_000, 500, 8.0, 1.05, seed)
+
+
+def label_columns(item_ids: np.ndarray, n_items_hint=None):
+    """Map numeric item ids to column indexes in *string* label order.
+
+    The reference orders item labels as strings (``cuApriori.py:397``), so
+    ``"10" < "2"``.  Returns ``(labels, columns)`` with ``labels`` the sorted
+    distinct label strings and ``columns`` uint32 indexes into it.
+    """
+    unique = np.unique(item_ids)
+    labels = sorted(str(int(v)) for v in unique)
+    index = {lab: i for i, lab in enumerate(labels)}
+    remap = np.array([index[str(int(v))] for v in unique], dtype=np.uint32)
+    columns = remap[np.searchsorted(unique, item_ids)]
+    return labels, columns.astype(np.uint32)

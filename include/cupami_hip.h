@@ -1,0 +1,150 @@
+/*
+ * cupami_hip.h -- C ABI of libcupami_hip.so, the gfx950 support-counting path.
+ *
+ * Plain C types only.  Every function returns an int status (CPM_OK == 0,
+ * negative on error); cpm_last_error() returns the calling thread's message
+ * for the last failure.  Device memory belongs to the handle that allocated
+ * it and is released by the matching *_destroy.  Host output buffers belong
+ * to the caller.  A context is bound to one device and one HIP stream; use a
+ * context from one host thread at a time.
+ *
+ * Items are addressed by COLUMN: the index of the item's label in string
+ * order (the reference sorts labels as strings, cuApriori.py:397).  Rows are
+ * transaction indexes 0..n_transactions-1.
+ */
+#ifndef CUPAMI_HIP_H
+#define CUPAMI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes; the Python layer maps them to the reference's exception types */
+#define CPM_OK              0
+#define CPM_ERR_INVALID    -1   /* ValueError: bad argument */
+#define CPM_ERR_RUNTIME    -2   /* RuntimeError: HIP call failed */
+#define CPM_ERR_NOMEM      -3   /* MemoryError: device allocation failed */
+#define CPM_ERR_NODEVICE   -4   /* RuntimeError: no usable gfx950 device */
+
+typedef struct cpm_ctx cpm_ctx;
+typedef struct cpm_bits cpm_bits;
+typedef struct cpm_patterns cpm_patterns;
+
+/* ABI version (major*10000 + minor*100 + patch). */
+int cpm_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char *cpm_last_error(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int cpm_device_count(void);
+
+/* Replaces cuApriori._getCudaModule (cuApriori.py:458-471): binds a device,
+ * creates its stream.  Fails with CPM_ERR_NODEVICE when none is present. */
+int cpm_ctx_create(int device, cpm_ctx **out);
+int cpm_ctx_destroy(cpm_ctx *ctx);
+/* Block until all work queued on the context's stream is done. */
+int cpm_ctx_synchronize(cpm_ctx *ctx);
+/* The context's hipStream_t, as an opaque pointer. */
+void *cpm_ctx_stream(cpm_ctx *ctx);
+
+/* ---- vertical bitset store ------------------------------------------- */
+
+/* Replaces _bitsets.build_bitsets (_bitsets.py:91-100): sets bit `rows[i]` of
+ * item `cols[i]` for every pair (duplicates allowed).  The store holds one
+ * row of ceil(n_transactions/64) uint64 words per item; bits past
+ * n_transactions are zero.  Host pointers. */
+int cpm_bits_build(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, uint64_t nnz,
+                   uint64_t n_transactions, uint32_t n_items, cpm_bits **out);
+/* Same, with rows/cols already in device memory (no host copies). */
+int cpm_bits_build_device(cpm_ctx *ctx, const uint64_t *d_rows, const uint32_t *d_cols,
+                          uint64_t nnz, uint64_t n_transactions, uint32_t n_items,
+                          cpm_bits **out);
+int cpm_bits_destroy(cpm_bits *bits);
+/* words per row (ceil(N/64)), and the padded row stride in words */
+int cpm_bits_shape(const cpm_bits *bits, uint64_t *n_transactions, uint32_t *n_items,
+                   uint64_t *words, uint64_t *stride);
+/* Copy the store to host: n_items rows of `words` uint64 each (no padding). */
+int cpm_bits_download(cpm_ctx *ctx, const cpm_bits *bits, uint64_t *host);
+
+/* ---- support operators ------------------------------------------------ */
+
+/* Replaces _bitsets.count_supports (_bitsets.py:103-113): for each of the
+ * n_cand candidates (width columns each, row-major in `cand`), the popcount
+ * of the AND of its item rows.  Host pointers. */
+int cpm_count_supports(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t *cand,
+                       uint64_t n_cand, uint32_t width, uint64_t *supports);
+
+/* Replaces _bitsets.intersect_extensions (_bitsets.py:116-125) with the keep
+ * step cuEclat applies to its result (cuEclat.py:234-242) fused in:
+ * child_j = prefix & ext_j for k extension rows of `words` uint64 each
+ * (host arrays), supports[j] = popcount(child_j).  Children whose support is
+ * >= keep_min are written, in extension order, to `kept_rows` (host,
+ * up to k x words; NULL to count only) and their indexes to `kept_index`
+ * (NULL allowed); *n_kept receives their count (NULL allowed).
+ * keep_min = 0 keeps every child, as the reference returns them all. */
+int cpm_intersect_extensions(cpm_ctx *ctx, const uint64_t *prefix, const uint64_t *ext,
+                             uint32_t k, uint64_t words, uint64_t keep_min,
+                             uint64_t *supports, uint64_t *kept_rows, uint32_t *kept_index,
+                             uint32_t *n_kept);
+
+/* Level-2 pair supports as an INT8 D^T D on MFMA (exact int32 accumulation):
+ * gram[a*n + b] = |rows(cols[a]) & rows(cols[b])| for a <= b; entries below
+ * the diagonal are left 0.  n <= 16384, n_transactions < 2^31.  Host output
+ * of n*n uint32. */
+int cpm_pair_gram_i8(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t *cols, uint32_t n,
+                     uint32_t *gram);
+
+/* Replaces TemporalInput._evaluate's periodic_counts (_temporal.py:28-55,
+ * 151-161): support and periodicity (largest gap of [0, ts of each
+ * occurrence in row order, end]) of each candidate.  `timestamps` (host,
+ * n_transactions int64, non-decreasing) is uploaded once per call. */
+int cpm_periodic_eval(cpm_ctx *ctx, const cpm_bits *bits, const int64_t *timestamps,
+                      int64_t end, const uint32_t *cand, uint64_t n_cand, uint32_t width,
+                      uint64_t *supports, int64_t *periods);
+
+/* ---- whole-search miners (device-resident search) --------------------- */
+
+/* Replaces cuEclat.mine / cuApriori.mine's search (cuEclat.py:168-245,
+ * cuApriori.py:501-535): every itemset whose support >= minsup, found by a
+ * level-synchronous search whose candidate generation, pruning and support
+ * counting all run on the device.  Level-2 supports come from
+ * cpm_pair_gram_i8 when use_gram != 0. */
+int cpm_eclat_mine(cpm_ctx *ctx, const cpm_bits *bits, uint64_t minsup, int use_gram,
+                   cpm_patterns **out);
+
+/* Replaces cuPFEclat.mine (cuPFEclat.py:146-175): every itemset with
+ * support >= minsup, support > 0 and periodicity <= maxper_floor (an integer
+ * period p satisfies p <= m for a real m >= 0 iff p <= floor(m)).
+ * timestamps: host, n_transactions non-decreasing int64. */
+int cpm_pfeclat_mine(cpm_ctx *ctx, const cpm_bits *bits, const int64_t *timestamps, int64_t end,
+                     uint64_t minsup, int64_t maxper_floor, cpm_patterns **out);
+
+/* ---- pattern results --------------------------------------------------- */
+
+/* Patterns are a forest: pattern i = pattern parent[i] plus column item[i]
+ * (parent == UINT32_MAX for singletons).  Parents always precede children. */
+int cpm_patterns_count(const cpm_patterns *p, uint64_t *n);
+/* Copy the forest to host arrays of length count (periods may be NULL; it is
+ * filled only for periodic results). */
+int cpm_patterns_download(const cpm_patterns *p, uint32_t *parent, uint32_t *item,
+                          uint64_t *support, int64_t *period);
+int cpm_patterns_destroy(cpm_patterns *p);
+
+/* ---- measurement -------------------------------------------------------- */
+
+/* Totals since the last cpm_stats_reset on this context, from HIP events on
+ * its stream around each launch of the support kernel:
+ * [0] launches, [1] kernel milliseconds, [2] algorithmic bitset bytes
+ * (8 * words per row read), [3] candidates counted, [4] gram launches,
+ * [5] gram milliseconds, [6] gram int8 ops (2 per multiply-add),
+ * [7] search levels. */
+int cpm_stats(cpm_ctx *ctx, double *out8);
+int cpm_stats_reset(cpm_ctx *ctx);
+/* Enable (1) / disable (0) per-launch event timing (default on). */
+int cpm_set_timing(cpm_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

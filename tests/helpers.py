@@ -1,0 +1,51 @@
+"""Shared test helpers: stores and label maps built the reference's way."""
+
+import json
+from pathlib import Path
+
+import numpy as np
+
+import oracle
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def kats():
+    return json.loads((GOLDEN / "reference_kats.json").read_text())
+
+
+def coo(rows):
+    """Clean baskets like the reference, drop empty ones, order labels as
+    strings (cuApriori.py:397-404); returns (labels, rows, cols, n)."""
+    baskets = [oracle.clean_basket(r) for r in rows]
+    baskets = [b for b in baskets if b]
+    labels = sorted({i for b in baskets for i in b})
+    index = {lab: i for i, lab in enumerate(labels)}
+    rr, cc = [], []
+    for r, b in enumerate(baskets):
+        for item in b:
+            rr.append(r)
+            cc.append(index[item])
+    return labels, np.asarray(rr, np.uint64), np.asarray(cc, np.uint32), len(baskets)
+
+
+def temporal_coo(rows, end=None):
+    """Temporal rows [ts, *items] -> (labels, rows, cols, n, ts, end); empty
+    baskets are kept and rows are stably sorted by timestamp."""
+    recs = [(int(r[0]), oracle.clean_basket(r[1:])) for r in rows if len(r)]
+    recs.sort(key=lambda x: x[0])
+    labels = sorted({i for _, b in recs for i in b})
+    index = {lab: i for i, lab in enumerate(labels)}
+    rr, cc = [], []
+    for r, (_, b) in enumerate(recs):
+        for item in b:
+            rr.append(r)
+            cc.append(index[item])
+    ts = np.asarray([t for t, _ in recs], np.int64)
+    last = int(ts[-1]) if len(ts) else 0
+    return (labels, np.asarray(rr, np.uint64), np.asarray(cc, np.uint32), len(recs), ts,
+            last if end is None else end)
+
+
+def random_rows(rng, n, n_items, max_len):
+    return [rng.sample(range(n_items), rng.randrange(max_len + 1)) for _ in range(n)]

@@ -1,0 +1,137 @@
+"""Operator parity: the HIP entry points against the CPU oracle, bit-exact."""
+
+import random
+from itertools import combinations
+
+import numpy as np
+import pytest
+
+import oracle
+from cupami_amd import _hip
+from helpers import coo, random_rows
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return _hip.context(0)
+
+
+def random_store(seed, n, n_items, density):
+    rng = np.random.default_rng(seed)
+    mask = rng.random((n, n_items)) < density
+    r, c = np.nonzero(mask)
+    return r.astype(np.uint64), c.astype(np.uint32)
+
+
+@pytest.mark.parametrize("n", [0, 1, 31, 63, 64, 65, 129, 4096, 100_003])
+def test_bits_build(ctx, n):
+    rows, cols = random_store(n, n, 7, 0.3)
+    # duplicates must OR together
+    rows = np.concatenate([rows, rows[:5]])
+    cols = np.concatenate([cols, cols[:5]])
+    bits = ctx.build_bits(rows, cols, n, 7)
+    assert np.array_equal(bits.download(), oracle.bits_build(rows, cols, n, 7))
+
+
+def test_bits_build_rejects_out_of_range(ctx):
+    with pytest.raises(ValueError):
+        ctx.build_bits(np.array([0, 5], np.uint64), np.array([0, 3], np.uint32), 5, 4)
+    with pytest.raises(ValueError):
+        ctx.build_bits(np.array([0, 1], np.uint64), np.array([0, 4], np.uint32), 5, 4)
+
+
+@pytest.mark.parametrize("n,width", [(70, 1), (70, 2), (5000, 3), (200_001, 4)])
+def test_count_supports(ctx, n, width):
+    rows, cols = random_store(width * 7 + n, n, 9, 0.6)
+    bits = ctx.build_bits(rows, cols, n, 9)
+    host = oracle.bits_build(rows, cols, n, 9)
+    rng = np.random.default_rng(n)
+    cand = np.stack([rng.choice(9, width, replace=False) for _ in range(300)]).astype(np.uint32)
+    assert np.array_equal(ctx.count_supports(bits, cand), oracle.count_supports(host, cand))
+
+
+def test_intersect_extensions(ctx):
+    rng = np.random.default_rng(3)
+    for words in (1, 2, 63, 64, 65, 3000):
+        prefix = rng.integers(0, 2**63, words, dtype=np.uint64) | rng.integers(0, 2**63, words, dtype=np.uint64)
+        ext = rng.integers(0, 2**63, (37, words), dtype=np.uint64)
+        expect_rows = prefix[None, :] & ext
+        expect_sup = np.array([sum(bin(int(x)).count("1") for x in r) for r in expect_rows])
+        sup, rows, idx = ctx.intersect_extensions(prefix, ext, keep_min=0)
+        assert np.array_equal(sup, expect_sup)
+        assert np.array_equal(rows, expect_rows) and np.array_equal(idx, np.arange(37))
+        thr = int(np.median(expect_sup))
+        sup, rows, idx = ctx.intersect_extensions(prefix, ext, keep_min=thr)
+        keep = np.nonzero(expect_sup >= thr)[0]
+        assert np.array_equal(idx, keep) and np.array_equal(rows, expect_rows[keep])
+
+
+@pytest.mark.parametrize("n,items", [(64, 3), (1000, 17), (70_000, 65), (300_001, 130)])
+def test_pair_gram_equals_popcount(ctx, n, items):
+    rows, cols = random_store(n + items, n, items, 0.35)
+    bits = ctx.build_bits(rows, cols, n, items)
+    host = oracle.bits_build(rows, cols, n, items)
+    order = np.random.default_rng(items).permutation(items).astype(np.uint32)
+    gram = ctx.pair_gram(bits, order)
+    pairs = [(a, b) for a in range(items) for b in range(a, items)]
+    cand = np.array([(order[a], order[b]) for a, b in pairs], np.uint32)
+    expect = oracle.count_supports(host, cand)
+    got = np.array([gram[a, b] for a, b in pairs], np.uint64)
+    assert np.array_equal(got, expect)
+    assert np.all(np.tril(gram, -1) == 0)
+
+
+@pytest.mark.parametrize("n", [1, 64, 65, 5000, 100_001])
+def test_periodic_eval(ctx, n):
+    rows, cols = random_store(n, n, 6, 0.4)
+    rng = np.random.default_rng(n + 1)
+    ts = np.cumsum(rng.integers(0, 4, n)).astype(np.int64)   # repeats allowed
+    end = int(ts[-1]) + 5 if n else 0
+    bits = ctx.build_bits(rows, cols, n, 6)
+    host = oracle.bits_build(rows, cols, n, 6)
+    cand = np.array([c for w in (1, 2, 3) for c in combinations(range(6), w)][:40], dtype=object)
+    for width in (1, 2, 3):
+        cw = np.array([c for c in combinations(range(6), width)], np.uint32)
+        s, p = ctx.periodic_eval(bits, ts, end, cw)
+        es, ep = oracle.periodic_eval(host, ts, n, end, cw)
+        assert np.array_equal(s, es) and np.array_equal(p, ep)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_eclat_mine_random(ctx, seed):
+    rng = random.Random(seed)
+    rows = random_rows(rng, 40 + 37 * seed, 10, 9)
+    labels, rr, cc, n = coo(rows)
+    bits = ctx.build_bits(rr, cc, n, len(labels))
+    host = oracle.bits_build(rr, cc, n, len(labels))
+    for ms in (0, 1, 2, 7, n // 4, n, n + 1):
+        expect = oracle.eclat(host, ms)
+        for gram in (True, False):
+            assert ctx.eclat_mine(bits, ms, use_gram=gram).to_dict() == expect, (ms, gram)
+
+
+def test_eclat_mine_zipf_medium(ctx):
+    from cupami_amd import datagen
+    db = datagen.transactional(60_000, 120, 8.0, 1.0, seed=11)
+    labels, cols = datagen.label_columns(db.items)
+    bits = ctx.build_bits(db.rows, cols, db.n_transactions, len(labels))
+    host = oracle.bits_build(db.rows, cols, db.n_transactions, len(labels))
+    expect = oracle.eclat(host, 150)
+    assert len(expect) > 1000
+    assert ctx.eclat_mine(bits, 150, use_gram=True).to_dict() == expect
+    assert ctx.eclat_mine(bits, 150, use_gram=False).to_dict() == expect
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_pfeclat_mine_random(ctx, seed):
+    from helpers import temporal_coo
+    rng = random.Random(seed)
+    rows = [[rng.randrange(0, 400), *rng.sample(range(8), rng.randint(0, 8))] for _ in range(150)]
+    labels, rr, cc, n, ts, end = temporal_coo(rows)
+    bits = ctx.build_bits(rr, cc, n, len(labels))
+    host = oracle.bits_build(rr, cc, n, len(labels))
+    for ms, per in [(1, 400), (3, 60), (5, 30), (20, 200)]:
+        expect = oracle.pfeclat(host, ts, n, end, ms, per)
+        assert ctx.pfeclat_mine(bits, ts, end, ms, per).to_dict() == expect, (ms, per)
