@@ -11,6 +11,12 @@ events on the library's stream) and a CPU baseline: the oracle's OpenMP
 vertical Eclat on the same workload, which doubles as the parity check.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                    [--workload eclat_1m_1k|pfeclat_2m_500] [--copy-calibration]
+
+``--workload pfeclat_2m_500`` measures the periodic-frequent search on
+2,000,000 timestamped baskets over 500 items (minSup 10,000, maxPer 2,000)
+instead.  ``--copy-calibration`` adds one 256 MiB device-to-device copy of
+known size, so HBM counters read under a profiler can be scaled to bytes.
 """
 
 from __future__ import annotations
@@ -28,10 +34,27 @@ sys.path.insert(0, str(ROOT))
 METRIC = "itemsets/s + bitset GB/s (Eclat 1M×1k), batched vs plain; serve p99 latency"
 HBM_PEAK_GBS = 8000.0
 MINSUP = 1000  # 0.001 x 1,000,000
+PF_MINSUP, PF_MAXPER = 10_000, 2_000  # 0.005 x 2,000,000; maxPer 2,000
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per support-kernel launch from the committed counter runs
+    (profiles/eclat_1m_1k_hbm_traffic.md), used only while the kernel source
+    is the one that was measured."""
+    import hashlib
+    prof = ROOT / "profiles" / "eclat_1m_1k_hbm_traffic.md"
+    src = ROOT / "cupami_amd" / "csrc" / "support.hip"
+    if not prof.exists() or not src.exists():
+        return None
+    fields = dict(line.split(": ", 1) for line in prof.read_text().splitlines()
+                  if line.startswith(("k_support_count_hbm_bytes_per_launch:", "support_hip_sha256:")))
+    if fields.get("support_hip_sha256") != hashlib.sha256(src.read_bytes()).hexdigest():
+        return None
+    return float(fields["k_support_count_hbm_bytes_per_launch"])
 
 
 def main():
@@ -40,6 +63,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="eclat_1m_1k", choices=["eclat_1m_1k", "pfeclat_2m_500"])
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -50,9 +75,12 @@ def main():
     import torch
     from cupami_amd import _hip, datagen
 
+    periodic = args.workload == "pfeclat_2m_500"
     t = time.perf_counter()
-    db = datagen.eclat_1m_1k()
+    db = datagen.temporal_2m_500() if periodic else datagen.eclat_1m_1k()
     labels, cols = datagen.label_columns(db.items)
+    minsup = PF_MINSUP if periodic else MINSUP
+    end = int(db.timestamps[-1]) if periodic else 0
     n, items, nnz = db.n_transactions, len(labels), db.rows.size
     log(f"bench: generated {n} x {items} ({nnz} pairs) in {time.perf_counter() - t:.1f}s")
 
@@ -65,7 +93,9 @@ def main():
     def step():
         bits = ctx.build_bits_device(d_rows.data_ptr(), d_cols.data_ptr(), nnz, n, items)
         try:
-            return ctx.eclat_mine(bits, MINSUP, use_gram=True)
+            if periodic:
+                return ctx.pfeclat_mine(bits, db.timestamps, end, minsup, PF_MAXPER)
+            return ctx.eclat_mine(bits, minsup, use_gram=True)
         finally:
             bits.close()
 
@@ -107,25 +137,47 @@ def main():
         host = oracle.bits_build(db.rows, cols, n, items)
         threads = oracle.threads()
         t = time.perf_counter()
-        expect = oracle.eclat(host, MINSUP)
+        if periodic:
+            expect = oracle.pfeclat(host, db.timestamps, n, end, minsup, PF_MAXPER)
+        else:
+            expect = oracle.eclat(host, minsup)
         cpu_s = time.perf_counter() - t
         parity = pats.to_dict() == expect
         cpu = {"value": round(len(expect) / cpu_s, 1), "unit": "itemsets/s", "cores": threads,
                "kind": "port",
                "sample": f"whole workload, one run ({len(expect)} itemsets in {cpu_s:.2f}s): "
-                         "oracle/cpm_oracle.c vertical Eclat, OpenMP over prefix classes"}
+                         f"oracle/cpm_oracle.c vertical {'PF-' if periodic else ''}Eclat, "
+                         "OpenMP over prefix classes"}
         log(f"bench: cpu baseline {cpu_s:.2f}s on {threads} threads, parity={parity}")
         if not parity:
             log("bench: PARITY FAILURE against the oracle")
 
+    if args.copy_calibration:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("bench: copied 268435456 bytes device to device (counter calibration)")
+
+    kernel = "k_support_periodic" if periodic else "k_support_count"
+    roofline["kernel"] = kernel
+    if not periodic:
+        roofline["traffic"] = measured_traffic()
+    if periodic:
+        config = {"workload": "pfeclat_2m_500", "transactions": n, "items": items, "lambda": 8,
+                  "zipf_s": 1.05, "minsup": minsup, "maxper": PF_MAXPER, "patterns": n_pat,
+                  "parallelism": "single GPU"}
+    else:
+        config = {"workload": "eclat_1m_1k", "transactions": n, "items": items,
+                  "lambda": 10, "zipf_s": 1.1, "minsup": minsup, "patterns": n_pat,
+                  "parallelism": "single GPU"}
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "itemsets/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
-        "data": "synthetic (numpy PCG64 seed 202)",
-        "config": {"workload": "eclat_1m_1k", "transactions": n, "items": items,
-                   "lambda": 10, "zipf_s": 1.1, "minsup": MINSUP, "patterns": n_pat,
-                   "parallelism": "single GPU"},
+        "data": f"synthetic (numpy PCG64 seed {404 if periodic else 202})",
+        "config": config,
         "roofline": roofline, "cpu_baseline": cpu, "gram": gram,
         "levels_per_step": stats["levels"], "parity": parity,
     }

@@ -148,6 +148,9 @@ int cpm_ctx_destroy(cpm_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &b : ctx->scratch) b.release();
     ctx->host_scratch.release();
+    if (ctx->search_ws_free) ctx->search_ws_free(ctx->search_ws);
+    ctx->search_ws = nullptr;
+    ctx->spare_store.release();
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -171,6 +174,10 @@ static int bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits
     b->words = (n_trans + 63) / 64;
     b->stride = row_stride(b->words);
     size_t bytes = (size_t)n_items * b->stride * sizeof(uint64_t);
+    if (ctx->spare_store.p && ctx->spare_store.cap >= bytes) {
+        std::swap(b->store.p, ctx->spare_store.p);
+        std::swap(b->store.cap, ctx->spare_store.cap);
+    }
     int s = b->store.ensure(bytes ? bytes : 8);
     if (s != CPM_OK) {
         delete b;
@@ -246,8 +253,14 @@ int cpm_bits_build(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, uin
 int cpm_bits_destroy(cpm_bits *b) {
     if (!b) return CPM_OK;
     if (b->ctx) {
-        (void)hipSetDevice(b->ctx->device);
-        (void)hipStreamSynchronize(b->ctx->stream);
+        cpm_ctx *ctx = b->ctx;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        // keep the larger allocation for the next build on this context
+        if (b->store.cap > ctx->spare_store.cap) {
+            std::swap(b->store.p, ctx->spare_store.p);
+            std::swap(b->store.cap, ctx->spare_store.cap);
+        }
     }
     delete b;
     return CPM_OK;

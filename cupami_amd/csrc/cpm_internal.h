@@ -73,6 +73,11 @@ struct cpm_ctx {
     // scratch used by the operator entry points
     cpm::DBuf scratch[6];
     cpm::HBuf host_scratch;
+    // search buffers (miner.hip), reused across searches
+    void *search_ws = nullptr;
+    void (*search_ws_free)(void *) = nullptr;
+    // allocation of the last destroyed store, reused by the next store build
+    cpm::DBuf spare_store;
 };
 
 struct cpm_bits {

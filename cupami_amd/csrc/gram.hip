@@ -2,10 +2,10 @@
 //
 // With D the N x F 0/1 matrix of the F item rows, the pair supports are
 // G = D^T D: G[a][b] = |row_a & row_b|.  Each 64-bit word of a row is 64
-// consecutive K elements.  A workgroup owns a 64x64 block of G (upper block
-// triangle only) and one slice of the K (transaction) axis; it stages 32-word
-// tiles of its 64 + 64 rows in LDS and each wave accumulates a 32x32 sub-block
-// with v_mfma_i32_16x16x64_i8 (2 x 2 tiles).  The bits become int8 0/1 in
+// consecutive K elements.  A workgroup owns a 128x128 block of G (upper
+// block triangle only) and one slice of the K (transaction) axis; it stages
+// 16-word tiles of its 128 + 128 rows in LDS and each wave accumulates a
+// 64x64 quadrant with v_mfma_i32_16x16x64_i8 (4 x 4 tiles).  The bits become int8 0/1 in
 // registers: lane group g (lane / 16) takes 16 bits of each word and spreads
 // them into 16 bytes with shift-and-mask; A (row items) and B (column items)
 // use the same lane -> bit map, so every bit is paired with itself exactly
@@ -21,8 +21,8 @@ namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int kBT = 64;   // block tile (items)
-constexpr int kKW = 32;   // words per K tile
+constexpr int kBT = 128;  // block tile (items)
+constexpr int kKW = 16;   // words per K tile
 
 // 16 bits of `w` -> 16 int8 0/1 values for lane group g (0..3).
 __device__ __forceinline__ v4i spread_bits(uint64_t w, uint32_t g) {
@@ -36,6 +36,9 @@ __device__ __forceinline__ v4i spread_bits(uint64_t w, uint32_t g) {
     return r;
 }
 
+// One workgroup: a 128x128 block of G (bi <= bj) over one K slice.  Each
+// wave owns a 64x64 quadrant as 4x4 MFMA tiles; the next K tile is loaded
+// into registers while the current one is multiplied out of LDS.
 __global__ __launch_bounds__(256) void k_gram_i8(const uint64_t *__restrict__ store,
                                                 uint64_t stride, uint64_t words,
                                                 const uint32_t *__restrict__ cols, uint32_t n,
@@ -45,7 +48,6 @@ __global__ __launch_bounds__(256) void k_gram_i8(const uint64_t *__restrict__ st
     __shared__ uint64_t sA[kBT][kKW + 1];
     __shared__ uint64_t sB[kBT][kKW + 1];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // block -> (bi, bj) with bi <= bj, and a K slice
     const uint32_t slice = blockIdx.x % slices;
     uint32_t tri = blockIdx.x / slices;
     uint32_t bi = 0;
@@ -57,56 +59,74 @@ __global__ __launch_bounds__(256) void k_gram_i8(const uint64_t *__restrict__ st
     const uint64_t k_begin = (uint64_t)slice * slice_words;
     const uint64_t k_end = std::min<uint64_t>(words, k_begin + slice_words);
 
+    // loader: lane pair (2r, 2r+1) stages row r, 8 words each
+    const uint32_t lrow = tid >> 1, lw = (tid & 1) * 8;
+    const uint32_t ia = bi * kBT + lrow, ib = bj * kBT + lrow;
+    const uint64_t *pa = ia < n ? store + (uint64_t)cols[ia] * stride : nullptr;
+    const uint64_t *pb = ib < n ? store + (uint64_t)cols[ib] * stride : nullptr;
+    uint4 ra[4], rb[4];
+    auto fetch = [&](uint64_t k0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint64_t w = k0 + lw + 2 * q;  // even; w + 1 < stride (zero padded)
+            const bool in = w < k_end;
+            ra[q] = (pa && in) ? *reinterpret_cast<const uint4 *>(pa + w) : make_uint4(0, 0, 0, 0);
+            rb[q] = (pb && in) ? *reinterpret_cast<const uint4 *>(pb + w) : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            sA[lrow][lw + 2 * q] = (uint64_t)ra[q].x | ((uint64_t)ra[q].y << 32);
+            sA[lrow][lw + 2 * q + 1] = (uint64_t)ra[q].z | ((uint64_t)ra[q].w << 32);
+            sB[lrow][lw + 2 * q] = (uint64_t)rb[q].x | ((uint64_t)rb[q].y << 32);
+            sB[lrow][lw + 2 * q + 1] = (uint64_t)rb[q].z | ((uint64_t)rb[q].w << 32);
+        }
+    };
+
     const uint32_t wr = wave >> 1, wc = wave & 1;
     const uint32_t li = lane & 15, lg = lane >> 4;
-    v4i acc[2][2];
+    // on a diagonal block the lower-left quadrant lies below the diagonal
+    const bool idle = (bi == bj) && (wr > wc);
+    v4i acc[4][4];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = (v4i){0, 0, 0, 0};
+        for (int b = 0; b < 4; ++b) acc[a][b] = (v4i){0, 0, 0, 0};
 
-    // each thread stages 8 words of A and 8 of B per K tile
+    if (k_begin < k_end) fetch(k_begin);
     for (uint64_t k0 = k_begin; k0 < k_end; k0 += kKW) {
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const uint32_t idx = tid + 256 * r;  // 0..2047 -> (row, word)
-            const uint32_t row = idx / kKW, wd = idx % kKW;
-            const uint64_t k = k0 + wd;
-            uint64_t va = 0, vb = 0;
-            const uint32_t ia = bi * kBT + row, ib = bj * kBT + row;
-            if (k < k_end) {
-                if (ia < n) va = store[(uint64_t)cols[ia] * stride + k];
-                if (ib < n) vb = store[(uint64_t)cols[ib] * stride + k];
-            }
-            sA[row][wd] = va;
-            sB[row][wd] = vb;
-        }
+        stash();
         __syncthreads();
-#pragma unroll 4
-        for (int w = 0; w < kKW; ++w) {
-            v4i fa[2], fb[2];
+        if (k0 + kKW < k_end) fetch(k0 + kKW);
+        if (!idle) {
+#pragma unroll 2
+            for (int w = 0; w < kKW; ++w) {
+                v4i fa[4], fb[4];
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                fa[t] = spread_bits(sA[wr * 32 + t * 16 + li][w], lg);
-                fb[t] = spread_bits(sB[wc * 32 + t * 16 + li][w], lg);
+                for (int t = 0; t < 4; ++t) {
+                    fa[t] = spread_bits(sA[wr * 64 + t * 16 + li][w], lg);
+                    fb[t] = spread_bits(sB[wc * 64 + t * 16 + li][w], lg);
+                }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
             }
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
         }
         __syncthreads();
     }
+    if (idle) return;
     // D tile layout: lane holds rows 4*(lane/16) + r, column lane % 16
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const uint32_t col = bj * kBT + wc * 32 + b * 16 + li;
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t col = bj * kBT + wc * 64 + b * 16 + li;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const uint32_t row = bi * kBT + wr * 32 + a * 16 + lg * 4 + r;
+                const uint32_t row = bi * kBT + wr * 64 + a * 16 + lg * 4 + r;
                 const int v = acc[a][b][r];
                 if (row < n && col < n && row <= col && v) atomicAdd(out + (uint64_t)row * ld + col, v);
             }
@@ -124,7 +144,7 @@ int launch_gram_i8(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t *d_cols, u
     if (bits->words == 0) return CPM_OK;
     const uint32_t nb = (n + kBT - 1) / kBT;
     const uint64_t tri = (uint64_t)nb * (nb + 1) / 2;
-    const uint64_t want = (uint64_t)ctx->cu_count * 4;
+    const uint64_t want = (uint64_t)ctx->cu_count * 2;
     uint64_t slices = std::max<uint64_t>(1, (want + tri - 1) / tri);
     const uint64_t ktiles = (bits->words + kKW - 1) / kKW;
     slices = std::min<uint64_t>(slices, ktiles);

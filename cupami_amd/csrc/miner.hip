@@ -92,6 +92,35 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *lds) {
     return t;
 }
 
+// The kept set of a task as a bit mask over the frequent items, in LDS: a
+// child task K_j takes the kept items g > K_j whose pair (K_j, g) is kept,
+// which is the AND of this mask with row K_j of the pair table.
+constexpr uint32_t kMaskWords = 2048;  // up to 65536 frequent items
+
+__device__ __forceinline__ uint32_t above(uint32_t f) { return ~((2u << (f & 31)) - 1u); }
+
+__device__ __forceinline__ void mask_set(uint32_t *lmask, const uint32_t *__restrict__ kept,
+                                         const uint32_t *__restrict__ ext, uint32_t a, uint32_t nk) {
+    for (uint32_t j = threadIdx.x; j < nk; j += blockDim.x) {
+        const uint32_t f = ext[kept[a + j]];
+        atomicOr(&lmask[f >> 5], 1u << (f & 31));
+    }
+}
+
+__device__ __forceinline__ void mask_clear(uint32_t *lmask, const uint32_t *__restrict__ kept,
+                                           const uint32_t *__restrict__ ext, uint32_t a, uint32_t nk) {
+    for (uint32_t j = threadIdx.x; j < nk; j += blockDim.x) lmask[ext[kept[a + j]] >> 5] = 0;
+}
+
+// number of child entries of kept item fj (mask path)
+__device__ __forceinline__ uint32_t child_count(const uint32_t *lmask, const uint32_t *__restrict__ prow,
+                                                uint32_t fj, uint32_t wlast) {
+    uint32_t w = fj >> 5;
+    uint32_t c = __popc(lmask[w] & prow[w] & above(fj));
+    for (++w; w <= wlast; ++w) c += __popc(lmask[w] & prow[w]);
+    return c;
+}
+
 // plan/1 -------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_plan_count(Level L, Keep keep, PairTable pairs,
                                                    int emit_children, uint32_t *__restrict__ kept,
@@ -99,7 +128,10 @@ __global__ __launch_bounds__(256) void k_plan_count(Level L, Keep keep, PairTabl
                                                    uint32_t *__restrict__ cnt_child,
                                                    uint32_t *__restrict__ cnt_cext) {
     __shared__ uint32_t lds[8];
+    __shared__ uint32_t lmask[kMaskWords];
     const uint32_t tid = threadIdx.x;
+    for (uint32_t w = tid; w < kMaskWords; w += blockDim.x) lmask[w] = 0;
+    __syncthreads();
     for (uint32_t t = blockIdx.x; t < L.n_tasks; t += gridDim.x) {
         const uint32_t a = L.ext_off[t], b = L.ext_off[t + 1];
         uint32_t nk = 0;
@@ -115,11 +147,19 @@ __global__ __launch_bounds__(256) void k_plan_count(Level L, Keep keep, PairTabl
         uint32_t cext = 0;
         if (emit_children && nk >= 2) {
             uint32_t c = 0;
-            for (uint32_t j = tid; j + 1 < nk; j += 256) {
-                const uint32_t fj = L.ext[kept[a + j]];
-                for (uint32_t l = j + 1; l < nk; ++l) c += pairs.ok(fj, L.ext[kept[a + l]]);
+            if (pairs.bits) {
+                mask_set(lmask, kept, L.ext, a, nk);
+                __syncthreads();
+                const uint32_t wlast = L.ext[kept[a + nk - 1]] >> 5;
+                for (uint32_t j = tid; j + 1 < nk; j += 256) {
+                    const uint32_t fj = L.ext[kept[a + j]];
+                    c += child_count(lmask, pairs.bits + (uint64_t)fj * pairs.words32, fj, wlast);
+                }
+            } else {
+                for (uint32_t j = tid; j + 1 < nk; j += 256) c += nk - 1 - j;
             }
             cext = block_sum(c, lds);
+            if (pairs.bits) mask_clear(lmask, kept, L.ext, a, nk);
         }
         if (tid == 0) {
             cnt_pat[t] = nk;
@@ -150,8 +190,11 @@ __global__ __launch_bounds__(256) void k_plan_write(Level L, Keep keep, PairTabl
                                                    const uint32_t *__restrict__ off_cext,
                                                    PlanOut o) {
     __shared__ uint32_t lds[8];
+    __shared__ uint32_t lmask[kMaskWords];
     const uint32_t tid = threadIdx.x;
     const uint32_t d1 = L.d + 1;
+    for (uint32_t w = tid; w < kMaskWords; w += blockDim.x) lmask[w] = 0;
+    __syncthreads();
     for (uint32_t t = blockIdx.x; t < L.n_tasks; t += gridDim.x) {
         const uint32_t a = L.ext_off[t];
         const uint32_t nk = cnt_pat[t];
@@ -166,6 +209,11 @@ __global__ __launch_bounds__(256) void k_plan_write(Level L, Keep keep, PairTabl
             if (o.pat_per) o.pat_per[pb + j] = keep.per[e];
         }
         if (!(emit_children && nk >= 2)) continue;
+        if (pairs.bits) {
+            mask_set(lmask, kept, L.ext, a, nk);
+            __syncthreads();
+        }
+        const uint32_t wlast = L.ext[kept[a + nk - 1]] >> 5;
         const uint32_t cb = off_child[t];
         const uint32_t xb = off_cext[t];
         uint32_t carry = 0;
@@ -173,9 +221,15 @@ __global__ __launch_bounds__(256) void k_plan_write(Level L, Keep keep, PairTabl
             const uint32_t j = j0 + tid;
             const bool live = j + 1 < nk;
             uint32_t cj = 0, fj = 0;
+            const uint32_t *prow = nullptr;
             if (live) {
                 fj = L.ext[kept[a + j]];
-                for (uint32_t l = j + 1; l < nk; ++l) cj += pairs.ok(fj, L.ext[kept[a + l]]);
+                if (pairs.bits) {
+                    prow = pairs.bits + (uint64_t)fj * pairs.words32;
+                    cj = child_count(lmask, prow, fj, wlast);
+                } else {
+                    cj = nk - 1 - j;
+                }
             }
             uint32_t tot;
             const uint32_t off = block_scan(cj, lds, &tot);
@@ -189,16 +243,31 @@ __global__ __launch_bounds__(256) void k_plan_write(Level L, Keep keep, PairTabl
                 uint32_t w = xb + carry + off;
                 o.ext_off[ct] = w;
                 o.ngrp[ct] = (cj + kGroup - 1) / kGroup;
-                for (uint32_t l = j + 1; l < nk; ++l) {
-                    const uint32_t fl = L.ext[kept[a + l]];
-                    if (pairs.ok(fj, fl)) {
-                        o.ext[w] = fl;
+                if (pairs.bits) {
+                    for (uint32_t q = fj >> 5; q <= wlast; ++q) {
+                        uint32_t m = lmask[q] & prow[q];
+                        if (q == (fj >> 5)) m &= above(fj);
+                        while (m) {
+                            o.ext[w] = q * 32 + (uint32_t)__builtin_ctz(m);
+                            o.sup[w] = 0;
+                            ++w;
+                            m &= m - 1;
+                        }
+                    }
+                } else {
+                    for (uint32_t l = j + 1; l < nk; ++l) {
+                        o.ext[w] = L.ext[kept[a + l]];
                         o.sup[w] = 0;
                         ++w;
                     }
                 }
             }
             carry += tot;
+        }
+        if (pairs.bits) {
+            __syncthreads();
+            mask_clear(lmask, kept, L.ext, a, nk);
+            __syncthreads();
         }
     }
 }
@@ -323,14 +392,33 @@ int grow_copy(cpm_ctx *ctx, DBuf &b, size_t keep_bytes, size_t want) {
 }
 
 int arena_reserve(cpm_ctx *ctx, Arena &A, uint64_t need) {
-    if (need <= A.cap) return CPM_OK;
-    uint64_t cap = std::max<uint64_t>(need, std::max<uint64_t>(A.cap * 2, 4096));
+    uint64_t cap = A.cap;
+    if (need > cap) cap = std::max<uint64_t>(need, std::max<uint64_t>(cap * 2, 4096));
     CPM_CHECK(grow_copy(ctx, A.parent, A.n * 4, cap * 4));
     CPM_CHECK(grow_copy(ctx, A.item, A.n * 4, cap * 4));
     CPM_CHECK(grow_copy(ctx, A.sup, A.n * 4, cap * 4));
     if (A.periodic) CPM_CHECK(grow_copy(ctx, A.per, A.n * 8, cap * 8));
     A.cap = cap;
     return CPM_OK;
+}
+
+// Search buffers, kept by the context between searches so that repeated
+// searches allocate nothing once the buffers have reached their sizes.
+struct SearchWs {
+    LevelBufs lv[2];
+    Arena arena;
+    DBuf kept, cnt[3], off[3], totals, fcol, pairbits, gram, parts, ts;
+    HBuf host;
+};
+
+void free_search_ws(void *p) { delete static_cast<SearchWs *>(p); }
+
+SearchWs &search_ws(cpm_ctx *ctx) {
+    if (!ctx->search_ws) {
+        ctx->search_ws = new SearchWs();
+        ctx->search_ws_free = free_search_ws;
+    }
+    return *static_cast<SearchWs *>(ctx->search_ws);
 }
 
 struct Miner {
@@ -343,11 +431,19 @@ struct Miner {
     bool use_gram;
     const int64_t *d_ts = nullptr;
 
-    LevelBufs lv[2];
-    Arena arena;
-    DBuf kept, cnt[3], off[3], totals, fcol, pairbits, gram, parts;
-    HBuf host;
+    LevelBufs (&lv)[2];
+    Arena &arena;
+    DBuf &kept;
+    DBuf (&cnt)[3];
+    DBuf (&off)[3];
+    DBuf &totals, &fcol, &pairbits, &gram, &parts;
+    HBuf &host;
     uint32_t nf = 0;
+
+    Miner(cpm_ctx *c, SearchWs &w)
+        : ctx(c), lv(w.lv), arena(w.arena), kept(w.kept), cnt(w.cnt), off(w.off),
+          totals(w.totals), fcol(w.fcol), pairbits(w.pairbits), gram(w.gram), parts(w.parts),
+          host(w.host) {}
 
     uint32_t grid_tasks(uint32_t n) const {
         return std::max<uint32_t>(1, std::min<uint32_t>(n, (uint32_t)ctx->cu_count * 8));
@@ -479,6 +575,7 @@ struct Miner {
     int run() {
         const uint32_t n_items = bits->n_items;
         arena.periodic = periodic;
+        arena.n = 0;
         CPM_CHECK(host.ensure(64));
         CPM_CHECK(totals.ensure(64));
         if (n_items == 0) return CPM_OK;
@@ -552,15 +649,19 @@ struct Miner {
             CPM_CHECK(count(L2, fcol.as<uint32_t>()));
         }
         ctx->stats[7] += 1;
-        // pair table from the level-2 outcome
+        // pair table from the level-2 outcome (the plan kernels hold a task's
+        // kept set in an LDS mask of 65536 bits)
+        PairTable pt{nullptr, 0};
         const uint32_t w32 = (nf + 31) / 32;
+        if (nf <= 65536) {
         CPM_CHECK(pairbits.ensure((size_t)nf * w32 * 4));
         CPM_HIP(hipMemsetAsync(pairbits.p, 0, (size_t)nf * w32 * 4, ctx->stream));
         hipLaunchKernelGGL(k_pair_table, dim3(std::min<uint32_t>(nf, 4096)), dim3(256), 0,
                            ctx->stream, nf, L2.ext_off.as<uint32_t>(), L2.ext.as<uint32_t>(),
                            keep_of(L2), pairbits.as<uint32_t>(), w32);
         CPM_HIP(hipGetLastError());
-        PairTable pt{pairbits.as<uint32_t>(), w32};
+        pt = PairTable{pairbits.as<uint32_t>(), w32};
+        }
 
         int cur = 1;
         for (;;) {
@@ -621,8 +722,7 @@ extern "C" int cpm_eclat_mine(cpm_ctx *ctx, const cpm_bits *bits, uint64_t minsu
     *out = nullptr;
     if (bits->n_transactions >= 0xffffffffull)
         return fail(CPM_ERR_INVALID, "supports are counted in 32 bits: too many transactions");
-    Miner m;
-    m.ctx = ctx;
+    Miner m(ctx, search_ws(ctx));
     m.bits = bits;
     m.minsup = minsup;
     m.maxper = 0;
@@ -643,15 +743,14 @@ extern "C" int cpm_pfeclat_mine(cpm_ctx *ctx, const cpm_bits *bits, const int64_
     if (bits->n_transactions >= 0xffffffffull)
         return fail(CPM_ERR_INVALID, "supports are counted in 32 bits: too many transactions");
     if (bits->n_transactions && !timestamps) return fail(CPM_ERR_INVALID, "null timestamps");
-    Miner m;
-    m.ctx = ctx;
+    Miner m(ctx, search_ws(ctx));
     m.bits = bits;
     m.minsup = minsup;
     m.maxper = maxper_floor;
     m.end = end;
     m.periodic = true;
     m.use_gram = false;
-    DBuf ts;
+    DBuf &ts = search_ws(ctx).ts;
     const uint64_t n = bits->n_transactions;
     CPM_CHECK(ts.ensure((n ? n : 1) * 8));
     if (n) CPM_HIP(hipMemcpyAsync(ts.p, timestamps, n * 8, hipMemcpyHostToDevice, ctx->stream));

@@ -55,3 +55,15 @@ def test_eclat_full_size_equals_oracle(ctx, c2):
     # the popcount level-2 path gives the same forest
     again = ctx.eclat_mine(bits, 1000, use_gram=False)
     assert again.to_dict() == expect
+
+
+def test_pfeclat_full_size_equals_oracle(ctx):
+    db = datagen.temporal_2m_500()
+    labels, cols = datagen.label_columns(db.items)
+    bits = ctx.build_bits(db.rows, cols, db.n_transactions, len(labels))
+    host = oracle.bits_build(db.rows, cols, db.n_transactions, len(labels))
+    end = int(db.timestamps[-1])
+    expect = oracle.pfeclat(host, db.timestamps, db.n_transactions, end, 10_000, 2_000)
+    got = ctx.pfeclat_mine(bits, db.timestamps, end, 10_000, 2_000).to_dict()
+    assert len(got) == len(expect) == 879
+    assert got == expect
