@@ -55,6 +55,17 @@ SIGNATURES = {
                                       ctypes.POINTER(_vp)]),
     "cpm_pfeclat_mine": (ctypes.c_int, [_vp, _vp, _i64p, ctypes.c_int64, ctypes.c_uint64,
                                         ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "cpm_unique_pairs": (ctypes.c_int, [_vp, _u64p, _u32p, ctypes.c_uint64, _u64p, _u32p, _u64p]),
+    "cpm_fptree_build": (ctypes.c_int, [_vp, _u64p, _u32p, ctypes.c_uint64, _u64p, ctypes.c_uint64,
+                                        ctypes.c_uint32, ctypes.c_uint64, _u64p,
+                                        ctypes.POINTER(_vp)]),
+    "cpm_fptree_conditional": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint64,
+                                              ctypes.POINTER(_vp)]),
+    "cpm_fptree_info": (ctypes.c_int, [_vp, _u64p, _u32p, ctypes.POINTER(ctypes.c_int), _u32p]),
+    "cpm_fptree_download": (ctypes.c_int, [_vp, _i64p, _i64p, _u64p, _u32p, _u64p]),
+    "cpm_fptree_destroy": (ctypes.c_int, [_vp]),
+    "cpm_fpgrowth_mine": (ctypes.c_int, [_vp, _u64p, _u32p, ctypes.c_uint64, ctypes.c_uint64,
+                                         ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "cpm_patterns_count": (ctypes.c_int, [_vp, _u64p]),
     "cpm_patterns_download": (ctypes.c_int, [_vp, _u32p, _u32p, _u64p, _i64p]),
     "cpm_patterns_destroy": (ctypes.c_int, [_vp]),
@@ -231,6 +242,85 @@ class Context:
         check(load().cpm_pfeclat_mine(self.handle, bits.handle, _p(ts, _i64p), int(end),
                                       clamp_u64(minsup), int(maxper_floor), ctypes.byref(handle)))
         return Patterns(handle, periodic=True)
+
+
+    # ---- FP-tree ----------------------------------------------------------------
+    def unique_pairs(self, rows, cols):
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        cols = np.ascontiguousarray(cols, dtype=np.uint32)
+        out_r = np.zeros(max(rows.size, 1), dtype=np.uint64)
+        out_c = np.zeros(max(rows.size, 1), dtype=np.uint32)
+        n = ctypes.c_uint64(0)
+        if rows.size:
+            check(load().cpm_unique_pairs(self.handle, _p(rows, _u64p), _p(cols, _u32p), rows.size,
+                                          _p(out_r, _u64p), _p(out_c, _u32p), ctypes.byref(n)))
+        return out_r[:n.value], out_c[:n.value]
+
+    def fptree_build(self, rows, cols, weights, item_count, min_support, supports=None):
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        cols = np.ascontiguousarray(cols, dtype=np.uint32)
+        weights = np.ascontiguousarray(weights, dtype=np.uint64)
+        sup = None if supports is None else np.ascontiguousarray(supports, dtype=np.uint64)
+        handle = _vp()
+        check(load().cpm_fptree_build(self.handle, _p(rows, _u64p), _p(cols, _u32p), rows.size,
+                                      _p(weights, _u64p), weights.size, int(item_count),
+                                      clamp_u64(min_support),
+                                      None if sup is None else _p(sup, _u64p),
+                                      ctypes.byref(handle)))
+        return FPTreeHandle(self, handle) if handle.value else None
+
+    def fptree_conditional(self, tree, item, min_support):
+        handle = _vp()
+        check(load().cpm_fptree_conditional(self.handle, tree.handle, int(item),
+                                            clamp_u64(min_support), ctypes.byref(handle)))
+        return FPTreeHandle(self, handle) if handle.value else None
+
+    def fpgrowth_mine(self, rows, cols, n_transactions, n_items, minsup):
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        cols = np.ascontiguousarray(cols, dtype=np.uint32)
+        handle = _vp()
+        check(load().cpm_fpgrowth_mine(self.handle, _p(rows, _u64p), _p(cols, _u32p), rows.size,
+                                       int(n_transactions), int(n_items), clamp_u64(minsup),
+                                       ctypes.byref(handle)))
+        return Patterns(handle)
+
+
+class FPTreeHandle:
+    """Device FP-tree (``cpm_fptree``) with the reference's array view."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    def close(self):
+        if self.handle:
+            load().cpm_fptree_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def arrays(self):
+        """``(parents, items, counts, header, single_path, depth)``."""
+        n = ctypes.c_uint64()
+        depth = ctypes.c_uint32()
+        single = ctypes.c_int()
+        hlen = ctypes.c_uint32()
+        lib = load()
+        check(lib.cpm_fptree_info(self.handle, ctypes.byref(n), ctypes.byref(depth),
+                                  ctypes.byref(single), ctypes.byref(hlen)))
+        parents = np.zeros(n.value, dtype=np.int64)
+        items = np.zeros(n.value, dtype=np.int64)
+        counts = np.zeros(n.value, dtype=np.uint64)
+        hi = np.zeros(max(hlen.value, 1), dtype=np.uint32)
+        hs = np.zeros(max(hlen.value, 1), dtype=np.uint64)
+        check(lib.cpm_fptree_download(self.handle, _p(parents, _i64p), _p(items, _i64p),
+                                      _p(counts, _u64p), _p(hi, _u32p), _p(hs, _u64p)))
+        header = list(zip(hi[:hlen.value].tolist(), hs[:hlen.value].tolist()))
+        return parents, items, counts, header, bool(single.value), int(depth.value)
 
 
 def clamp_u64(value):

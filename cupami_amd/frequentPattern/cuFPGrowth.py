@@ -3,8 +3,10 @@
 Surface of ``cuFPGrowth.py:161-230`` (Han, Pei and Yin, SIGMOD 2000): same
 constructor, ``mine()``, ``_storePattern`` hook and CLI; a positive minimum
 support count is required on non-empty input (``cuFPGrowth.py:173-174``).
-The itemsets and supports are the complete set of frequent itemsets, which
-``mine()`` obtains from the device search.
+``mine()`` runs ``cpm_fpgrowth_mine``: pair dedupe, the weighted FP-tree
+build and every conditional tree are device kernels (sorts and segmented
+group-bys, ``csrc/fptree.hip``); the depth-first order over header items is
+the reference's.
 """
 
 from __future__ import annotations
@@ -21,7 +23,7 @@ from .cuApriori import cuApriori, _cli
 
 
 class cuFPGrowth(cuApriori):
-    """FP-growth surface over the device search."""
+    """FP-growth (Han, Pei and Yin, SIGMOD 2000) with device FP-trees."""
 
     def mine(self):
         self._startTime = time.perf_counter()
@@ -31,11 +33,8 @@ class cuFPGrowth(cuApriori):
         if items and self._minSup < 1:
             raise ValueError("FP-growth requires a positive minimum support count")
         if items:
-            bits = ctx.build_bits(rows, columns, self._transactionCount, len(items))
-            try:
-                patterns = self._deviceSearch(ctx, bits, items)
-            finally:
-                bits.close()
+            patterns = ctx.fpgrowth_mine(rows, columns, self._transactionCount, len(items),
+                                         self._minSup)
             for ids, support in zip(patterns.itemsets(), patterns.support.tolist()):
                 self._storePattern(ids, support, items)
         ctx.synchronize()

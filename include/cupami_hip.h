@@ -31,6 +31,7 @@ extern "C" {
 typedef struct cpm_ctx cpm_ctx;
 typedef struct cpm_bits cpm_bits;
 typedef struct cpm_patterns cpm_patterns;
+typedef struct cpm_fptree cpm_fptree;
 
 /* ABI version (major*10000 + minor*100 + patch). */
 int cpm_version(void);
@@ -119,6 +120,41 @@ int cpm_eclat_mine(cpm_ctx *ctx, const cpm_bits *bits, uint64_t minsup, int use_
  * timestamps: host, n_transactions non-decreasing int64. */
 int cpm_pfeclat_mine(cpm_ctx *ctx, const cpm_bits *bits, const int64_t *timestamps, int64_t end,
                      uint64_t minsup, int64_t maxper_floor, cpm_patterns **out);
+
+/* ---- FP-tree (device arrays) -------------------------------------------- */
+
+/* Replaces _fptree.unique_pairs (_fptree.py:97-105): distinct (row, col)
+ * pairs in (row, col) order.  Host arrays; out_rows/out_cols hold up to nnz. */
+int cpm_unique_pairs(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, uint64_t nnz,
+                     uint64_t *out_rows, uint32_t *out_cols, uint64_t *out_n);
+
+/* Replaces _fptree.build_tree (_fptree.py:108-177): weighted FP-tree of the
+ * baskets given as (row, col) pairs with per-row weights (host arrays).
+ * Node 0 is the root (parent -1, item -1); nodes of each depth follow,
+ * ordered by (parent, item); baskets are ordered by (weighted support desc,
+ * item id asc).  `supports` (item_count values, may be NULL) replaces the
+ * support count and the infrequent-pair filter, as for conditional bases.
+ * *out is set to NULL when no frequent pair remains (the reference's None). */
+int cpm_fptree_build(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, uint64_t nnz,
+                     const uint64_t *weights, uint64_t n_rows, uint32_t item_count,
+                     uint64_t minsup, const uint64_t *supports, cpm_fptree **out);
+/* Replaces _fptree.conditional_tree (_fptree.py:204-231); *out may be NULL. */
+int cpm_fptree_conditional(cpm_ctx *ctx, cpm_fptree *tree, uint32_t item, uint64_t minsup,
+                           cpm_fptree **out);
+int cpm_fptree_info(const cpm_fptree *tree, uint64_t *n_nodes, uint32_t *depth,
+                    int *single_path, uint32_t *header_len);
+/* parents/items/counts: n_nodes each; header (items, supports): header_len
+ * each, in ascending support order.  Any pointer may be NULL. */
+int cpm_fptree_download(const cpm_fptree *tree, int64_t *parents, int64_t *items,
+                        uint64_t *counts, uint32_t *header_items, uint64_t *header_supports);
+int cpm_fptree_destroy(cpm_fptree *tree);
+
+/* Replaces cuFPGrowth.mine (cuFPGrowth.py:161-213): unique pairs, unit
+ * weights, tree build, then the depth-first search over conditional trees
+ * (single-path trees enumerated directly).  minsup must be >= 1. */
+int cpm_fpgrowth_mine(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, uint64_t nnz,
+                      uint64_t n_transactions, uint32_t n_items, uint64_t minsup,
+                      cpm_patterns **out);
 
 /* ---- pattern results --------------------------------------------------- */
 
