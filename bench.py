@@ -45,16 +45,25 @@ def measured_traffic():
     """HBM bytes per support-kernel launch from the committed counter runs
     (profiles/eclat_1m_1k_hbm_traffic.md), used only while the kernel source
     is the one that was measured."""
-    import hashlib
     prof = ROOT / "profiles" / "eclat_1m_1k_hbm_traffic.md"
-    src = ROOT / "cupami_amd" / "csrc" / "support.hip"
-    if not prof.exists() or not src.exists():
+    if not prof.exists():
         return None
     fields = dict(line.split(": ", 1) for line in prof.read_text().splitlines()
-                  if line.startswith(("k_support_count_hbm_bytes_per_launch:", "support_hip_sha256:")))
-    if fields.get("support_hip_sha256") != hashlib.sha256(src.read_bytes()).hexdigest():
+                  if line.startswith(("k_support_count_hbm_bytes_per_launch:", "csrc_sha256:")))
+    if fields.get("csrc_sha256") != csrc_digest():
         return None
     return float(fields["k_support_count_hbm_bytes_per_launch"])
+
+
+def csrc_digest():
+    """SHA-256 over the kernel sources (names and contents, sorted)."""
+    import hashlib
+    h = hashlib.sha256()
+    for path in sorted((ROOT / "cupami_amd" / "csrc").iterdir()):
+        if path.suffix in (".hip", ".h"):
+            h.update(path.name.encode())
+            h.update(path.read_bytes())
+    return h.hexdigest()
 
 
 def main():

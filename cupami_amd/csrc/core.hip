@@ -95,6 +95,61 @@ __global__ __launch_bounds__(256) void k_bits_scatter(const uint64_t *__restrict
     }
 }
 
+// Is rows[] non-decreasing, and is its last row inside the store?  Raises
+// flags[0] when a descent is found.
+__global__ __launch_bounds__(256) void k_rows_sorted(const uint64_t *__restrict__ rows, uint64_t nnz,
+                                                    unsigned int *flags) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i + 1 < nnz;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        if (rows[i] > rows[i + 1]) {
+            atomicOr(flags, 1u);
+            return;
+        }
+}
+
+// Store build for row-sorted pairs: workgroup b owns words [b*B, b*B+B) of
+// every item row.  Its pairs are one contiguous run of the input; they are
+// ORed into an LDS tile (n_items x B words) and the tile is written out once,
+// so the store needs no clearing and HBM sees no atomics.
+__global__ __launch_bounds__(256) void k_bits_tile(const uint64_t *__restrict__ rows,
+                                                  const uint32_t *__restrict__ cols, uint64_t nnz,
+                                                  uint64_t n_trans, uint32_t n_items, uint64_t stride,
+                                                  uint32_t B, uint64_t *__restrict__ store,
+                                                  unsigned int *bad) {
+    extern __shared__ unsigned long long tile[];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t w0 = (uint64_t)blockIdx.x * B;
+    const uint32_t cells = n_items * B;
+    for (uint32_t i = tid; i < cells; i += blockDim.x) tile[i] = 0;
+    // [lo, hi): pairs whose row lies in this word range
+    const uint64_t r_lo = w0 * 64, r_hi = min((w0 + B) * 64, n_trans);
+    uint64_t lo = 0, hi = nnz;
+    while (lo < hi) {
+        const uint64_t m = (lo + hi) >> 1;
+        if (rows[m] < r_lo) lo = m + 1; else hi = m;
+    }
+    uint64_t lo2 = lo, hi2 = nnz;
+    while (lo2 < hi2) {
+        const uint64_t m = (lo2 + hi2) >> 1;
+        if (rows[m] < r_hi) lo2 = m + 1; else hi2 = m;
+    }
+    __syncthreads();
+    for (uint64_t i = lo + tid; i < lo2; i += blockDim.x) {
+        const uint64_t r = rows[i];
+        const uint32_t c = cols[i];
+        if (c >= n_items) {
+            atomicOr(bad, 1u);
+            continue;
+        }
+        atomicOr(&tile[c * B + (uint32_t)((r >> 6) - w0)], 1ull << (r & 63));
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < cells; i += blockDim.x) {
+        const uint32_t c = i / B, k = i % B;
+        if (w0 + k < stride) store[(uint64_t)c * stride + w0 + k] = tile[i];
+    }
+}
+
 }  // namespace cpm
 
 using namespace cpm;
@@ -166,7 +221,8 @@ int cpm_ctx_synchronize(cpm_ctx *ctx) {
 
 void *cpm_ctx_stream(cpm_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
-static int bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits **out) {
+static int bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits **out,
+                      bool clear = true) {
     cpm_bits *b = new cpm_bits();
     b->ctx = ctx;
     b->n_transactions = n_trans;
@@ -183,7 +239,7 @@ static int bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits
         delete b;
         return s;
     }
-    hipError_t e = hipMemsetAsync(b->store.p, 0, bytes ? bytes : 8, ctx->stream);
+    hipError_t e = clear ? hipMemsetAsync(b->store.p, 0, bytes ? bytes : 8, ctx->stream) : hipSuccess;
     if (e != hipSuccess) {
         delete b;
         return fail(CPM_ERR_RUNTIME, "hipMemsetAsync failed: %s", hipGetErrorString(e));
@@ -200,12 +256,51 @@ int cpm_bits_build_device(cpm_ctx *ctx, const uint64_t *d_rows, const uint32_t *
     *out = nullptr;
     if (nnz && (!d_rows || !d_cols)) return fail(CPM_ERR_INVALID, "rows/cols are NULL");
     if (n_trans >= (1ull << 40)) return fail(CPM_ERR_INVALID, "too many transactions");
+    // row-sorted input with a small enough item count takes the LDS tile build
+    const uint32_t tileB = n_items ? std::min<uint32_t>(8, 8192 / std::max<uint32_t>(n_items, 1)) : 0;
+    bool tiled = false;
+    CPM_CHECK(ctx->scratch[5].ensure(2 * sizeof(unsigned int)));
+    unsigned int *flags = ctx->scratch[5].as<unsigned int>();
+    if (nnz && tileB >= 1) {
+        unsigned int hf[2] = {0, 0};
+        CPM_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(unsigned int), ctx->stream));
+        const uint64_t g = std::min<uint64_t>((nnz + 255) / 256, (uint64_t)ctx->cu_count * 16);
+        hipLaunchKernelGGL(k_rows_sorted, dim3((unsigned)g), dim3(256), 0, ctx->stream, d_rows, nnz,
+                           flags);
+        CPM_HIP(hipGetLastError());
+        uint64_t last = 0;
+        CPM_HIP(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, ctx->stream));
+        CPM_HIP(hipMemcpyAsync(&last, d_rows + nnz - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+        CPM_HIP(hipStreamSynchronize(ctx->stream));
+        if (hf[0] == 0 && last >= n_trans)
+            return fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
+                        (unsigned long long)n_trans, n_items);
+        tiled = hf[0] == 0;
+    }
     cpm_bits *b = nullptr;
-    CPM_CHECK(bits_alloc(ctx, n_trans, n_items, &b));
-    if (nnz) {
-        int s = ctx->scratch[5].ensure(sizeof(unsigned int));
-        if (s != CPM_OK) { cpm_bits_destroy(b); return s; }
-        unsigned int *bad = ctx->scratch[5].as<unsigned int>();
+    CPM_CHECK(bits_alloc(ctx, n_trans, n_items, &b, !tiled));
+    if (tiled) {
+        unsigned int *bad = flags + 1;
+        const uint64_t blocks = (b->stride + tileB - 1) / tileB;
+        hipLaunchKernelGGL(k_bits_tile, dim3((unsigned)blocks), dim3(256),
+                           (size_t)n_items * tileB * 8, ctx->stream, d_rows, d_cols, nnz, n_trans,
+                           n_items, b->stride, tileB, b->store.as<uint64_t>(), bad);
+        hipError_t e = hipGetLastError();
+        unsigned int hbad = 0;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            cpm_bits_destroy(b);
+            return fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(e));
+        }
+        if (hbad) {
+            cpm_bits_destroy(b);
+            return fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
+                        (unsigned long long)n_trans, n_items);
+        }
+    } else if (nnz) {
+        unsigned int *bad = flags;
         hipError_t e = hipMemsetAsync(bad, 0, sizeof(unsigned int), ctx->stream);
         if (e == hipSuccess) {
             uint64_t blocks = std::min<uint64_t>((nnz + 255) / 256, (uint64_t)ctx->cu_count * 32);

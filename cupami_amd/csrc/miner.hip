@@ -23,6 +23,7 @@
 // prefix and every item; level 2 is built over the frequent items with pair
 // supports from the INT8 Gram (gram.hip) or from the support kernel.
 #include "support_kernels.h"
+#include "sort.h"
 
 #include <string.h>
 #include <algorithm>
@@ -179,6 +180,9 @@ struct PlanOut {
     const uint32_t *fcol;
     // next level
     uint32_t *prefix, *pid, *ext_off, *ext, *sup, *ngrp;
+    // this level's frequent itemsets, (d+1) items each, in pattern order
+    // (lexicographic), for the subset test of the next level; may be null
+    uint32_t *lv;
 };
 
 __global__ __launch_bounds__(256) void k_plan_write(Level L, Keep keep, PairTable pairs,
@@ -207,6 +211,12 @@ __global__ __launch_bounds__(256) void k_plan_write(Level L, Keep keep, PairTabl
             o.pat_item[pb + j] = o.fcol ? o.fcol[f] : f;
             o.pat_sup[pb + j] = keep.sup[e];
             if (o.pat_per) o.pat_per[pb + j] = keep.per[e];
+            if (o.lv) {
+                uint32_t *dst = o.lv + (uint64_t)(off_pat[t] + j) * d1;
+                const uint32_t *src = L.prefix + (uint64_t)t * L.d;
+                for (uint32_t i = 0; i < L.d; ++i) dst[i] = src[i];
+                dst[L.d] = f;
+            }
         }
         if (!(emit_children && nk >= 2)) continue;
         if (pairs.bits) {
@@ -270,6 +280,77 @@ __global__ __launch_bounds__(256) void k_plan_write(Level L, Keep keep, PairTabl
             __syncthreads();
         }
     }
+}
+
+// Subset test (cuApriori.py:487-490): candidate prefix(ct) + {x} of the next
+// level survives only if every subset that drops one item of the parent
+// prefix P is a frequent itemset of this level (the two subsets that drop the
+// last prefix item or x are frequent by construction).  lv holds this
+// level's itemsets in lexicographic order, d1 items each.
+// Is (prefix without item `drop`) + {x} -- d1 items, ascending -- in lv?
+__device__ __forceinline__ bool lv_contains(const uint32_t *__restrict__ lv, uint32_t n_lv, uint32_t d1,
+                                            const uint32_t *__restrict__ pre, uint32_t drop,
+                                            uint32_t x) {
+    uint32_t lo = 0, hi = n_lv;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t *m = lv + (uint64_t)mid * d1;
+        int c = 0;
+        for (uint32_t i = 0; i < d1 && c == 0; ++i) {
+            const uint32_t k = i < drop ? pre[i] : (i + 1 < d1 ? pre[i + 1] : x);
+            c = m[i] < k ? -1 : (m[i] > k ? 1 : 0);
+        }
+        if (c == 0) return true;
+        if (c < 0) lo = mid + 1; else hi = mid;
+    }
+    return false;
+}
+
+constexpr uint32_t kMaxPrefix = 64;
+
+__global__ __launch_bounds__(256) void k_prune_flags(const uint32_t *__restrict__ prefix, uint32_t d1,
+                                                    const uint32_t *__restrict__ ext_off,
+                                                    uint32_t n_child, uint64_t n_ext,
+                                                    const uint32_t *__restrict__ ext,
+                                                    const uint32_t *__restrict__ lv, uint32_t n_lv,
+                                                    uint32_t *__restrict__ live) {
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_ext;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        // task of entry w: largest ct with ext_off[ct] <= w
+        uint32_t lo = 0, hi = n_child;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (ext_off[mid] <= w) lo = mid; else hi = mid;
+        }
+        const uint32_t *pre = prefix + (uint64_t)lo * d1;
+        const uint32_t x = ext[w];
+        bool ok = true;
+        for (uint32_t drop = 0; ok && drop + 1 < d1; ++drop) ok = lv_contains(lv, n_lv, d1, pre, drop, x);
+        live[w] = ok ? 1u : 0u;
+    }
+}
+
+// new entry offsets and group counts after the prune
+__global__ void k_prune_offsets(const uint32_t *__restrict__ old_off, uint32_t n_child,
+                                const uint32_t *__restrict__ pos, uint32_t *__restrict__ new_off,
+                                uint32_t *__restrict__ ngrp) {
+    for (uint32_t ct = blockIdx.x * blockDim.x + threadIdx.x; ct <= n_child;
+         ct += gridDim.x * blockDim.x) {
+        const uint32_t a = pos[old_off[ct]];
+        new_off[ct] = a;
+        if (ct < n_child) ngrp[ct] = (pos[old_off[ct + 1]] - a + kGroup - 1) / kGroup;
+    }
+}
+
+__global__ void k_prune_compact(const uint32_t *__restrict__ ext, const uint32_t *__restrict__ live,
+                                const uint32_t *__restrict__ pos, uint64_t n,
+                                uint32_t *__restrict__ out, uint32_t *__restrict__ sup) {
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n;
+         w += (uint64_t)gridDim.x * blockDim.x)
+        if (live[w]) {
+            out[pos[w]] = ext[w];
+            sup[pos[w]] = 0;
+        }
 }
 
 // Exclusive scans of up to three uint32 arrays of length n (one array per
@@ -408,6 +489,8 @@ struct SearchWs {
     LevelBufs lv[2];
     Arena arena;
     DBuf kept, cnt[3], off[3], totals, fcol, pairbits, gram, parts, ts;
+    DBuf lvitems, live, pos, ext_tmp, off_tmp, ptotal;
+    SortWs sort;
     HBuf host;
 };
 
@@ -438,12 +521,14 @@ struct Miner {
     DBuf (&off)[3];
     DBuf &totals, &fcol, &pairbits, &gram, &parts;
     HBuf &host;
+    SearchWs &ws;
     uint32_t nf = 0;
+    bool subset_prune = true;
 
     Miner(cpm_ctx *c, SearchWs &w)
         : ctx(c), lv(w.lv), arena(w.arena), kept(w.kept), cnt(w.cnt), off(w.off),
           totals(w.totals), fcol(w.fcol), pairbits(w.pairbits), gram(w.gram), parts(w.parts),
-          host(w.host) {}
+          host(w.host), ws(w) {}
 
     uint32_t grid_tasks(uint32_t n) const {
         return std::max<uint32_t>(1, std::min<uint32_t>(n, (uint32_t)ctx->cu_count * 8));
@@ -525,7 +610,8 @@ struct Miner {
         unsigned long long *ht = host.as<unsigned long long>();
         CPM_HIP(hipMemcpyAsync(ht, totals.p, 3 * 8, hipMemcpyDeviceToHost, ctx->stream));
         CPM_HIP(hipStreamSynchronize(ctx->stream));
-        const uint64_t npat = ht[0], nchild = ht[1], ncext = ht[2];
+        const uint64_t npat = ht[0], nchild = ht[1];
+        uint64_t ncext = ht[2];
         if (arena.n + npat >= 0xffffffffull) return fail(CPM_ERR_NOMEM, "more than 2^32 patterns");
         if (ncext >= 0xffffffffull) return fail(CPM_ERR_NOMEM, "more than 2^32 candidates in a level");
         CPM_CHECK(arena_reserve(ctx, arena, arena.n + npat));
@@ -547,6 +633,13 @@ struct Miner {
             CPM_CHECK(N.sup.ensure(std::max<uint64_t>(ncext, 1) * 4));
             if (periodic) CPM_CHECK(N.per.ensure(std::max<uint64_t>(ncext, 1) * 8));
         }
+        // subset test for candidates of size >= 4 (size 3 is the pair table)
+        const bool prune = subset_prune && children && nchild && L.d >= 2 && d1 <= kMaxPrefix;
+        o.lv = nullptr;
+        if (prune) {
+            CPM_CHECK(ws.lvitems.ensure(std::max<uint64_t>(npat, 1) * d1 * 4));
+            o.lv = ws.lvitems.as<uint32_t>();
+        }
         o.prefix = N.prefix.as<uint32_t>();
         o.pid = N.pid.as<uint32_t>();
         o.ext_off = N.ext_off.as<uint32_t>();
@@ -562,6 +655,41 @@ struct Miner {
             hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, ctx->stream,
                                N.ext_off.as<uint32_t>() + nchild, (uint32_t)ncext);
             CPM_HIP(hipGetLastError());
+            uint64_t live_ext = ncext;
+            if (prune && ncext) {
+                CPM_CHECK(ws.live.ensure((ncext + 1) * 4));
+                CPM_CHECK(ws.pos.ensure((ncext + 1) * 4));
+                CPM_CHECK(ws.ptotal.ensure(16));
+                CPM_CHECK(ws.off_tmp.ensure((nchild + 1) * 4));
+                CPM_CHECK(ws.ext_tmp.ensure(ncext * 4));
+                hipLaunchKernelGGL(k_prune_flags,
+                                   dim3((unsigned)std::min<uint64_t>((ncext + 255) / 256, 16384)),
+                                   dim3(256), 0, ctx->stream, N.prefix.as<uint32_t>(), d1,
+                                   N.ext_off.as<uint32_t>(), (uint32_t)nchild, ncext,
+                                   N.ext.as<uint32_t>(), ws.lvitems.as<uint32_t>(), (uint32_t)npat,
+                                   ws.live.as<uint32_t>());
+                CPM_HIP(hipGetLastError());
+                CPM_CHECK(exclusive_scan_u32(ctx, ws.sort, ws.live.as<uint32_t>(), ws.pos.as<uint32_t>(),
+                                             ncext, ws.ptotal.as<uint64_t>()));
+                hipLaunchKernelGGL(k_prune_offsets, dim3((unsigned)((nchild + 256) / 256)), dim3(256), 0,
+                                   ctx->stream, N.ext_off.as<uint32_t>(), (uint32_t)nchild,
+                                   ws.pos.as<uint32_t>(), ws.off_tmp.as<uint32_t>(),
+                                   N.ngrp.as<uint32_t>());
+                hipLaunchKernelGGL(k_prune_compact, dim3((unsigned)std::min<uint64_t>((ncext + 255) / 256, 16384)),
+                                   dim3(256), 0, ctx->stream, N.ext.as<uint32_t>(), ws.live.as<uint32_t>(),
+                                   ws.pos.as<uint32_t>(), ncext, ws.ext_tmp.as<uint32_t>(),
+                                   N.sup.as<uint32_t>());
+                CPM_HIP(hipGetLastError());
+                unsigned long long *ht = host.as<unsigned long long>();
+                CPM_HIP(hipMemcpyAsync(ht, ws.ptotal.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+                CPM_HIP(hipStreamSynchronize(ctx->stream));
+                live_ext = ht[0];
+                std::swap(N.ext.p, ws.ext_tmp.p);
+                std::swap(N.ext.cap, ws.ext_tmp.cap);
+                std::swap(N.ext_off.p, ws.off_tmp.p);
+                std::swap(N.ext_off.cap, ws.off_tmp.cap);
+            }
+            ncext = live_ext;
             CPM_CHECK(scan(N.ngrp.as<uint32_t>(), N.grp_off.as<uint32_t>(), nullptr, nullptr,
                            nullptr, nullptr, 1, (uint32_t)nchild));
             N.d = d1;

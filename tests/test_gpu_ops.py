@@ -26,20 +26,36 @@ def random_store(seed, n, n_items, density):
 
 
 @pytest.mark.parametrize("n", [0, 1, 31, 63, 64, 65, 129, 4096, 100_003])
-def test_bits_build(ctx, n):
+@pytest.mark.parametrize("row_sorted", [True, False])
+def test_bits_build(ctx, n, row_sorted):
     rows, cols = random_store(n, n, 7, 0.3)
     # duplicates must OR together
     rows = np.concatenate([rows, rows[:5]])
     cols = np.concatenate([cols, cols[:5]])
+    if row_sorted:
+        order = np.argsort(rows, kind="stable")
+        rows, cols = rows[order], cols[order]
     bits = ctx.build_bits(rows, cols, n, 7)
     assert np.array_equal(bits.download(), oracle.bits_build(rows, cols, n, 7))
 
 
+def test_bits_build_many_items_and_reuse(ctx):
+    # item counts on both sides of the LDS tile limit, store buffers reused
+    for items in (1, 1000, 8192, 9000):
+        rows, cols = random_store(items, 3000, items, 0.002)
+        for _ in range(2):
+            bits = ctx.build_bits(rows, cols, 3000, items)
+            assert np.array_equal(bits.download(), oracle.bits_build(rows, cols, 3000, items))
+            bits.close()
+
+
 def test_bits_build_rejects_out_of_range(ctx):
-    with pytest.raises(ValueError):
-        ctx.build_bits(np.array([0, 5], np.uint64), np.array([0, 3], np.uint32), 5, 4)
-    with pytest.raises(ValueError):
-        ctx.build_bits(np.array([0, 1], np.uint64), np.array([0, 4], np.uint32), 5, 4)
+    for sorted_rows in ([0, 5], [5, 0]):
+        with pytest.raises(ValueError):
+            ctx.build_bits(np.array(sorted_rows, np.uint64), np.array([0, 3], np.uint32), 5, 4)
+    for rows in ([0, 1], [1, 0]):
+        with pytest.raises(ValueError):
+            ctx.build_bits(np.array(rows, np.uint64), np.array([0, 4], np.uint32), 5, 4)
 
 
 @pytest.mark.parametrize("n,width", [(70, 1), (70, 2), (5000, 3), (200_001, 4)])
