@@ -74,6 +74,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="eclat_1m_1k", choices=["eclat_1m_1k", "pfeclat_2m_500"])
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch level steps directly")
+    ap.add_argument("--group", type=int, default=32, help="entries per support work item")
+    ap.add_argument("--tile", type=int, default=2048, help="words per support tile (1024|2048)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -98,6 +101,7 @@ def main():
     d_cols = torch.from_numpy(cols.view(np.int32)).to(dev)
     torch.cuda.synchronize()
     ctx = _hip.context(0)
+    ctx.set_search_options(args.group, args.tile, not args.no_graph, True)
 
     def step():
         bits = ctx.build_bits_device(d_rows.data_ptr(), d_cols.data_ptr(), nnz, n, items)

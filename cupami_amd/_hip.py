@@ -66,6 +66,11 @@ SIGNATURES = {
     "cpm_fptree_destroy": (ctypes.c_int, [_vp]),
     "cpm_fpgrowth_mine": (ctypes.c_int, [_vp, _u64p, _u32p, ctypes.c_uint64, ctypes.c_uint64,
                                          ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "cpm_eclat_mine_batch": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _u64p, ctypes.c_uint32,
+                                            ctypes.c_int, ctypes.POINTER(_vp)]),
+    "cpm_set_search_options": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                              ctypes.c_int]),
+    "cpm_patterns_databases": (ctypes.c_int, [_vp, _u32p]),
     "cpm_patterns_count": (ctypes.c_int, [_vp, _u64p]),
     "cpm_patterns_download": (ctypes.c_int, [_vp, _u32p, _u32p, _u64p, _i64p]),
     "cpm_patterns_destroy": (ctypes.c_int, [_vp]),
@@ -234,6 +239,20 @@ class Context:
                                     1 if use_gram else 0, ctypes.byref(handle)))
         return Patterns(handle)
 
+    def eclat_mine_batch(self, bits_list, minsups, use_gram=True):
+        """One search over several databases; ``Patterns.db`` tells them apart."""
+        n = len(bits_list)
+        handles = (_vp * max(n, 1))(*[b.handle for b in bits_list])
+        ms = np.ascontiguousarray([clamp_u64(m) for m in minsups], dtype=np.uint64)
+        handle = _vp()
+        check(load().cpm_eclat_mine_batch(self.handle, handles, _p(ms, _u64p) if n else None, n,
+                                          1 if use_gram else 0, ctypes.byref(handle)))
+        return Patterns(handle, databases=True)
+
+    def set_search_options(self, group=32, tile_words=2048, use_graph=True, subset_prune=True):
+        check(load().cpm_set_search_options(self.handle, int(group), int(tile_words),
+                                            1 if use_graph else 0, 1 if subset_prune else 0))
+
     def pfeclat_mine(self, bits, timestamps, end, minsup, maxper_floor):
         ts = np.ascontiguousarray(timestamps, dtype=np.int64)
         if ts.size == 0:
@@ -366,7 +385,7 @@ class Bits:
 class Patterns:
     """Pattern forest returned by a search (``cpm_patterns``)."""
 
-    def __init__(self, handle, periodic=False):
+    def __init__(self, handle, periodic=False, databases=False):
         lib = load()
         n = ctypes.c_uint64()
         try:
@@ -376,10 +395,13 @@ class Patterns:
             self.item = np.zeros(self.n, dtype=np.uint32)
             self.support = np.zeros(self.n, dtype=np.uint64)
             self.period = np.zeros(self.n, dtype=np.int64) if periodic else None
+            self.db = np.zeros(self.n, dtype=np.uint32)
             if self.n:
                 check(lib.cpm_patterns_download(
                     handle, _p(self.parent, _u32p), _p(self.item, _u32p),
                     _p(self.support, _u64p), _p(self.period, _i64p) if periodic else None))
+                if databases:
+                    check(lib.cpm_patterns_databases(handle, _p(self.db, _u32p)))
         finally:
             lib.cpm_patterns_destroy(handle)
         self.periodic = periodic
@@ -395,6 +417,14 @@ class Patterns:
         for i in range(self.n):
             p = parent[i]
             out[i] = (item[i],) if p == NO_PARENT else out[p] + (item[i],)
+        return out
+
+    def split(self, n_db):
+        """Per-database ``{column tuple: support}`` dictionaries."""
+        sets = self.itemsets()
+        out = [dict() for _ in range(n_db)]
+        for s, v, d in zip(sets, self.support.tolist(), self.db.tolist()):
+            out[d][s] = int(v)
         return out
 
     def to_dict(self, labels=None):

@@ -384,55 +384,6 @@ int cpm_bits_download(cpm_ctx *ctx, const cpm_bits *b, uint64_t *host) {
 
 // ---- operators ----------------------------------------------------------------
 
-int cpm_count_supports(cpm_ctx *ctx, const cpm_bits *b, const uint32_t *cand, uint64_t n_cand,
-                       uint32_t width, uint64_t *supports) {
-    clear_error();
-    CPM_CHECK(bind(ctx));
-    if (!b) return fail(CPM_ERR_INVALID, "null store");
-    if (n_cand == 0) return CPM_OK;
-    if (!cand || !supports) return fail(CPM_ERR_INVALID, "null candidates/supports");
-    if (width == 0) return fail(CPM_ERR_INVALID, "candidate width must be >= 1");
-    if (n_cand >= (1ull << 31)) return fail(CPM_ERR_INVALID, "too many candidates in one call");
-    for (uint64_t i = 0; i < n_cand * width; ++i)
-        if (cand[i] >= b->n_items)
-            return fail(CPM_ERR_INVALID, "candidate column %u >= %u items", cand[i], b->n_items);
-    // One task per candidate: prefix = its first width-1 columns, one extension.
-    const uint32_t d = width - 1;
-    std::vector<uint32_t> prefix((size_t)n_cand * (d ? d : 1)), off(n_cand + 1), ext(n_cand);
-    for (uint64_t c = 0; c < n_cand; ++c) {
-        for (uint32_t j = 0; j < d; ++j) prefix[c * d + j] = cand[c * width + j];
-        ext[c] = cand[c * width + d];
-        off[c] = (uint32_t)c;
-    }
-    off[n_cand] = (uint32_t)n_cand;
-    DBuf &dp = ctx->scratch[0], &doff = ctx->scratch[1], &dext = ctx->scratch[2],
-         &dsup = ctx->scratch[3];
-    CPM_CHECK(dp.ensure(prefix.size() * 4));
-    CPM_CHECK(doff.ensure(off.size() * 4));
-    CPM_CHECK(dext.ensure(ext.size() * 4));
-    CPM_CHECK(dsup.ensure(n_cand * 4));
-    uint32_t nt = (uint32_t)n_cand;
-    CPM_HIP(hipMemcpyAsync(dp.p, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    CPM_HIP(hipMemcpyAsync(doff.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    CPM_HIP(hipMemcpyAsync(dext.p, ext.data(), ext.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    CPM_HIP(hipMemsetAsync(dsup.p, 0, n_cand * 4, ctx->stream));
-    TaskView tv;
-    tv.prefix = dp.as<uint32_t>();
-    tv.d = d;
-    tv.ext_off = doff.as<uint32_t>();
-    tv.ext = dext.as<uint32_t>();
-    tv.grp_off = doff.as<uint32_t>();  // one extension per task -> one group per task
-    tv.n_tasks = nt;
-    tv.fcol = nullptr;
-    tv.G = 1;
-    CPM_CHECK(launch_support_count(ctx, b, tv, n_cand, dsup.as<uint32_t>()));
-    std::vector<uint32_t> s32(n_cand);
-    CPM_HIP(hipMemcpyAsync(s32.data(), dsup.p, n_cand * 4, hipMemcpyDeviceToHost, ctx->stream));
-    CPM_HIP(hipStreamSynchronize(ctx->stream));
-    for (uint64_t c = 0; c < n_cand; ++c) supports[c] = s32[c];
-    return CPM_OK;
-}
-
 int cpm_intersect_extensions(cpm_ctx *ctx, const uint64_t *prefix, const uint64_t *ext, uint32_t k,
                              uint64_t words, uint64_t keep_min, uint64_t *supports,
                              uint64_t *kept_rows, uint32_t *kept_index, uint32_t *n_kept) {
@@ -476,65 +427,6 @@ int cpm_intersect_extensions(cpm_ctx *ctx, const uint64_t *prefix, const uint64_
     }
     CPM_HIP(hipStreamSynchronize(ctx->stream));
     if (n_kept) *n_kept = nk;
-    return CPM_OK;
-}
-
-int cpm_periodic_eval(cpm_ctx *ctx, const cpm_bits *b, const int64_t *timestamps, int64_t end,
-                      const uint32_t *cand, uint64_t n_cand, uint32_t width, uint64_t *supports,
-                      int64_t *periods) {
-    clear_error();
-    CPM_CHECK(bind(ctx));
-    if (!b) return fail(CPM_ERR_INVALID, "null store");
-    if (n_cand == 0) return CPM_OK;
-    if (!cand || !supports || !periods) return fail(CPM_ERR_INVALID, "null argument");
-    if (b->n_transactions && !timestamps) return fail(CPM_ERR_INVALID, "null timestamps");
-    if (width == 0) return fail(CPM_ERR_INVALID, "candidate width must be >= 1");
-    if (n_cand >= (1ull << 31)) return fail(CPM_ERR_INVALID, "too many candidates in one call");
-    for (uint64_t i = 0; i < n_cand * width; ++i)
-        if (cand[i] >= b->n_items)
-            return fail(CPM_ERR_INVALID, "candidate column %u >= %u items", cand[i], b->n_items);
-    const uint32_t d = width - 1;
-    std::vector<uint32_t> prefix((size_t)n_cand * (d ? d : 1)), off(n_cand + 1), ext(n_cand);
-    for (uint64_t c = 0; c < n_cand; ++c) {
-        for (uint32_t j = 0; j < d; ++j) prefix[c * d + j] = cand[c * width + j];
-        ext[c] = cand[c * width + d];
-        off[c] = (uint32_t)c;
-    }
-    off[n_cand] = (uint32_t)n_cand;
-    DBuf tsb, parts, per;
-    DBuf &dp = ctx->scratch[0], &doff = ctx->scratch[1], &dext = ctx->scratch[2],
-         &dsup = ctx->scratch[3];
-    const uint64_t n = b->n_transactions;
-    CPM_CHECK(tsb.ensure((n ? n : 1) * 8));
-    CPM_CHECK(dp.ensure(prefix.size() * 4));
-    CPM_CHECK(doff.ensure(off.size() * 4));
-    CPM_CHECK(dext.ensure(ext.size() * 4));
-    CPM_CHECK(dsup.ensure(n_cand * 4));
-    CPM_CHECK(per.ensure(n_cand * 8));
-    const uint32_t n_chunks = (uint32_t)((b->stride + kTileWords - 1) / kTileWords);
-    CPM_CHECK(parts.ensure((size_t)n_cand * n_chunks * sizeof(PeriodPart)));
-    uint32_t nt = (uint32_t)n_cand;
-    if (n) CPM_HIP(hipMemcpyAsync(tsb.p, timestamps, n * 8, hipMemcpyHostToDevice, ctx->stream));
-    CPM_HIP(hipMemcpyAsync(dp.p, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    CPM_HIP(hipMemcpyAsync(doff.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    CPM_HIP(hipMemcpyAsync(dext.p, ext.data(), ext.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    TaskView tv;
-    tv.prefix = dp.as<uint32_t>();
-    tv.d = d;
-    tv.ext_off = doff.as<uint32_t>();
-    tv.ext = dext.as<uint32_t>();
-    tv.grp_off = doff.as<uint32_t>();
-    tv.n_tasks = nt;
-    tv.fcol = nullptr;
-    tv.G = 1;
-    CPM_CHECK(launch_support_periodic(ctx, b, tv, n_cand, tsb.as<int64_t>(), end,
-                                      parts.as<PeriodPart>(), dsup.as<uint32_t>(),
-                                      per.as<int64_t>(), dext.as<uint32_t>() /*unused*/, 0));
-    std::vector<uint32_t> s32(n_cand);
-    CPM_HIP(hipMemcpyAsync(s32.data(), dsup.p, n_cand * 4, hipMemcpyDeviceToHost, ctx->stream));
-    CPM_HIP(hipMemcpyAsync(periods, per.p, n_cand * 8, hipMemcpyDeviceToHost, ctx->stream));
-    CPM_HIP(hipStreamSynchronize(ctx->stream));
-    for (uint64_t c = 0; c < n_cand; ++c) supports[c] = s32[c];
     return CPM_OK;
 }
 

@@ -78,6 +78,9 @@ struct cpm_ctx {
     void (*search_ws_free)(void *) = nullptr;
     // allocation of the last destroyed store, reused by the next store build
     cpm::DBuf spare_store;
+    // search schedule (cpm_set_search_options)
+    uint32_t opt_group = 32, opt_tile_words = 2048;
+    bool opt_graph = true, opt_prune = true;
 };
 
 struct cpm_bits {
@@ -92,7 +95,7 @@ struct cpm_bits {
 struct cpm_patterns {
     uint64_t n = 0;
     bool periodic = false;
-    std::vector<uint32_t> parent, item;
+    std::vector<uint32_t> parent, item, db;
     std::vector<uint64_t> support;
     std::vector<int64_t> period;
 };
@@ -125,9 +128,59 @@ struct Timed {
 
 struct TaskLevel;  // miner.hip
 
-// gram.hip: out[a*ld + b] = |row(cols[a]) & row(cols[b])| for a <= b (int32),
-// computed for the upper block triangle; cols are device store columns.
-int launch_gram_i8(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t *d_cols, uint32_t n,
-                   int32_t *d_out, uint32_t ld);
+// gram.hip ------------------------------------------------------------------
+// One database of a batched pair-count launch: G[a][b] = |row(cols[a]) &
+// row(cols[b])| for a <= b, written to out + out_off with leading dimension
+// n_items (int32).
+// Pointer into global memory.  Row pointers that a kernel reads out of a
+// table (not from its arguments) are generic to the compiler, which then
+// emits flat loads: those share the LDS wait counter, so every wait on them
+// also drains LDS traffic.  Casting to the global address space restores
+// global loads.
+//
+// Pointer into the constant address space: memory that no lane writes while
+// the kernel runs.  Loads from it at wave-uniform addresses compile to scalar
+// loads, whatever else the kernel does (a clock read or a global store
+// ahead of a loop otherwise makes the compiler assume the memory may have
+// changed and fall back to vector loads).
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using gptr = const __attribute__((address_space(1))) T *;
+template <class T>
+using cptr = const __attribute__((address_space(4))) T *;
+#else
+template <class T>
+using gptr = const T *;  // host pass: kernels are parsed, never run
+template <class T>
+using cptr = const T *;
+#endif
+
+template <class T>
+__device__ __forceinline__ gptr<T> to_global(const T *p) {
+    return (gptr<T>)p;
+}
+
+template <class T>
+__device__ __forceinline__ cptr<T> to_const(const T *p) {
+    return (cptr<T>)p;
+}
+
+struct GramDb {
+    const uint64_t *store;
+    uint64_t stride, words, n_transactions;
+    const uint32_t *cols;
+    uint32_t n_items, pad;
+    uint64_t out_off;
+};
+struct GramJob {
+    uint32_t db, bi, bj, pad;
+    uint64_t k_begin, k_end;
+};
+// Work table for a batch of databases; *out_elems = sum of n_items^2.
+int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<GramJob> &jobs,
+              uint64_t *out_elems);
+// Zero `out` and run the jobs (device tables); n_ops feeds the statistics.
+int launch_gram_i8(cpm_ctx *ctx, const GramJob *d_jobs, uint64_t n_jobs, const GramDb *d_dbs,
+                   int32_t *d_out, uint64_t out_elems, uint64_t n_ops);
 
 }  // namespace cpm

@@ -10,7 +10,9 @@
 // them into 16 bytes with shift-and-mask; A (row items) and B (column items)
 // use the same lane -> bit map, so every bit is paired with itself exactly
 // once whatever order the hardware gives the K slots.  Accumulation is int32:
-// exact while N < 2^31.  Slices are summed with integer atomics.
+// exact while N < 2^31.  Slices are summed with integer atomics.  One launch
+// serves any number of databases: the job table lists each workgroup's
+// (database, block row, block column, K range).
 #include "cpm_internal.h"
 
 #include <algorithm>
@@ -36,42 +38,42 @@ __device__ __forceinline__ v4i spread_bits(uint64_t w, uint32_t g) {
     return r;
 }
 
-// One workgroup: a 128x128 block of G (bi <= bj) over one K slice.  Each
-// wave owns a 64x64 quadrant as 4x4 MFMA tiles; the next K tile is loaded
-// into registers while the current one is multiplied out of LDS.
-__global__ __launch_bounds__(256) void k_gram_i8(const uint64_t *__restrict__ store,
-                                                uint64_t stride, uint64_t words,
-                                                const uint32_t *__restrict__ cols, uint32_t n,
-                                                uint32_t nb, uint32_t slices,
-                                                uint64_t slice_words, int32_t *__restrict__ out,
-                                                uint32_t ld) {
+// One workgroup: a 128x128 block of G (bi <= bj) over one K slice of one
+// database, as listed in the job table.  Each wave owns a 64x64 quadrant as
+// 4x4 MFMA tiles; the next K tile is loaded into registers while the current
+// one is multiplied out of LDS.
+__global__ __launch_bounds__(256) void k_gram_i8(const GramJob *__restrict__ jobs,
+                                                const GramDb *__restrict__ dbs,
+                                                int32_t *__restrict__ out) {
     __shared__ uint64_t sA[kBT][kKW + 1];
     __shared__ uint64_t sB[kBT][kKW + 1];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t slice = blockIdx.x % slices;
-    uint32_t tri = blockIdx.x / slices;
-    uint32_t bi = 0;
-    while (tri >= nb - bi) {
-        tri -= nb - bi;
-        ++bi;
-    }
-    const uint32_t bj = bi + tri;
-    const uint64_t k_begin = (uint64_t)slice * slice_words;
-    const uint64_t k_end = std::min<uint64_t>(words, k_begin + slice_words);
+    const GramJob job = jobs[blockIdx.x];
+    const GramDb db = dbs[job.db];
+    const gptr<uint64_t> store = to_global(db.store);
+    const uint64_t stride = db.stride;
+    const uint32_t n = db.n_items;
+    const gptr<uint32_t> cols = to_global(db.cols);
+    const uint32_t bi = job.bi, bj = job.bj;
+    const uint64_t k_begin = job.k_begin, k_end = job.k_end;
+    int32_t *res = out + db.out_off;
+    const uint32_t ld = n;
 
     // loader: lane pair (2r, 2r+1) stages row r, 8 words each
     const uint32_t lrow = tid >> 1, lw = (tid & 1) * 8;
     const uint32_t ia = bi * kBT + lrow, ib = bj * kBT + lrow;
-    const uint64_t *pa = ia < n ? store + (uint64_t)cols[ia] * stride : nullptr;
-    const uint64_t *pb = ib < n ? store + (uint64_t)cols[ib] * stride : nullptr;
+    const gptr<uint4> pa = ia < n ? (gptr<uint4>)(store + (uint64_t)cols[ia] * stride) : nullptr;
+    const gptr<uint4> pb = ib < n ? (gptr<uint4>)(store + (uint64_t)cols[ib] * stride) : nullptr;
     uint4 ra[4], rb[4];
     auto fetch = [&](uint64_t k0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const uint64_t w = k0 + lw + 2 * q;  // even; w + 1 < stride (zero padded)
             const bool in = w < k_end;
-            ra[q] = (pa && in) ? *reinterpret_cast<const uint4 *>(pa + w) : make_uint4(0, 0, 0, 0);
-            rb[q] = (pb && in) ? *reinterpret_cast<const uint4 *>(pb + w) : make_uint4(0, 0, 0, 0);
+            ra[q] = make_uint4(0, 0, 0, 0);
+            rb[q] = make_uint4(0, 0, 0, 0);
+            if (pa && in) ra[q] = pa[w / 2];
+            if (pb && in) rb[q] = pb[w / 2];
         }
     };
     auto stash = [&]() {
@@ -90,9 +92,9 @@ __global__ __launch_bounds__(256) void k_gram_i8(const uint64_t *__restrict__ st
     const bool idle = (bi == bj) && (wr > wc);
     v4i acc[4][4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = (v4i){0, 0, 0, 0};
+        for (int y = 0; y < 4; ++y) acc[x][y] = (v4i){0, 0, 0, 0};
 
     if (k_begin < k_end) fetch(k_begin);
     for (uint64_t k0 = k_begin; k0 < k_end; k0 += kKW) {
@@ -109,10 +111,10 @@ __global__ __launch_bounds__(256) void k_gram_i8(const uint64_t *__restrict__ st
                     fb[t] = spread_bits(sB[wc * 64 + t * 16 + li][w], lg);
                 }
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
+                for (int x = 0; x < 4; ++x)
 #pragma unroll
-                    for (int b = 0; b < 4; ++b)
-                        acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
+                    for (int y = 0; y < 4; ++y)
+                        acc[x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[x], fb[y], acc[x][y], 0, 0, 0);
             }
         }
         __syncthreads();
@@ -120,47 +122,63 @@ __global__ __launch_bounds__(256) void k_gram_i8(const uint64_t *__restrict__ st
     if (idle) return;
     // D tile layout: lane holds rows 4*(lane/16) + r, column lane % 16
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const uint32_t col = bj * kBT + wc * 64 + b * 16 + li;
+        for (int y = 0; y < 4; ++y) {
+            const uint32_t col = bj * kBT + wc * 64 + y * 16 + li;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const uint32_t row = bi * kBT + wr * 64 + a * 16 + lg * 4 + r;
-                const int v = acc[a][b][r];
-                if (row < n && col < n && row <= col && v) atomicAdd(out + (uint64_t)row * ld + col, v);
+                const uint32_t row = bi * kBT + wr * 64 + x * 16 + lg * 4 + r;
+                const int v = acc[x][y][r];
+                if (row < n && col < n && row <= col && v) atomicAdd(res + (uint64_t)row * ld + col, v);
             }
         }
 }
 
 }  // namespace
 
-int launch_gram_i8(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t *d_cols, uint32_t n,
-                   int32_t *d_out, uint32_t ld) {
-    if (n == 0) return CPM_OK;
-    if (bits->n_transactions >= (1ull << 31))
-        return fail(CPM_ERR_INVALID, "int32 pair counts need fewer than 2^31 transactions");
-    CPM_HIP(hipMemsetAsync(d_out, 0, (size_t)n * ld * sizeof(int32_t), ctx->stream));
-    if (bits->words == 0) return CPM_OK;
-    const uint32_t nb = (n + kBT - 1) / kBT;
-    const uint64_t tri = (uint64_t)nb * (nb + 1) / 2;
+int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<GramJob> &jobs,
+              uint64_t *out_elems) {
+    jobs.clear();
+    uint64_t tri_total = 0, out = 0;
+    for (const GramDb &d : dbs) {
+        const uint64_t nb = (d.n_items + kBT - 1) / kBT;
+        tri_total += nb * (nb + 1) / 2;
+        out += (uint64_t)d.n_items * d.n_items;
+    }
+    *out_elems = out;
+    // enough workgroups to fill the device: split K when there are few blocks
     const uint64_t want = (uint64_t)ctx->cu_count * 2;
-    uint64_t slices = std::max<uint64_t>(1, (want + tri - 1) / tri);
-    const uint64_t ktiles = (bits->words + kKW - 1) / kKW;
-    slices = std::min<uint64_t>(slices, ktiles);
-    const uint64_t slice_words = (ktiles + slices - 1) / slices * kKW;
-    slices = (bits->words + slice_words - 1) / slice_words;
-    const uint64_t grid = tri * slices;
-    if (grid >= (1ull << 31)) return fail(CPM_ERR_INVALID, "pair block grid too large");
+    for (uint32_t i = 0; i < dbs.size(); ++i) {
+        const GramDb &d = dbs[i];
+        if (d.n_items == 0 || d.words == 0) continue;
+        if (d.n_transactions >= (1ull << 31))
+            return fail(CPM_ERR_INVALID, "int32 pair counts need fewer than 2^31 transactions");
+        const uint32_t nb = (d.n_items + kBT - 1) / kBT;
+        const uint64_t ktiles = (d.words + kKW - 1) / kKW;
+        uint64_t slices = tri_total ? std::max<uint64_t>(1, (want + tri_total - 1) / tri_total) : 1;
+        slices = std::min<uint64_t>(slices, ktiles);
+        const uint64_t slice_words = (ktiles + slices - 1) / slices * kKW;
+        for (uint32_t bi = 0; bi < nb; ++bi)
+            for (uint32_t bj = bi; bj < nb; ++bj)
+                for (uint64_t k0 = 0; k0 < d.words; k0 += slice_words)
+                    jobs.push_back({i, bi, bj, 0, k0, std::min<uint64_t>(d.words, k0 + slice_words)});
+    }
+    if (jobs.size() >= (1ull << 31)) return fail(CPM_ERR_INVALID, "pair block grid too large");
+    return CPM_OK;
+}
+
+int launch_gram_i8(cpm_ctx *ctx, const GramJob *d_jobs, uint64_t n_jobs, const GramDb *d_dbs,
+                   int32_t *d_out, uint64_t out_elems, uint64_t n_ops) {
+    CPM_HIP(hipMemsetAsync(d_out, 0, out_elems * sizeof(int32_t), ctx->stream));
+    if (n_jobs == 0) return CPM_OK;
     Timed timer(ctx, 5);
-    hipLaunchKernelGGL(k_gram_i8, dim3((unsigned)grid), dim3(256), 0, ctx->stream,
-                       bits->store.as<const uint64_t>(), bits->stride, bits->words, d_cols, n, nb,
-                       (uint32_t)slices, slice_words, d_out, ld);
+    hipLaunchKernelGGL(k_gram_i8, dim3((unsigned)n_jobs), dim3(256), 0, ctx->stream, d_jobs, d_dbs,
+                       d_out);
     CPM_HIP(hipGetLastError());
     CPM_CHECK(timer.finish());
     ctx->stats[4] += 1;
-    // 2 ops per multiply-add over the upper triangle including the diagonal
-    ctx->stats[6] += (double)bits->n_transactions * (double)n * (double)(n + 1);
+    ctx->stats[6] += (double)n_ops;
     return CPM_OK;
 }
 
@@ -178,11 +196,30 @@ extern "C" int cpm_pair_gram_i8(cpm_ctx *ctx, const cpm_bits *bits, const uint32
     for (uint32_t i = 0; i < n; ++i)
         if (cols[i] >= bits->n_items)
             return fail(CPM_ERR_INVALID, "column %u >= %u items", cols[i], bits->n_items);
-    DBuf dc, dout;
+    DBuf dc, dout, djobs, ddb;
     CPM_CHECK(dc.ensure((size_t)n * 4));
-    CPM_CHECK(dout.ensure((size_t)n * n * 4));
     CPM_HIP(hipMemcpyAsync(dc.p, cols, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
-    CPM_CHECK(launch_gram_i8(ctx, bits, dc.as<uint32_t>(), n, dout.as<int32_t>(), n));
+    std::vector<GramDb> dbs(1);
+    dbs[0].store = bits->store.as<const uint64_t>();
+    dbs[0].stride = bits->stride;
+    dbs[0].words = bits->words;
+    dbs[0].n_transactions = bits->n_transactions;
+    dbs[0].cols = dc.as<uint32_t>();
+    dbs[0].n_items = n;
+    dbs[0].out_off = 0;
+    std::vector<GramJob> jobs;
+    uint64_t elems = 0;
+    CPM_CHECK(gram_plan(ctx, dbs, jobs, &elems));
+    CPM_CHECK(dout.ensure(elems * 4));
+    CPM_CHECK(djobs.ensure(std::max<size_t>(jobs.size(), 1) * sizeof(GramJob)));
+    CPM_CHECK(ddb.ensure(sizeof(GramDb)));
+    if (!jobs.empty())
+        CPM_HIP(hipMemcpyAsync(djobs.p, jobs.data(), jobs.size() * sizeof(GramJob),
+                               hipMemcpyHostToDevice, ctx->stream));
+    CPM_HIP(hipMemcpyAsync(ddb.p, dbs.data(), sizeof(GramDb), hipMemcpyHostToDevice, ctx->stream));
+    const uint64_t ops = (uint64_t)bits->n_transactions * n * (uint64_t)(n + 1);
+    CPM_CHECK(launch_gram_i8(ctx, djobs.as<GramJob>(), jobs.size(), ddb.as<GramDb>(),
+                             dout.as<int32_t>(), elems, ops));
     CPM_HIP(hipMemcpyAsync(gram, dout.p, (size_t)n * n * 4, hipMemcpyDeviceToHost, ctx->stream));
     CPM_HIP(hipStreamSynchronize(ctx->stream));
     return CPM_OK;

@@ -114,6 +114,20 @@ int cpm_periodic_eval(cpm_ctx *ctx, const cpm_bits *bits, const int64_t *timesta
 int cpm_eclat_mine(cpm_ctx *ctx, const cpm_bits *bits, uint64_t minsup, int use_gram,
                    cpm_patterns **out);
 
+/* Many databases in one search: the levels of every database share each
+ * launch (batched, persistent support kernel; one INT8 Gram launch for all
+ * level-2 matrices).  bits[i] is mined at minsup[i]; the patterns of all
+ * databases come back in one result, cpm_patterns_databases tells them
+ * apart.  Results equal n_db separate cpm_eclat_mine calls. */
+int cpm_eclat_mine_batch(cpm_ctx *ctx, const cpm_bits *const *bits, const uint64_t *minsup,
+                         uint32_t n_db, int use_gram, cpm_patterns **out);
+
+/* Search schedule of this context: entries per support work item (1..64),
+ * words per tile (1024 or 2048), replay of the level step as a hipGraph
+ * (use_graph), and the subset test of candidates before counting. */
+int cpm_set_search_options(cpm_ctx *ctx, uint32_t group, uint32_t tile_words, int use_graph,
+                           int subset_prune);
+
 /* Replaces cuPFEclat.mine (cuPFEclat.py:146-175): every itemset with
  * support >= minsup, support > 0 and periodicity <= maxper_floor (an integer
  * period p satisfies p <= m for a real m >= 0 iff p <= floor(m)).
@@ -166,6 +180,8 @@ int cpm_patterns_count(const cpm_patterns *p, uint64_t *n);
 int cpm_patterns_download(const cpm_patterns *p, uint32_t *parent, uint32_t *item,
                           uint64_t *support, int64_t *period);
 int cpm_patterns_destroy(cpm_patterns *p);
+/* Database index of each pattern (0 for single-database results). */
+int cpm_patterns_databases(const cpm_patterns *p, uint32_t *db);
 
 /* ---- measurement -------------------------------------------------------- */
 

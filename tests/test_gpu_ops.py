@@ -151,3 +151,46 @@ def test_pfeclat_mine_random(ctx, seed):
     for ms, per in [(1, 400), (3, 60), (5, 30), (20, 200)]:
         expect = oracle.pfeclat(host, ts, n, end, ms, per)
         assert ctx.pfeclat_mine(bits, ts, end, ms, per).to_dict() == expect, (ms, per)
+
+
+@pytest.mark.parametrize("graph,prune,group,tile", [(True, True, 32, 2048), (False, True, 16, 1024),
+                                                      (True, False, 64, 1024), (False, False, 8, 2048)])
+def test_search_schedules_agree(ctx, graph, prune, group, tile):
+    """Every schedule (graph replay or direct launches, subset test on or
+    off, group and tile sizes) gives the oracle's forest, several times in a
+    row on the same buffers."""
+    from cupami_amd import datagen
+    db = datagen.transactional(30_000, 80, 7.0, 1.0, seed=21)
+    labels, cols = datagen.label_columns(db.items)
+    bits = ctx.build_bits(db.rows, cols, db.n_transactions, len(labels))
+    host = oracle.bits_build(db.rows, cols, db.n_transactions, len(labels))
+    try:
+        ctx.set_search_options(group, tile, graph, prune)
+        for ms in (90, 300, 90):
+            expect = oracle.eclat(host, ms)
+            assert ctx.eclat_mine(bits, ms, use_gram=True).to_dict() == expect
+            assert ctx.eclat_mine(bits, ms, use_gram=False).to_dict() == expect
+    finally:
+        ctx.set_search_options()
+
+
+def test_batch_equals_separate_searches(ctx):
+    from cupami_amd import datagen
+    rng = np.random.default_rng(33)
+    stores, hosts, mins = [], [], []
+    for i in range(12):
+        n = int(rng.integers(200, 20_000))
+        items = int(rng.integers(3, 60))
+        db = datagen.transactional(n, items, float(rng.uniform(2, 6)), 1.0, seed=1000 + i)
+        labels, cols = datagen.label_columns(db.items)
+        stores.append(ctx.build_bits(db.rows, cols, db.n_transactions, len(labels)))
+        hosts.append(oracle.bits_build(db.rows, cols, db.n_transactions, len(labels)))
+        mins.append(max(1, int(n * rng.uniform(0.005, 0.05))))
+    got = ctx.eclat_mine_batch(stores, mins).split(len(stores))
+    for i in range(len(stores)):
+        assert got[i] == oracle.eclat(hosts[i], mins[i]), i
+    # and again, reusing the captured graphs with other thresholds
+    mins2 = [m + 3 for m in mins]
+    got = ctx.eclat_mine_batch(stores, mins2, use_gram=False).split(len(stores))
+    for i in range(len(stores)):
+        assert got[i] == oracle.eclat(hosts[i], mins2[i]), i
