@@ -3,27 +3,29 @@
 // The search is level-synchronous over prefix tasks.  A task is a frequent
 // prefix P (|P| = d) of one database with a list of extension items E; entry
 // e of E is the candidate P + {e}.  After levels 1 and 2 are set up, one
-// level step is a fixed sequence of launches whose sizes all live in device
-// memory:
+// level step is a fixed sequence of six launches whose sizes all live in
+// device memory:
 //
-//   plan/1   per task: compact the kept entries K; count patterns, child
-//            tasks (K_j, j < |K|-1) and child entries {K_l : l > j, pair
-//            (K_j, K_l) kept}
-//   scan     offsets of patterns / child tasks / child entries
-//   alloc    one lane: check the capacities, size the next level
-//   plan/2   write the patterns (parent id, item, support) and next level
-//   prune    drop child candidates with an infrequent subset (binary search
-//            in this level's sorted itemsets), compact the survivors
-//   scan     support work offsets
-//   support  count the next level (support.hip)
+//   plan/count    per task: compact the kept entries K; count patterns, child
+//                 tasks (K_j, j < |K|-1) and child entries {K_l : l > j, pair
+//                 (K_j, K_l) kept}; per-block sums over a contiguous chunk
+//   plan/offsets  offsets of patterns / child tasks / child entries (a block
+//                 adds the sums of the chunks before its own); the last block
+//                 checks the capacities and sizes the next level
+//   plan/write    write the patterns (parent id, item, support) and the next
+//                 level (to staging when the subset test follows)
+//   prune/flags   test each staged candidate's subsets (binary search in this
+//                 level's sorted itemsets); per-block survivor counts
+//   prune/compact move the survivors into place, child entry offsets
+//   support       count the next level (support.hip)
 //
 // so the step can be captured once as a hipGraph and replayed level after
-// level with no host round trip.  The children of a task are cuEclat's child
-// prefix classes (cuEclat.py:211-245); the join/prune is cuApriori's
-// (cuApriori.py:473-492).  Pruning removes only candidates that cannot be
-// frequent, so the itemsets are the reference's.  When a level outgrows a
-// capacity, the step raises an overflow flag, later steps do nothing, and the
-// host reruns the search with larger buffers.
+// level with no host round trip, and no block ever waits for another.  The
+// children of a task are cuEclat's child prefix classes (cuEclat.py:211-245);
+// the join/prune is cuApriori's (cuApriori.py:473-492).  Pruning removes only
+// candidates that cannot be frequent, so the itemsets are the reference's.
+// When a level outgrows a capacity, the step raises an overflow flag, later
+// steps do nothing, and the host reruns the search with larger buffers.
 #include "support_kernels.h"
 #include "sort.h"
 
@@ -36,25 +38,20 @@ namespace {
 
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kMaskWords = 2048;   // LDS kept mask: up to 65536 frequent items per database
-constexpr uint32_t kScanBlocks = 256;
 
 enum Overflow : uint32_t {
-    kOvTasks = 1, kOvEntries = 2, kOvPatterns = 4, kOvDepth = 8, kOvLevelSets = 16, kOvParts = 32
+    kOvTasks = 1, kOvEntries = 2, kOvPatterns = 4, kOvDepth = 8, kOvLevelSets = 16, kOvParts = 32,
+    kOvIndex = 64  // a level with 2^32 or more entries
 };
 
 // Control block in device memory.
 struct Ctl {
     LevelSizes sz[2];
+    SupportAcc acc;                  // support launch totals
     unsigned long long n_patterns;   // patterns written so far
     unsigned long long pat_base;     // first pattern id of the level being planned
     unsigned int overflow;           // Overflow bits
     unsigned int pad;
-    unsigned long long live_total;   // survivors of the subset test
-    unsigned long long clock[2];     // support launch span (earliest start, latest end)
-    unsigned long long clock_ticks;  // accumulated spans
-    unsigned long long clock_launches;
-    unsigned long long alg_bytes;    // algorithmic bitset bytes of the support launches
-    unsigned long long candidates;
     unsigned long long levels;
 };
 
@@ -69,12 +66,17 @@ struct PlanArgs {
     const uint32_t *pairbits;   // pair tables, null for none
     uint32_t *kept, *cnt_pat, *cnt_child, *cnt_cext, *off_pat, *off_child, *off_cext;
     uint32_t *lvitems;          // this level's itemsets for the subset test
-    uint32_t *off2, *ext2, *live, *pos;  // next level's prune buffers
+    uint32_t *off2, *ext2, *live, *pos;  // next level's staging for the subset test
+    unsigned long long *part;   // plan/count chunk sums: 3 x (plan grid)
+    unsigned long long *partl;  // prune/flags chunk survivor counts: (prune grid)
+    unsigned long long *stamps; // the last support launch's per-block stamps
+    uint32_t stamp_blocks;
     uint32_t *pat_parent, *pat_item, *pat_sup;
     int64_t *pat_per;
     Ctl *ctl;
     Caps caps;                  // capacities of the next level set
-    uint32_t G;
+    uint32_t G, te;             // support work: largest entries per range, tiles per range
+    uint64_t work_target;       // work items a short level is cut into
     int emit_children;
     int periodic;
     int prune;                  // run the subset test
@@ -133,16 +135,108 @@ __device__ __forceinline__ uint32_t child_count(const uint32_t *lmask, const uin
     return c;
 }
 
-// plan/1 -------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long block_sum64(unsigned long long v, unsigned long long *red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const unsigned long long t = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    return t;
+}
+
+// Contiguous chunk [lo, hi) of n items for block b of nb.
+__device__ __forceinline__ void chunk_of(uint64_t n, uint32_t b, uint32_t nb, uint64_t *lo, uint64_t *hi) {
+    const uint64_t c = (n + nb - 1) / nb;
+    *lo = min(n, (uint64_t)b * c);
+    *hi = min(n, *lo + c);
+}
+
+// sum of part[0 .. b), in every lane of the block
+__device__ __forceinline__ unsigned long long sum_before(const unsigned long long *part, uint32_t b,
+                                                         unsigned long long *red) {
+    unsigned long long v = 0;
+    for (uint32_t j = threadIdx.x; j < b; j += blockDim.x) v += part[j];
+    return block_sum64(v, red);
+}
+
+// first i <= n with off[i] >= x (off[0 .. n] non-decreasing)
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *off, uint32_t n, uint64_t x) {
+    uint32_t lo = 0, hi = n + 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Fold the per-block stamps a support launch left (start, end, bytes,
+// candidates; start 0 = no launch since the last fold) into the totals and
+// clear them.  One 256-lane block of a later launch runs it, so the support
+// kernel itself needs no fence or shared counter.
+__device__ void fold_stamps(unsigned long long *stamps, uint32_t blocks, SupportAcc *acc) {
+    __shared__ unsigned long long red[4][4];
+    const uint32_t tid = threadIdx.x;
+    unsigned long long lo = ~0ull, hi = 0, by = 0, ca = 0;
+    for (uint32_t b = tid; b < blocks; b += blockDim.x) {
+        unsigned long long *s = stamps + 4ull * b;
+        if (s[0]) {
+            lo = min(lo, s[0]);
+            hi = max(hi, s[1]);
+            by += s[2];
+            ca += s[3];
+            s[0] = s[1] = s[2] = s[3] = 0;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, (unsigned long long)__shfl_xor(lo, off, 64));
+        hi = max(hi, (unsigned long long)__shfl_xor(hi, off, 64));
+        by += __shfl_xor(by, off, 64);
+        ca += __shfl_xor(ca, off, 64);
+    }
+    if ((tid & 63) == 0) {
+        red[tid >> 6][0] = lo;
+        red[tid >> 6][1] = hi;
+        red[tid >> 6][2] = by;
+        red[tid >> 6][3] = ca;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 0; w < 4; ++w) {
+            lo = min(lo, red[w][0]);
+            hi = max(hi, red[w][1]);
+        }
+        if (hi) {
+            acc->ticks += hi > lo ? hi - lo : 0;
+            acc->launches += 1;
+            acc->bytes += red[0][2] + red[1][2] + red[2][2] + red[3][2];
+            acc->candidates += red[0][3] + red[1][3] + red[2][3] + red[3][3];
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_fold_stamps(unsigned long long *stamps, uint32_t blocks, Ctl *ctl) {
+    fold_stamps(stamps, blocks, &ctl->acc);
+}
+
+// plan/count -----------------------------------------------------------------------
+// (its last block -- whose chunk is the shortest -- also folds the stamps of
+// the support launch before it)
 __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
     __shared__ uint32_t lds[8];
     __shared__ uint32_t lmask[kMaskWords];
+    if (p.stamps && blockIdx.x == gridDim.x - 1) fold_stamps(p.stamps, p.stamp_blocks, &p.ctl->acc);
     if (p.ctl->overflow) return;
     const uint32_t tid = threadIdx.x;
     const uint32_t n_tasks = (uint32_t)p.L.sz->n_tasks;
+    uint64_t lo, hi;
+    chunk_of(n_tasks, blockIdx.x, gridDim.x, &lo, &hi);
     for (uint32_t w = tid; w < kMaskWords; w += blockDim.x) lmask[w] = 0;
     __syncthreads();
-    for (uint32_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {
+    unsigned long long s_pat = 0, s_child = 0, s_cext = 0;
+    for (uint32_t t = (uint32_t)lo; t < hi; ++t) {
         const DbDev &D = p.dbs[p.L.db[t]];
         const uint32_t a = p.L.ext_off[t], b = p.L.ext_off[t + 1];
         uint32_t nk = 0;
@@ -173,23 +267,28 @@ __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
             (void)block_scan(c, lds, &cext);
             if (p.pairbits) mask_clear(lmask, p.kept, p.L.ext, a, nk, D.fbase);
         }
+        const uint32_t nc = (p.emit_children && nk >= 2) ? nk - 1 : 0;
         if (tid == 0) {
             p.cnt_pat[t] = nk;
-            p.cnt_child[t] = (p.emit_children && nk >= 2) ? nk - 1 : 0;
+            p.cnt_child[t] = nc;
             p.cnt_cext[t] = cext;
         }
+        s_pat += nk;
+        s_child += nc;
+        s_cext += cext;
         __syncthreads();
+    }
+    if (tid == 0) {
+        p.part[blockIdx.x] = s_pat;
+        p.part[gridDim.x + blockIdx.x] = s_child;
+        p.part[2 * gridDim.x + blockIdx.x] = s_cext;
     }
 }
 
-// alloc (one lane) ------------------------------------------------------------------
-__global__ void k_alloc(PlanArgs p) {
+// Size the next level from this level's totals (one lane).
+__device__ void alloc_next(const PlanArgs &p) {
     Ctl *c = p.ctl;
     LevelSizes *L = p.L.sz, *N = p.N.sz;
-    if (c->overflow) {
-        N->n_tasks = N->n_ext = N->n_work = 0;
-        return;
-    }
     const uint64_t n_pat = L->n_pat;
     const uint64_t n_child = p.emit_children ? L->n_child : 0;
     const uint64_t n_cext = p.emit_children ? L->n_cext : 0;
@@ -197,13 +296,14 @@ __global__ void k_alloc(PlanArgs p) {
     uint32_t ov = 0;
     if (n_child > p.caps.tasks) ov |= kOvTasks;
     if (n_cext > p.caps.ext) ov |= kOvEntries;
+    if (n_cext >= 0xffffffffull || n_child >= 0xffffffffull) ov |= kOvIndex;
     if (c->n_patterns + n_pat > p.caps.patterns) ov |= kOvPatterns;
     if (n_child && d1 > kMaxDepth) ov |= kOvDepth;
     if (p.prune && n_child && L->d >= 2 && n_pat * d1 > p.caps.lv) ov |= kOvLevelSets;
     if (p.periodic && n_cext * p.parts_tiles > p.caps.parts) ov |= kOvParts;
     if (ov) {
         c->overflow |= ov;
-        N->n_tasks = N->n_ext = N->n_work = 0;
+        N->n_tasks = N->n_ext = N->n_work = N->n_cand = 0;
         return;
     }
     c->pat_base = c->n_patterns;
@@ -211,14 +311,59 @@ __global__ void k_alloc(PlanArgs p) {
     N->n_tasks = n_child;
     N->d = d1;
     N->n_ext = n_cext;
-    N->n_work = 0;
+    N->n_cand = n_cext;
+    N->group = level_group(n_cext, p.G, p.te, p.work_target);
+    N->n_work = n_child ? level_work(n_cext, (uint32_t)N->group, p.te) : 0;
     if (n_child) {
         p.N.ext_off[n_child] = (uint32_t)n_cext;
+        p.off2[n_child] = (uint32_t)n_cext;
         c->levels += 1;
     }
 }
 
-// plan/2 -------------------------------------------------------------------------
+// plan/offsets: same grid as plan/count -----------------------------------------------
+__global__ __launch_bounds__(256) void k_plan_offsets(PlanArgs p) {
+    __shared__ uint32_t lds[8];
+    __shared__ unsigned long long red[4];
+    const bool last = blockIdx.x == gridDim.x - 1;
+    if (p.ctl->overflow) {
+        if (last && threadIdx.x == 0) p.N.sz->n_tasks = p.N.sz->n_ext = p.N.sz->n_work = p.N.sz->n_cand = 0;
+        return;
+    }
+    const uint32_t n = (uint32_t)p.L.sz->n_tasks;
+    uint64_t lo, hi;
+    chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+    const uint32_t *cnt[3] = {p.cnt_pat, p.cnt_child, p.cnt_cext};
+    uint32_t *off[3] = {p.off_pat, p.off_child, p.off_cext};
+    unsigned long long tot[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        unsigned long long base = sum_before(p.part + (uint64_t)a * gridDim.x, blockIdx.x, red);
+        for (uint64_t i0 = lo; i0 < hi; i0 += 256) {
+            const uint64_t i = i0 + threadIdx.x;
+            const uint32_t v = i < hi ? cnt[a][i] : 0u;
+            uint32_t t;
+            const uint32_t x = block_scan(v, lds, &t);
+            if (i < hi) off[a][i] = (uint32_t)(base + x);
+            base += t;
+        }
+        tot[a] = base;  // in the last block: the total
+    }
+    if (last && threadIdx.x == 0) {
+        for (int a = 0; a < 3; ++a) off[a][n] = (uint32_t)tot[a];
+        p.L.sz->n_pat = tot[0];
+        p.L.sz->n_child = tot[1];
+        p.L.sz->n_cext = tot[2];
+        alloc_next(p);
+    }
+}
+
+// subset test (cuApriori.py:487-490) ----------------------------------------------
+__device__ __forceinline__ bool prune_active(const PlanArgs &p) {
+    return p.prune && !p.ctl->overflow && p.emit_children && p.L.sz->d >= 2 && p.N.sz->n_tasks > 0;
+}
+
+// plan/write -------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
     __shared__ uint32_t lds[8];
     __shared__ uint32_t lmask[kMaskWords];
@@ -229,9 +374,12 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
     const uint32_t d1 = d + 1;
     const uint64_t pat0 = p.ctl->pat_base;
     const bool children = p.emit_children && p.N.sz->n_tasks > 0;
-    // this level's itemsets feed the subset test of its children (the
-    // allocator sized lvitems for exactly this case)
-    const bool write_lv = p.prune && p.lvitems && d >= 2 && children;
+    // With the subset test, the children's entries go to staging and this
+    // level's itemsets feed the test (the allocator sized lvitems for exactly
+    // this case); without it, straight into the next level.
+    const bool stage = prune_active(p);
+    uint32_t *const xdst = stage ? p.ext2 : p.N.ext;
+    uint32_t *const odst = stage ? p.off2 : p.N.ext_off;
     for (uint32_t w = tid; w < kMaskWords; w += blockDim.x) lmask[w] = 0;
     __syncthreads();
     for (uint32_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {
@@ -249,7 +397,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             p.pat_item[pb + j] = p.fcol ? p.fcol[f] : f;
             p.pat_sup[pb + j] = p.L.sup[e];
             if (p.pat_per) p.pat_per[pb + j] = p.L.per[e];
-            if (write_lv) {
+            if (stage) {
                 uint32_t *dst = p.lvitems + (uint64_t)(p.off_pat[t] + j) * d1;
                 for (uint32_t i = 0; i < d; ++i) dst[i] = src[i];
                 dst[d] = f;
@@ -289,24 +437,23 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
                 p.N.pid[ct] = (uint32_t)(pb + j);
                 p.N.db[ct] = dbi;
                 uint32_t w = xb + carry + off;
-                p.N.ext_off[ct] = w;
-                p.N.nwork[ct] = task_work(cj, p.G, D.tiles_eff);
+                odst[ct] = w;
                 if (table) {
                     const uint32_t fl = fj - D.fbase;
                     for (uint32_t q = fl >> 5; q <= wlast; ++q) {
                         uint32_t m = lmask[q] & prow[q];
                         if (q == (fl >> 5)) m &= above(fl);
                         while (m) {
-                            p.N.ext[w] = D.fbase + q * 32 + (uint32_t)__builtin_ctz(m);
-                            p.N.sup[w] = 0;
+                            xdst[w] = D.fbase + q * 32 + (uint32_t)__builtin_ctz(m);
+                            if (!stage) p.N.sup[w] = 0;
                             ++w;
                             m &= m - 1;
                         }
                     }
                 } else {
                     for (uint32_t l = j + 1; l < nk; ++l) {
-                        p.N.ext[w] = p.L.ext[p.kept[a + l]];
-                        p.N.sup[w] = 0;
+                        xdst[w] = p.L.ext[p.kept[a + l]];
+                        if (!stage) p.N.sup[w] = 0;
                         ++w;
                     }
                 }
@@ -321,7 +468,6 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
     }
 }
 
-// subset test (cuApriori.py:487-490) ----------------------------------------------
 // Is (prefix without item `drop`) + {x} -- d1 items, ascending -- in lv?
 __device__ __forceinline__ bool lv_contains(const uint32_t *__restrict__ lv, uint64_t n_lv, uint32_t d1,
                                             const uint32_t *__restrict__ pre, uint32_t drop,
@@ -341,10 +487,6 @@ __device__ __forceinline__ bool lv_contains(const uint32_t *__restrict__ lv, uin
     return false;
 }
 
-__device__ __forceinline__ bool prune_active(const PlanArgs &p) {
-    return p.prune && !p.ctl->overflow && p.emit_children && p.L.sz->d >= 2 && p.N.sz->n_tasks > 0;
-}
-
 // largest t < n with off[t] <= i
 __device__ __forceinline__ uint32_t upper_task(const uint32_t *__restrict__ off, uint32_t n, uint64_t i) {
     uint32_t lo = 0, hi = n;
@@ -355,225 +497,71 @@ __device__ __forceinline__ uint32_t upper_task(const uint32_t *__restrict__ off,
     return lo;
 }
 
+// prune/flags: a contiguous chunk of staged candidates per block ---------------------
 __global__ __launch_bounds__(256) void k_prune_flags(PlanArgs p) {
+    __shared__ unsigned long long red[4];
     if (!prune_active(p)) return;
     const uint32_t n_child = (uint32_t)p.N.sz->n_tasks;
-    const uint64_t n_ext = p.N.sz->n_ext;
+    const uint64_t n_cand = p.N.sz->n_cand;
     const uint32_t d1 = (uint32_t)p.N.sz->d;  // prefix length of the children
     const uint64_t n_lv = p.L.sz->n_pat;
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_ext;
-         w += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t ct = upper_task(p.N.ext_off, n_child, w);
+    uint64_t lo, hi;
+    chunk_of(n_cand, blockIdx.x, gridDim.x, &lo, &hi);
+    unsigned long long c = 0;
+    for (uint64_t w = lo + threadIdx.x; w < hi; w += blockDim.x) {
+        const uint32_t ct = upper_task(p.off2, n_child, w);
         const uint32_t *pre = p.N.prefix + (uint64_t)ct * kMaxDepth;
-        const uint32_t x = p.N.ext[w];
+        const uint32_t x = p.ext2[w];
         bool ok = true;
         // the subsets without the last prefix item or without x are kept by
         // construction; test the ones without each earlier prefix item
         for (uint32_t drop = 0; ok && drop + 1 < d1; ++drop) ok = lv_contains(p.lvitems, n_lv, d1, pre, drop, x);
         p.live[w] = ok ? 1u : 0u;
+        c += ok ? 1u : 0u;
     }
+    c = block_sum64(c, red);
+    if (threadIdx.x == 0) p.partl[blockIdx.x] = c;
 }
 
-__global__ __launch_bounds__(256) void k_prune_offsets(PlanArgs p) {
-    if (!prune_active(p)) return;
-    const uint32_t n_child = (uint32_t)p.N.sz->n_tasks;
-    for (uint32_t ct = blockIdx.x * blockDim.x + threadIdx.x; ct <= n_child; ct += gridDim.x * blockDim.x) {
-        const uint32_t a = p.pos[p.N.ext_off[ct]];
-        p.off2[ct] = a;
-        if (ct < n_child) {
-            const DbDev &D = p.dbs[p.N.db[ct]];
-            p.N.nwork[ct] = task_work(p.pos[p.N.ext_off[ct + 1]] - a, p.G, D.tiles_eff);
-        }
-    }
-}
-
+// prune/compact: same grid as prune/flags ---------------------------------------------
 __global__ __launch_bounds__(256) void k_prune_compact(PlanArgs p) {
-    if (!prune_active(p)) return;
-    const uint64_t n_ext = p.N.sz->n_ext;
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < n_ext;
-         w += (uint64_t)gridDim.x * blockDim.x)
-        if (p.live[w]) p.ext2[p.pos[w]] = p.N.ext[w];
-}
-
-__global__ __launch_bounds__(256) void k_prune_copyback(PlanArgs p) {
+    __shared__ uint32_t lds[8];
+    __shared__ unsigned long long red[4];
     if (!prune_active(p)) return;
     const uint32_t n_child = (uint32_t)p.N.sz->n_tasks;
-    const uint64_t live = p.ctl->live_total;
-    // nothing in this launch reads the entry count
-    if (blockIdx.x == 0 && threadIdx.x == 0) p.N.sz->n_ext = live;
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < live;
-         w += (uint64_t)gridDim.x * blockDim.x) {
-        p.N.ext[w] = p.ext2[w];
-        p.N.sup[w] = 0;
-    }
-    for (uint32_t ct = blockIdx.x * blockDim.x + threadIdx.x; ct <= n_child; ct += gridDim.x * blockDim.x)
-        p.N.ext_off[ct] = p.off2[ct];
-}
-
-// Algorithmic bitset bytes of the next support launch: every task reads its
-// d prefix rows and one row per entry, W words each.
-__global__ __launch_bounds__(256) void k_level_stats(LevelPtrs N, const DbDev *dbs, Ctl *ctl) {
-    __shared__ unsigned long long red[4];
-    if (ctl->overflow) return;
-    const uint32_t n = (uint32_t)N.sz->n_tasks;
-    const uint64_t d = N.sz->d;
-    unsigned long long s = 0, c = 0;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        const uint64_t k = N.ext_off[t + 1] - N.ext_off[t];
-        s += 8ull * dbs[N.db[t]].words * (d + k);
-        c += k;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        s += __shfl_xor(s, off, 64);
-        c += __shfl_xor(c, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned long long tot = red[0] + red[1] + red[2] + red[3];
-        if (tot) atomicAdd(&ctl->alg_bytes, tot);
-    }
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned long long tot = red[0] + red[1] + red[2] + red[3];
-        if (tot) atomicAdd(&ctl->candidates, tot);
-    }
-}
-
-// Fold the last support launch's per-block clock stamps (earliest start,
-// latest end) into the totals.  One block; stamps of blocks that did not
-// run this launch are older than its start and are skipped.
-__global__ __launch_bounds__(256) void k_clock_acc(Ctl *ctl, unsigned long long *stamps, uint32_t blocks,
-                                                  const LevelSizes *sz) {
-    __shared__ unsigned long long lo[4], hi[4];
-    unsigned long long a = ~0ull, b = 0;
-    for (uint32_t i = threadIdx.x; i < blocks; i += blockDim.x) {
-        const unsigned long long s = stamps[2 * i], e = stamps[2 * i + 1];
-        if (e >= s && s > ctl->clock[0]) {
-            a = s < a ? s : a;
-            b = e > b ? e : b;
-        }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long x = __shfl_xor(a, off, 64), y = __shfl_xor(b, off, 64);
-        a = x < a ? x : a;
-        b = y > b ? y : b;
-    }
-    if ((threadIdx.x & 63) == 0) {
-        lo[threadIdx.x >> 6] = a;
-        hi[threadIdx.x >> 6] = b;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < 4; ++w) {
-            a = lo[w] < a ? lo[w] : a;
-            b = hi[w] > b ? hi[w] : b;
-        }
-        a = lo[0] < a ? lo[0] : a;
-        b = hi[0] > b ? hi[0] : b;
-        if (sz->n_work && !ctl->overflow && b > a) {
-            ctl->clock_ticks += b - a;
-            ctl->clock_launches += 1;
-            ctl->clock[0] = b;  // later launches start after this end
-        }
-    }
-}
-
-// ---- device-sized scans ------------------------------------------------------------
-// Exclusive scan of up to three uint32 arrays whose length is *n (device);
-// out[i] for i <= n, out[n] = total, *total = total.  blockIdx.y = array.
-struct DScan {
-    const uint32_t *in[3];
-    uint32_t *out[3];
-    unsigned long long *total[3];
-    const unsigned long long *n;
-    unsigned long long *part;  // 3 x (kScanBlocks + 1)
-};
-
-__device__ __forceinline__ void dscan_range(uint64_t n, uint32_t b, uint64_t *lo, uint64_t *hi) {
-    const uint64_t chunk = (n + kScanBlocks - 1) / kScanBlocks;
-    *lo = std::min<uint64_t>(n, (uint64_t)b * chunk);
-    *hi = std::min<uint64_t>(n, *lo + chunk);
-}
-
-__device__ __forceinline__ unsigned long long block_sum64(unsigned long long v, unsigned long long *red) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    const unsigned long long t = red[0] + red[1] + red[2] + red[3];
-    __syncthreads();
-    return t;
-}
-
-__global__ __launch_bounds__(256) void k_dscan_part(DScan s) {
-    __shared__ unsigned long long red[4];
-    const uint32_t a = blockIdx.y;
-    if (!s.in[a]) return;
+    const uint64_t n_cand = p.N.sz->n_cand;
+    const bool last = blockIdx.x == gridDim.x - 1;
     uint64_t lo, hi;
-    dscan_range(*s.n, blockIdx.x, &lo, &hi);
-    unsigned long long v = 0;
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) v += s.in[a][i];
-    v = block_sum64(v, red);
-    if (threadIdx.x == 0) s.part[a * (kScanBlocks + 1) + blockIdx.x] = v;
-}
-
-__global__ __launch_bounds__(256) void k_dscan_top(DScan s) {
-    __shared__ unsigned long long lds[4];
-    const uint32_t a = blockIdx.x;
-    if (!s.in[a]) return;
-    unsigned long long *part = s.part + a * (kScanBlocks + 1);
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const unsigned long long v = part[tid];
-    unsigned long long x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const unsigned long long y = __shfl_up(x, off, 64);
-        if (lane >= (uint32_t)off) x += y;
-    }
-    if (lane == 63) lds[wave] = x;
-    __syncthreads();
-    unsigned long long base = 0;
-    for (uint32_t w = 0; w < wave; ++w) base += lds[w];
-    const unsigned long long total = lds[0] + lds[1] + lds[2] + lds[3];
-    __syncthreads();
-    part[tid] = base + x - v;
-    if (tid == 0) {
-        part[kScanBlocks] = total;
-        if (s.total[a]) *s.total[a] = total;
-        s.out[a][*s.n] = (uint32_t)total;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_dscan_down(DScan s) {
-    __shared__ uint32_t lds[4];
-    const uint32_t a = blockIdx.y;
-    if (!s.in[a]) return;
-    uint64_t lo, hi;
-    dscan_range(*s.n, blockIdx.x, &lo, &hi);
-    unsigned long long carry = s.part[a * (kScanBlocks + 1) + blockIdx.x];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    chunk_of(n_cand, blockIdx.x, gridDim.x, &lo, &hi);
+    unsigned long long base = sum_before(p.partl, blockIdx.x, red);
     for (uint64_t i0 = lo; i0 < hi; i0 += 256) {
-        const uint64_t i = i0 + tid;
-        const uint32_t v = i < hi ? s.in[a][i] : 0;
-        uint32_t x = v;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off, 64);
-            if (lane >= (uint32_t)off) x += y;
+        const uint64_t w = i0 + threadIdx.x;
+        const uint32_t f = w < hi ? p.live[w] : 0u;
+        uint32_t tot;
+        const uint32_t x = block_scan(f, lds, &tot);  // ends with a barrier
+        if (w < hi) {
+            const uint64_t q = base + x;
+            p.pos[w] = (uint32_t)q;
+            if (f) {
+                p.N.ext[q] = p.ext2[w];
+                p.N.sup[q] = 0;
+            }
         }
-        if (lane == 63) lds[wave] = x;
-        __syncthreads();
-        uint32_t base = 0;
-        for (uint32_t w = 0; w < wave; ++w) base += lds[w];
-        const uint32_t tot = lds[0] + lds[1] + lds[2] + lds[3];
-        __syncthreads();
-        if (i < hi) s.out[a][i] = (uint32_t)(carry + base + x - v);
-        carry += tot;
+        base += tot;
+    }
+    __syncthreads();  // pos of the chunk is visible to the whole block
+    // children whose staged entries start in [lo, hi); the last block also
+    // takes those starting at n_cand, and the end sentinel
+    const uint32_t c_lo = lower_bound_u32(p.off2, n_child, lo);
+    const uint32_t c_hi = last ? n_child + 1 : lower_bound_u32(p.off2, n_child, hi);
+    for (uint32_t ct = c_lo + threadIdx.x; ct < c_hi; ct += blockDim.x) {
+        const uint32_t x = p.off2[ct];
+        p.N.ext_off[ct] = x < hi ? p.pos[x] : (uint32_t)base;
+    }
+    if (last && threadIdx.x == 0) {
+        p.N.sz->n_ext = base;
+        p.N.sz->group = level_group(base, p.G, p.te, p.work_target);
+        p.N.sz->n_work = level_work(base, (uint32_t)p.N.sz->group, p.te);
     }
 }
 
@@ -609,8 +597,7 @@ struct L2Db {
     uint32_t ext0, pad;       // first entry
 };
 
-__global__ void k_level2(const L2Db *l2, uint32_t n_db, uint32_t n_tasks, LevelPtrs L, const DbDev *dbs,
-                         uint32_t G) {
+__global__ void k_level2(const L2Db *l2, uint32_t n_db, uint32_t n_tasks, LevelPtrs L, const DbDev *dbs) {
     for (uint32_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {
         uint32_t lo = 0, hi = n_db;
         while (hi - lo > 1) {
@@ -630,7 +617,6 @@ __global__ void k_level2(const L2Db *l2, uint32_t n_db, uint32_t n_tasks, LevelP
             L.pid[t] = D.fbase + a;
             L.db[t] = i;
             L.ext_off[t] = base;
-            L.nwork[t] = task_work(k, G, D.tiles_eff);
         }
         for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) {
             L.ext[base + j] = D.fbase + a + 1 + j;
@@ -677,7 +663,7 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
 // ---- host side ---------------------------------------------------------------------
 
 struct LevelSet {
-    DBuf prefix, pid, db, ext_off, work_off, nwork, ext, sup, per;
+    DBuf prefix, pid, db, ext_off, ext, sup, per;
     DBuf kept, cnt[3], off[3], off2, ext2, live, pos, lvitems, parts;
     LevelPtrs ptrs(LevelSizes *sz) const {
         LevelPtrs p;
@@ -685,8 +671,6 @@ struct LevelSet {
         p.pid = pid.as<uint32_t>();
         p.db = db.as<uint32_t>();
         p.ext_off = ext_off.as<uint32_t>();
-        p.work_off = work_off.as<uint32_t>();
-        p.nwork = nwork.as<uint32_t>();
         p.ext = ext.as<uint32_t>();
         p.sup = sup.as<uint32_t>();
         p.per = per.as<int64_t>();
@@ -699,8 +683,6 @@ struct LevelSet {
         CPM_CHECK(pid.ensure(t1 * 4));
         CPM_CHECK(db.ensure(t1 * 4));
         CPM_CHECK(ext_off.ensure(t1 * 4));
-        CPM_CHECK(work_off.ensure(t1 * 4));
-        CPM_CHECK(nwork.ensure(t1 * 4));
         CPM_CHECK(off2.ensure(t1 * 4));
         for (int i = 0; i < 3; ++i) {
             CPM_CHECK(cnt[i].ensure(t1 * 4));
@@ -725,14 +707,14 @@ struct LevelSet {
 struct GraphKey {
     const void *ptrs[64];
     uint64_t caps[5];
-    uint32_t G, tile_words, parts_tiles;
+    uint32_t G, te, tile_words, parts_tiles;
     int periodic, prune;
     bool operator==(const GraphKey &o) const { return memcmp(this, &o, sizeof(GraphKey)) == 0; }
 };
 
 struct SearchWs {
     LevelSet set[2];
-    DBuf ctl, dbs, fcol, pairbits, gram, goff, gjobs, gdbs, l2, scanpart, pat_parent, pat_item, pat_sup,
+    DBuf ctl, dbs, fcol, pairbits, gram, goff, gjobs, gdbs, l2, part, partl, pat_parent, pat_item, pat_sup,
         pat_per, ts, stamps;
     HBuf host;
     Caps caps{};
@@ -770,6 +752,7 @@ struct Search {
     std::vector<DbIn> in;
     bool periodic = false, use_gram = true, prune = true, use_graph = true;
     uint32_t G = 32, tile_words = kTileWords;
+    uint32_t te = 1;                 // tiles per support work range
     std::vector<DbDev> dbs;          // host copy
     std::vector<uint32_t> nf;        // frequent items per database
     uint32_t parts_tiles = 0;
@@ -800,6 +783,10 @@ struct Search {
         p.ext2 = N.ext2.as<uint32_t>();
         p.live = N.live.as<uint32_t>();
         p.pos = N.pos.as<uint32_t>();
+        p.part = w.part.as<unsigned long long>();
+        p.partl = w.partl.as<unsigned long long>();
+        p.stamps = w.stamps.as<unsigned long long>();
+        p.stamp_blocks = grid();
         p.pat_parent = w.pat_parent.as<uint32_t>();
         p.pat_item = w.pat_item.as<uint32_t>();
         p.pat_sup = w.pat_sup.as<uint32_t>();
@@ -807,6 +794,8 @@ struct Search {
         p.ctl = ctl();
         p.caps = w.caps;
         p.G = G;
+        p.te = te;
+        p.work_target = work_target();
         p.emit_children = emit_children;
         p.periodic = periodic ? 1 : 0;
         p.prune = prune ? 1 : 0;
@@ -820,65 +809,39 @@ struct Search {
         a.L = L.ptrs(sz(c));
         a.dbs = w.dbs.as<DbDev>();
         a.fcol = level1 ? nullptr : w.fcol.as<uint32_t>();
-        a.G = G;
         a.stop = &ctl()->overflow;
         a.parts = L.parts.as<PeriodPart>();
         a.parts_tiles = parts_tiles;
-        a.clock = w.stamps.as<unsigned long long>();
+        a.te = te;
+        a.stamps = w.stamps.as<unsigned long long>();
         return a;
     }
 
     uint32_t grid() const { return support_grid(ctx); }
+    uint64_t work_target() const { return 4ull * grid(); }
     uint32_t grid_tasks() const { return (uint32_t)ctx->cu_count * 4; }
-
-    int dscan(const uint32_t *in0, uint32_t *out0, unsigned long long *t0, const uint32_t *in1,
-              uint32_t *out1, unsigned long long *t1, const uint32_t *in2, uint32_t *out2,
-              unsigned long long *t2, const unsigned long long *n) {
-        DScan s;
-        s.in[0] = in0; s.out[0] = out0; s.total[0] = t0;
-        s.in[1] = in1; s.out[1] = out1; s.total[1] = t1;
-        s.in[2] = in2; s.out[2] = out2; s.total[2] = t2;
-        s.n = n;
-        s.part = w.scanpart.as<unsigned long long>();
-        const uint32_t arrays = in2 ? 3 : (in1 ? 2 : 1);
-        hipLaunchKernelGGL(k_dscan_part, dim3(kScanBlocks, arrays), dim3(256), 0, ctx->stream, s);
-        hipLaunchKernelGGL(k_dscan_top, dim3(arrays), dim3(256), 0, ctx->stream, s);
-        hipLaunchKernelGGL(k_dscan_down, dim3(kScanBlocks, arrays), dim3(256), 0, ctx->stream, s);
-        CPM_HIP(hipGetLastError());
-        return CPM_OK;
-    }
 
     int count(int c, bool level1) {
         SupportArgs a = support_args(c, level1);
-        if (periodic) CPM_CHECK(launch_support_periodic(ctx, a, grid()));
-        else CPM_CHECK(launch_support_count(ctx, a, tile_words, grid()));
-        hipLaunchKernelGGL(k_clock_acc, dim3(1), dim3(256), 0, ctx->stream, ctl(),
-                           w.stamps.as<unsigned long long>(), grid(), sz(c));
-        CPM_HIP(hipGetLastError());
-        return CPM_OK;
+        if (periodic) return launch_support_periodic(ctx, a, grid());
+        return launch_support_count(ctx, a, tile_words, grid());
+    }
+
+    // plan: count, offsets (+ sizing of the next level), write
+    void plan(const PlanArgs &p) {
+        hipLaunchKernelGGL(k_plan_count, dim3(grid_tasks()), dim3(256), 0, ctx->stream, p);
+        hipLaunchKernelGGL(k_plan_offsets, dim3(grid_tasks()), dim3(256), 0, ctx->stream, p);
+        hipLaunchKernelGGL(k_plan_write, dim3(grid_tasks()), dim3(256), 0, ctx->stream, p);
     }
 
     // plan level c and produce + count level c^1 (one level step)
     int step(int c) {
         PlanArgs p = plan_args(c, 1);
-        const LevelSizes *L = sz(c);
-        LevelSizes *N = sz(c ^ 1);
-        hipLaunchKernelGGL(k_plan_count, dim3(grid_tasks()), dim3(256), 0, ctx->stream, p);
-        CPM_CHECK(dscan(p.cnt_pat, p.off_pat, &sz(c)->n_pat, p.cnt_child, p.off_child, &sz(c)->n_child,
-                        p.cnt_cext, p.off_cext, &sz(c)->n_cext, &L->n_tasks));
-        hipLaunchKernelGGL(k_alloc, dim3(1), dim3(1), 0, ctx->stream, p);
-        hipLaunchKernelGGL(k_plan_write, dim3(grid_tasks()), dim3(256), 0, ctx->stream, p);
+        plan(p);
         if (prune) {
             hipLaunchKernelGGL(k_prune_flags, dim3(grid()), dim3(256), 0, ctx->stream, p);
-            CPM_CHECK(dscan(p.live, p.pos, &ctl()->live_total, nullptr, nullptr, nullptr, nullptr, nullptr,
-                            nullptr, &N->n_ext));
-            hipLaunchKernelGGL(k_prune_offsets, dim3(grid()), dim3(256), 0, ctx->stream, p);
             hipLaunchKernelGGL(k_prune_compact, dim3(grid()), dim3(256), 0, ctx->stream, p);
-            hipLaunchKernelGGL(k_prune_copyback, dim3(grid()), dim3(256), 0, ctx->stream, p);
         }
-        CPM_CHECK(dscan(p.N.nwork, p.N.work_off, &N->n_work, nullptr, nullptr, nullptr, nullptr, nullptr,
-                        nullptr, &N->n_tasks));
-        hipLaunchKernelGGL(k_level_stats, dim3(grid_tasks()), dim3(256), 0, ctx->stream, p.N, p.dbs, ctl());
         CPM_HIP(hipGetLastError());
         return count(c ^ 1, false);
     }
@@ -890,12 +853,11 @@ struct Search {
         auto add = [&](const DBuf &b) { k.ptrs[i++] = b.p; };
         for (int j = 0; j < 2; ++j) {
             const LevelSet &S = w.set[(c + j) & 1];
-            add(S.prefix); add(S.pid); add(S.db); add(S.ext_off); add(S.work_off); add(S.nwork);
-            add(S.ext); add(S.sup); add(S.per); add(S.kept); add(S.off2); add(S.ext2); add(S.live);
+            add(S.prefix); add(S.pid); add(S.db); add(S.ext_off); add(S.ext); add(S.sup); add(S.per); add(S.kept); add(S.off2); add(S.ext2); add(S.live);
             add(S.pos); add(S.lvitems); add(S.parts);
             for (int q = 0; q < 3; ++q) { add(S.cnt[q]); add(S.off[q]); }
         }
-        add(w.ctl); add(w.dbs); add(w.fcol); add(w.pairbits); add(w.scanpart); add(w.stamps);
+        add(w.ctl); add(w.dbs); add(w.fcol); add(w.pairbits); add(w.part); add(w.partl); add(w.stamps);
         add(w.pat_parent); add(w.pat_item); add(w.pat_sup); add(w.pat_per);
         k.caps[0] = w.caps.tasks;
         k.caps[1] = w.caps.ext;
@@ -903,6 +865,7 @@ struct Search {
         k.caps[3] = w.caps.lv;
         k.caps[4] = w.caps.parts;
         k.G = G;
+        k.te = te;
         k.tile_words = tile_words;
         k.parts_tiles = parts_tiles;
         k.periodic = periodic;
@@ -946,9 +909,10 @@ struct Search {
         CPM_CHECK(w.pat_item.ensure((w.caps.patterns + 1) * 4));
         CPM_CHECK(w.pat_sup.ensure((w.caps.patterns + 1) * 4));
         if (periodic) CPM_CHECK(w.pat_per.ensure((w.caps.patterns + 1) * 8));
-        CPM_CHECK(w.scanpart.ensure(3 * (kScanBlocks + 1) * 8));
+        CPM_CHECK(w.part.ensure((size_t)3 * grid_tasks() * 8));
+        CPM_CHECK(w.partl.ensure((size_t)grid() * 8));
         CPM_CHECK(w.ctl.ensure(sizeof(Ctl)));
-        CPM_CHECK(w.stamps.ensure((size_t)grid() * 2 * 8));
+        CPM_CHECK(w.stamps.ensure((size_t)grid() * 4 * 8));
         CPM_CHECK(w.host.ensure(std::max<size_t>(sizeof(Ctl), 4096)));
         return CPM_OK;
     }
@@ -966,17 +930,15 @@ struct Search {
         const uint32_t n_db = (uint32_t)in.size();
         hipStream_t st = ctx->stream;
         CPM_CHECK(reserve());
-        Ctl init;
-        memset(&init, 0, sizeof(init));
-        init.clock[0] = 0;
         Ctl *hc = w.host.as<Ctl>();
-        *hc = init;
+        memset(hc, 0, sizeof(Ctl));
         CPM_HIP(hipMemcpyAsync(w.ctl.p, hc, sizeof(Ctl), hipMemcpyHostToDevice, st));
-        CPM_HIP(hipMemsetAsync(w.stamps.p, 0, (size_t)grid() * 2 * 8, st));
-        CPM_HIP(hipStreamSynchronize(st));
+        // stamps left by an attempt that overflowed (outside any graph)
+        CPM_HIP(hipMemsetAsync(w.stamps.p, 0, (size_t)grid() * 4 * 8, st));
 
         // database table (frequent ranges filled after level 1)
         dbs.assign(n_db, DbDev{});
+        te = 1;
         uint64_t total_items = 0;
         for (uint32_t i = 0; i < n_db; ++i) {
             const cpm_bits *b = in[i].bits;
@@ -987,6 +949,7 @@ struct Search {
             D.n = b->n_transactions;
             D.tiles = tiles_per_row(b->stride, periodic ? kTileWords : tile_words);
             D.tiles_eff = tiles_eff_of(D.tiles);
+            te = std::max(te, D.tiles_eff);
             D.minsup = in[i].minsup;
             D.maxper = in[i].maxper;
             D.end = in[i].end;
@@ -1007,46 +970,34 @@ struct Search {
         }
         LevelSet &S0 = w.set[0];
         {
-            std::vector<uint32_t> off(n_db + 1), work(n_db + 1);
-            uint64_t e = 0, wk = 0;
+            std::vector<uint32_t> off(n_db + 1);
+            uint64_t e = 0;
             for (uint32_t i = 0; i < n_db; ++i) {
                 off[i] = (uint32_t)e;
-                work[i] = (uint32_t)wk;
                 e += in[i].bits->n_items;
-                wk += task_work(in[i].bits->n_items, G, dbs[i].tiles_eff);
             }
             off[n_db] = (uint32_t)e;
-            work[n_db] = (uint32_t)wk;
             CPM_HIP(hipMemcpyAsync(S0.ext_off.p, off.data(), (n_db + 1) * 4, hipMemcpyHostToDevice, st));
-            CPM_HIP(hipMemcpyAsync(S0.work_off.p, work.data(), (n_db + 1) * 4, hipMemcpyHostToDevice, st));
             LevelSizes l1;
             memset(&l1, 0, sizeof(l1));
             l1.n_tasks = n_db;
             l1.d = 0;
             l1.n_ext = e;
-            l1.n_work = wk;
+            l1.n_cand = e;
+            l1.group = level_group(e, G, te, work_target());
+            l1.n_work = level_work(e, (uint32_t)l1.group, te);
             CPM_HIP(hipMemcpyAsync(sz(0), &l1, sizeof(l1), hipMemcpyHostToDevice, st));
             hipLaunchKernelGGL(k_level1, dim3(std::min<uint32_t>(n_db, 4096)), dim3(256), 0, st, n_db,
                                S0.ext_off.as<uint32_t>(), S0.ext.as<uint32_t>(), S0.sup.as<uint32_t>(),
                                S0.db.as<uint32_t>(), S0.pid.as<uint32_t>());
             CPM_HIP(hipGetLastError());
-            // algorithmic bytes of level 1: every row once
-            uint64_t bytes = 0;
-            for (uint32_t i = 0; i < n_db; ++i) bytes += 8ull * dbs[i].words * in[i].bits->n_items;
-            CPM_HIP(hipStreamSynchronize(st));
-            ctx_bytes_level1 = bytes;
             CPM_CHECK(count(0, true));
+            // the host vectors must outlive the copies
+            CPM_HIP(hipStreamSynchronize(st));
         }
         // plan level 1: the frequent items (no children)
-        {
-            PlanArgs p = plan_args(0, 0);
-            hipLaunchKernelGGL(k_plan_count, dim3(grid_tasks()), dim3(256), 0, st, p);
-            CPM_CHECK(dscan(p.cnt_pat, p.off_pat, &sz(0)->n_pat, p.cnt_child, p.off_child, &sz(0)->n_child,
-                            p.cnt_cext, p.off_cext, &sz(0)->n_cext, &sz(0)->n_tasks));
-            hipLaunchKernelGGL(k_alloc, dim3(1), dim3(1), 0, st, p);
-            hipLaunchKernelGGL(k_plan_write, dim3(grid_tasks()), dim3(256), 0, st, p);
-            CPM_HIP(hipGetLastError());
-        }
+        plan(plan_args(0, 0));
+        CPM_HIP(hipGetLastError());
         nf.assign(n_db, 0);
         std::vector<uint32_t> fbase(n_db + 1);
         CPM_HIP(hipMemcpyAsync(nf.data(), S0.cnt[0].p, n_db * 4, hipMemcpyDeviceToHost, st));
@@ -1107,14 +1058,15 @@ struct Search {
             ls.n_tasks = t2;
             ls.d = 1;
             ls.n_ext = e2;
+            ls.n_cand = e2;
+            ls.group = level_group(e2, G, te, work_target());
+            ls.n_work = level_work(e2, (uint32_t)ls.group, te);
             CPM_HIP(hipMemcpyAsync(sz(1), &ls, sizeof(ls), hipMemcpyHostToDevice, st));
             const uint32_t e2u = (uint32_t)e2;
             CPM_HIP(hipMemcpyAsync(S1.ext_off.as<uint32_t>() + t2, &e2u, 4, hipMemcpyHostToDevice, st));
             hipLaunchKernelGGL(k_level2, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0, st,
-                               w.l2.as<L2Db>(), n_db, (uint32_t)t2, S1.ptrs(sz(1)), w.dbs.as<DbDev>(), G);
+                               w.l2.as<L2Db>(), n_db, (uint32_t)t2, S1.ptrs(sz(1)), w.dbs.as<DbDev>());
             CPM_HIP(hipGetLastError());
-            CPM_CHECK(dscan(S1.nwork.as<uint32_t>(), S1.work_off.as<uint32_t>(), &sz(1)->n_work, nullptr,
-                            nullptr, nullptr, nullptr, nullptr, nullptr, &sz(1)->n_tasks));
             CPM_HIP(hipStreamSynchronize(st));
         }
         bool gram_ok = use_gram && !periodic;
@@ -1155,8 +1107,6 @@ struct Search {
                                w.goff.as<uint64_t>());
             CPM_HIP(hipGetLastError());
         } else {
-            hipLaunchKernelGGL(k_level_stats, dim3(grid_tasks()), dim3(256), 0, st, S1.ptrs(sz(1)),
-                               w.dbs.as<DbDev>(), ctl());
             CPM_CHECK(count(1, false));
         }
         // pair tables
@@ -1189,8 +1139,6 @@ struct Search {
         return CPM_OK;
     }
 
-    uint64_t ctx_bytes_level1 = 0;
-
     int run(cpm_patterns **out) {
         // first capacities: generous for the first level set, grown on overflow
         uint64_t items = 0, e2 = 0;
@@ -1218,6 +1166,7 @@ struct Search {
             if (!ov) break;
             if (tries > 24) return fail(CPM_ERR_NOMEM, "search buffers keep overflowing (%u)", ov);
             if (ov & kOvDepth) return fail(CPM_ERR_NOMEM, "an itemset longer than %u items", kMaxDepth + 1);
+            if (ov & kOvIndex) return fail(CPM_ERR_NOMEM, "a level with 2^32 or more candidates");
             if (ov & kOvTasks) w.caps.tasks *= 2;
             if (ov & kOvEntries) w.caps.ext *= 2;
             if (ov & kOvPatterns) w.caps.patterns *= 2;
@@ -1229,6 +1178,9 @@ struct Search {
 
     int download(cpm_patterns **out) {
         Ctl *hc = w.host.as<Ctl>();
+        hipLaunchKernelGGL(k_fold_stamps, dim3(1), dim3(256), 0, ctx->stream, w.stamps.as<unsigned long long>(),
+                           grid(), ctl());
+        CPM_HIP(hipGetLastError());
         CPM_CHECK(read_ctl(hc));
         const uint64_t n = hc->n_patterns;
         cpm_patterns *p = new cpm_patterns();
@@ -1268,10 +1220,10 @@ struct Search {
         int wall_khz = 0;
         (void)hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, ctx->device);
         const double tick_ms = wall_khz > 0 ? 1.0 / wall_khz : 0.0;
-        ctx->stats[0] += (double)hc->clock_launches;
-        ctx->stats[1] += (double)hc->clock_ticks * tick_ms;
-        ctx->stats[2] += (double)hc->alg_bytes + (double)ctx_bytes_level1;
-        ctx->stats[3] += (double)hc->candidates;
+        ctx->stats[0] += (double)hc->acc.launches;
+        ctx->stats[1] += (double)hc->acc.ticks * tick_ms;
+        ctx->stats[2] += (double)hc->acc.bytes;
+        ctx->stats[3] += (double)hc->acc.candidates;
         ctx->stats[7] = (double)hc->levels + 2;
         *out = p;
         return CPM_OK;
@@ -1380,30 +1332,24 @@ static int run_candidates(cpm_ctx *ctx, const cpm_bits *b, const uint32_t *cand,
     D.tiles = tiles_per_row(b->stride, tw);
     D.tiles_eff = tiles_eff_of(D.tiles);
     D.end = end;
-    std::vector<uint32_t> prefix, off, work, ext(n_cand);
-    uint64_t wk = 0;
+    std::vector<uint32_t> prefix, off, ext(n_cand);
     for (uint64_t c = 0; c < n_cand; ++c) {
         const uint32_t *it = cand + c * width;
         const bool fresh = c == 0 || memcmp(it, cand + (c - 1) * width, (size_t)d * 4) != 0;
         if (fresh) {
-            if (!off.empty()) wk += task_work((uint32_t)(c - off.back()), G, D.tiles_eff);
             off.push_back((uint32_t)c);
-            work.push_back((uint32_t)wk);
             prefix.resize(prefix.size() + kMaxDepth, 0);
             memcpy(prefix.data() + prefix.size() - kMaxDepth, it, (size_t)d * 4);
         }
         ext[c] = it[d];
     }
-    wk += task_work((uint32_t)(n_cand - off.back()), G, D.tiles_eff);
     const uint64_t nt = off.size();
     off.push_back((uint32_t)n_cand);
-    work.push_back((uint32_t)wk);
     std::vector<uint32_t> dbz(nt, 0);
-    DBuf dprefix, doff, dwork, dext, dsup, dper, ddb, dD, dsz, dparts, dts;
+    DBuf dprefix, doff, dext, dsup, dper, ddb, dD, dsz, dparts, dts;
     hipStream_t st = ctx->stream;
     CPM_CHECK(dprefix.ensure(prefix.size() * 4));
     CPM_CHECK(doff.ensure(off.size() * 4));
-    CPM_CHECK(dwork.ensure(work.size() * 4));
     CPM_CHECK(dext.ensure(n_cand * 4));
     CPM_CHECK(dsup.ensure(n_cand * 4));
     CPM_CHECK(ddb.ensure(nt * 4));
@@ -1421,10 +1367,11 @@ static int run_candidates(cpm_ctx *ctx, const cpm_bits *b, const uint32_t *cand,
     ls.n_tasks = nt;
     ls.d = d;
     ls.n_ext = n_cand;
-    ls.n_work = wk;
+    ls.n_cand = n_cand;
+    ls.group = level_group(n_cand, G, D.tiles_eff, 4ull * support_grid(ctx));
+    ls.n_work = level_work(n_cand, (uint32_t)ls.group, D.tiles_eff);
     CPM_HIP(hipMemcpyAsync(dprefix.p, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice, st));
     CPM_HIP(hipMemcpyAsync(doff.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, st));
-    CPM_HIP(hipMemcpyAsync(dwork.p, work.data(), work.size() * 4, hipMemcpyHostToDevice, st));
     CPM_HIP(hipMemcpyAsync(dext.p, ext.data(), n_cand * 4, hipMemcpyHostToDevice, st));
     CPM_HIP(hipMemcpyAsync(ddb.p, dbz.data(), nt * 4, hipMemcpyHostToDevice, st));
     CPM_HIP(hipMemcpyAsync(dD.p, &D, sizeof(D), hipMemcpyHostToDevice, st));
@@ -1434,13 +1381,12 @@ static int run_candidates(cpm_ctx *ctx, const cpm_bits *b, const uint32_t *cand,
     a.L.prefix = dprefix.as<uint32_t>();
     a.L.db = ddb.as<uint32_t>();
     a.L.ext_off = doff.as<uint32_t>();
-    a.L.work_off = dwork.as<uint32_t>();
     a.L.ext = dext.as<uint32_t>();
     a.L.sup = dsup.as<uint32_t>();
     a.L.per = dper.as<int64_t>();
     a.L.sz = dsz.as<LevelSizes>();
     a.dbs = dD.as<DbDev>();
-    a.G = G;
+    a.te = D.tiles_eff;
     a.parts = dparts.as<PeriodPart>();
     a.parts_tiles = D.tiles;
     if (periodic) CPM_CHECK(launch_support_periodic(ctx, a, support_grid(ctx)));

@@ -1,12 +1,13 @@
 // Support-counting kernels over the vertical stores of one or more databases.
 //
-// A level of prefix tasks is cut into work items (task, group of G entries,
-// tile of words).  A 256-lane workgroup builds its tile of the task's prefix
-// in registers (the AND of the d prefix item rows), then streams the group's
-// extension rows over it: AND, popcount, wave reduction, one atomic per entry
-// per workgroup.  The grid is persistent and a multiple of 8, and every
-// task's work items start on a multiple of 8, so a block -- which stays on
-// XCD blockIdx % 8 -- only ever touches tiles congruent to that XCD: each
+// A level's entries are cut into work items (a range of G consecutive
+// entries, a tile of words).  A 256-lane workgroup walks the tasks its range
+// touches; for each it builds its tile of the task's prefix in registers (the
+// AND of the d prefix item rows), then streams the task's extension rows of
+// the range over it: AND, popcount, wave reduction, one atomic per entry per
+// workgroup.  The grid is persistent and a multiple of 8 and the tiles per
+// range are too (when rows have 8 or more tiles), so a block -- which stays
+// on XCD blockIdx % 8 -- only ever touches tiles congruent to that XCD: each
 // XCD's L2 holds its own slice of the hot rows.  Sizes are read from device
 // memory, so the launch can be replayed from a graph.
 //
@@ -34,7 +35,8 @@ __device__ __forceinline__ uint4 and4(uint4 a, uint4 b) {
     return make_uint4(a.x & b.x, a.y & b.y, a.z & b.z, a.w & b.w);
 }
 
-// largest t < n with off[t] <= i
+// largest t < n with off[t] <= i (off non-decreasing, off[n] > i): the task
+// holding entry i
 __device__ __forceinline__ uint32_t find_task(cptr<uint32_t> off, uint32_t n, uint64_t i) {
     uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
@@ -52,52 +54,68 @@ __device__ __forceinline__ uint32_t col_of(cptr<uint32_t> fcol, uint32_t f) {
 // through the constant address space, so that the per-row lookups of a work
 // item (task offsets, prefix and entry items, columns) are scalar loads.
 struct LevelView {
-    cptr<uint32_t> work_off, ext_off, dbi, prefix, ext, fcol;
+    cptr<uint32_t> ext_off, dbi, prefix, ext, fcol;
     cptr<DbDev> dbs;
-    uint32_t G, n_tasks, d;
+    uint32_t G, te, n_tasks, d, n_ext;
     uint64_t total;
 };
 
+// false when the launch has nothing to do (the same answer in every block)
 __device__ __forceinline__ bool level_view(const SupportArgs &a, LevelView &v) {
+    if (a.stop && *a.stop) return false;
     v.n_tasks = (uint32_t)a.L.sz->n_tasks;
-    if (v.n_tasks == 0) return false;
-    v.d = (uint32_t)a.L.sz->d;
     v.total = a.L.sz->n_work;
-    v.work_off = to_const(a.L.work_off);
+    if (v.n_tasks == 0 || v.total == 0) return false;
+    v.d = (uint32_t)a.L.sz->d;
+    v.n_ext = (uint32_t)a.L.sz->n_ext;
     v.ext_off = to_const(a.L.ext_off);
     v.dbi = to_const(a.L.db);
     v.prefix = to_const(a.L.prefix);
     v.ext = to_const(a.L.ext);
     v.fcol = a.fcol ? to_const(a.fcol) : nullptr;
     v.dbs = to_const(a.dbs);
-    v.G = a.G;
+    v.G = (uint32_t)a.L.sz->group;
+    v.te = a.te;
     return true;
 }
 
-// One work item decoded: task, database, entry range, tile.
-struct Item {
-    uint32_t t, e0, e1, tile;
-    cptr<DbDev> db;
+// Work item `it`: entries [r0, r1) on tile `tile`.
+struct Range {
+    uint32_t r0, r1, tile;
 };
 
-__device__ __forceinline__ bool decode(const LevelView &v, uint64_t it, Item &w) {
-    w.t = find_task(v.work_off, v.n_tasks, it);
-    w.db = v.dbs + v.dbi[w.t];
-    const uint32_t te = w.db->tiles_eff;
-    const uint32_t local = (uint32_t)(it - v.work_off[w.t]);
-    const uint32_t gi = local / te;
-    w.tile = local - gi * te;
-    const uint32_t lo = v.ext_off[w.t], hi = v.ext_off[w.t + 1];
-    w.e0 = lo + gi * v.G;
-    w.e1 = min(hi, w.e0 + v.G);
-    return w.tile < w.db->tiles && w.e0 < hi;
+__device__ __forceinline__ Range range_of(const LevelView &v, uint64_t it) {
+    const uint64_t g = it / v.te;
+    Range r;
+    r.tile = (uint32_t)(it - g * v.te);
+    r.r0 = (uint32_t)(g * v.G);
+    r.r1 = min(v.n_ext, r.r0 + v.G);
+    return r;
+}
+
+// valid data words of tile `tile` (tw words per tile) of a database
+__device__ __forceinline__ uint64_t tile_words_of(uint64_t words, uint32_t tile, uint32_t tw) {
+    const uint64_t w0 = (uint64_t)tile * tw;
+    return w0 >= words ? 0 : min<uint64_t>(tw, words - w0);
+}
+
+// This block's stamps (start, end, algorithmic bytes, candidates).  Plain
+// stores: the launch that follows folds them (fold_stamps), so no block
+// fences or contends on a shared counter here.
+__device__ __forceinline__ void launch_epilogue(const SupportArgs &a, unsigned long long t_start,
+                                                unsigned long long bytes, unsigned long long cands) {
+    if (!a.stamps || threadIdx.x != 0) return;
+    unsigned long long *s = a.stamps + 4ull * blockIdx.x;
+    s[0] = t_start;
+    s[1] = wall_clock64();
+    s[2] = bytes;
+    s[3] = cands;
 }
 
 // ---- frequency ----------------------------------------------------------------
 
 template <int QPL>  // uint4 per lane per tile: tile = 512 * QPL words
 __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
-    if (a.stop && *a.stop) return;
     LevelView v;
     if (!level_view(a, v)) return;
     uint32_t *__restrict__ sup = a.L.sup;
@@ -105,75 +123,86 @@ __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr uint32_t kQuadsPerTile = 256 * QPL;
     const unsigned long long t_start = wall_clock64();
+    unsigned long long bytes = 0, cands = 0;
     for (uint64_t it = blockIdx.x; it < v.total; it += gridDim.x) {
-        Item w;
-        if (!decode(v, it, w)) continue;
-        const gptr<uint4> store = to_global(reinterpret_cast<const uint4 *>(w.db->store));
-        const uint64_t stride_q = w.db->stride / 2;
-        const uint64_t q0 = (uint64_t)w.tile * kQuadsPerTile + tid;
-        uint4 P[QPL];
-        bool valid[QPL];
+        const Range r = range_of(v, it);
+        uint32_t t = find_task(v.ext_off, v.n_tasks, r.r0);
+        for (uint32_t e0 = r.r0; e0 < r.r1;) {
+            while (v.ext_off[t + 1] <= e0) ++t;  // empty tasks
+            const uint32_t e1 = min(r.r1, v.ext_off[t + 1]);
+            const cptr<DbDev> db = v.dbs + v.dbi[t];
+            if (r.tile == 0) cands += e1 - e0;
+            if (r.tile >= db->tiles) {
+                if (lane == 0)
+                    for (uint32_t e = e0; e < e1; ++e) part[wave][e - r.r0] = 0;
+                e0 = e1;
+                continue;
+            }
+            bytes += 8ull * tile_words_of(db->words, r.tile, 2 * kQuadsPerTile) * (v.d + e1 - e0);
+            const gptr<uint4> store = to_global(reinterpret_cast<const uint4 *>(db->store));
+            const uint64_t stride_q = db->stride / 2;
+            const uint64_t q0 = (uint64_t)r.tile * kQuadsPerTile + tid;
+            uint4 P[QPL];
+            bool valid[QPL];
 #pragma unroll
-        for (int r = 0; r < QPL; ++r) {
-            valid[r] = q0 + (uint64_t)r * 256 < stride_q;
-            P[r] = valid[r] ? make_uint4(~0u, ~0u, ~0u, ~0u) : make_uint4(0, 0, 0, 0);
-        }
-        const cptr<uint32_t> pre = v.prefix + (uint64_t)w.t * kMaxDepth;
-        for (uint32_t j = 0; j < v.d; ++j) {
-            const gptr<uint4> row = store + (uint64_t)col_of(v.fcol, pre[j]) * stride_q + q0;
+            for (int q = 0; q < QPL; ++q) {
+                valid[q] = q0 + (uint64_t)q * 256 < stride_q;
+                P[q] = valid[q] ? make_uint4(~0u, ~0u, ~0u, ~0u) : make_uint4(0, 0, 0, 0);
+            }
+            const cptr<uint32_t> pre = v.prefix + (uint64_t)t * kMaxDepth;
+            for (uint32_t j = 0; j < v.d; ++j) {
+                const gptr<uint4> row = store + (uint64_t)col_of(v.fcol, pre[j]) * stride_q + q0;
 #pragma unroll
-            for (int r = 0; r < QPL; ++r)
-                if (valid[r]) P[r] = and4(P[r], row[r * 256]);
-        }
-        uint32_t e = w.e0;
-        for (; e + 1 < w.e1; e += 2) {
-            const gptr<uint4> ra = store + (uint64_t)col_of(v.fcol, v.ext[e]) * stride_q + q0;
-            const gptr<uint4> rb = store + (uint64_t)col_of(v.fcol, v.ext[e + 1]) * stride_q + q0;
-            uint4 A[QPL], B[QPL];
+                for (int q = 0; q < QPL; ++q)
+                    if (valid[q]) P[q] = and4(P[q], row[q * 256]);
+            }
+            uint32_t e = e0;
+            for (; e + 1 < e1; e += 2) {
+                const gptr<uint4> ra = store + (uint64_t)col_of(v.fcol, v.ext[e]) * stride_q + q0;
+                const gptr<uint4> rb = store + (uint64_t)col_of(v.fcol, v.ext[e + 1]) * stride_q + q0;
+                uint4 A[QPL], B[QPL];
 #pragma unroll
-            for (int r = 0; r < QPL; ++r) {
-                A[r] = make_uint4(0, 0, 0, 0);
-                B[r] = make_uint4(0, 0, 0, 0);
-                if (valid[r]) {
-                    A[r] = ra[r * 256];
-                    B[r] = rb[r * 256];
+                for (int q = 0; q < QPL; ++q) {
+                    A[q] = make_uint4(0, 0, 0, 0);
+                    B[q] = make_uint4(0, 0, 0, 0);
+                    if (valid[q]) {
+                        A[q] = ra[q * 256];
+                        B[q] = rb[q * 256];
+                    }
+                }
+                uint32_t ca = 0, cb = 0;
+#pragma unroll
+                for (int q = 0; q < QPL; ++q) {
+                    ca += pop4(and4(P[q], A[q]));
+                    cb += pop4(and4(P[q], B[q]));
+                }
+                ca = wave_sum(ca);
+                cb = wave_sum(cb);
+                if (lane == 0) {
+                    part[wave][e - r.r0] = ca;
+                    part[wave][e + 1 - r.r0] = cb;
                 }
             }
-            uint32_t ca = 0, cb = 0;
+            if (e < e1) {
+                const gptr<uint4> ra = store + (uint64_t)col_of(v.fcol, v.ext[e]) * stride_q + q0;
+                uint32_t ca = 0;
 #pragma unroll
-            for (int r = 0; r < QPL; ++r) {
-                ca += pop4(and4(P[r], A[r]));
-                cb += pop4(and4(P[r], B[r]));
+                for (int q = 0; q < QPL; ++q)
+                    if (valid[q]) ca += pop4(and4(P[q], ra[q * 256]));
+                ca = wave_sum(ca);
+                if (lane == 0) part[wave][e - r.r0] = ca;
             }
-            ca = wave_sum(ca);
-            cb = wave_sum(cb);
-            if (lane == 0) {
-                part[wave][e - w.e0] = ca;
-                part[wave][e + 1 - w.e0] = cb;
-            }
-        }
-        if (e < w.e1) {
-            const gptr<uint4> ra = store + (uint64_t)col_of(v.fcol, v.ext[e]) * stride_q + q0;
-            uint32_t ca = 0;
-#pragma unroll
-            for (int r = 0; r < QPL; ++r)
-                if (valid[r]) ca += pop4(and4(P[r], ra[r * 256]));
-            ca = wave_sum(ca);
-            if (lane == 0) part[wave][e - w.e0] = ca;
+            e0 = e1;
         }
         __syncthreads();
-        if (tid < w.e1 - w.e0) {
+        if (tid < r.r1 - r.r0) {
             const uint32_t c = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
-            if (c) atomicAdd(sup + w.e0 + tid, c);
+            if (c) atomicAdd(sup + r.r0 + tid, c);
         }
         __syncthreads();
     }
-    if (a.clock && tid == 0) {
-        a.clock[2 * blockIdx.x] = t_start;
-        a.clock[2 * blockIdx.x + 1] = wall_clock64();
-    }
+    launch_epilogue(a, t_start, bytes, cands);
 }
-
 
 // ---- periodicity ----------------------------------------------------------------
 
@@ -246,58 +275,68 @@ __device__ __forceinline__ void pp_scan_word(PeriodPart &m, uint64_t v, uint64_t
 __global__ __launch_bounds__(256) void k_support_periodic(SupportArgs a) {
     __shared__ PeriodPart wpart[4][16];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (a.stop && *a.stop) return;
     LevelView lv;
     if (!level_view(a, lv)) return;
     const unsigned long long t_start = wall_clock64();
+    unsigned long long bytes = 0, cands = 0;
     for (uint64_t it = blockIdx.x; it < lv.total; it += gridDim.x) {
-        Item w;
-        if (!decode(lv, it, w)) continue;
-        const gptr<uint64_t> store = to_global(w.db->store);
-        const uint64_t stride = w.db->stride;
-        const uint64_t w0 = (uint64_t)w.tile * kTileWords + (uint64_t)tid * 8;
-        const bool valid = w0 < stride;  // stride is a multiple of 64 words
-        uint64_t P[8];
-#pragma unroll
-        for (int v = 0; v < 8; ++v) P[v] = valid ? ~0ull : 0ull;
-        const cptr<uint32_t> pre = lv.prefix + (uint64_t)w.t * kMaxDepth;
-        if (valid) {
-            for (uint32_t j = 0; j < lv.d; ++j) {
-                const gptr<uint64_t> row = store + (uint64_t)col_of(lv.fcol, pre[j]) * stride + w0;
-#pragma unroll
-                for (int v = 0; v < 8; ++v) P[v] &= row[v];
+        const Range r = range_of(lv, it);
+        uint32_t t = find_task(lv.ext_off, lv.n_tasks, r.r0);
+        for (uint32_t e0 = r.r0; e0 < r.r1;) {
+            while (lv.ext_off[t + 1] <= e0) ++t;  // empty tasks
+            const uint32_t e1 = min(r.r1, lv.ext_off[t + 1]);
+            const cptr<DbDev> db = lv.dbs + lv.dbi[t];
+            if (r.tile == 0) cands += e1 - e0;
+            if (r.tile >= db->tiles) {  // the fold reads tiles < db->tiles only
+                e0 = e1;
+                continue;
             }
-        }
-        for (uint32_t eb = w.e0; eb < w.e1; eb += 16) {
-            const uint32_t ee = min(w.e1, eb + 16);
-            for (uint32_t e = eb; e < ee; ++e) {
-                PeriodPart m;
-                pp_empty(m);
-                if (valid) {
-                    const gptr<uint64_t> row = store + (uint64_t)col_of(lv.fcol, lv.ext[e]) * stride + w0;
-                    uint64_t X[8];
+            bytes += 8ull * tile_words_of(db->words, r.tile, kTileWords) * (lv.d + e1 - e0);
+            const gptr<uint64_t> store = to_global(db->store);
+            const gptr<int64_t> ts = to_global(db->ts);
+            const uint64_t n = db->n;
+            const uint64_t stride = db->stride;
+            const uint64_t w0 = (uint64_t)r.tile * kTileWords + (uint64_t)tid * 8;
+            const bool valid = w0 < stride;  // stride is a multiple of 64 words
+            uint64_t P[8];
 #pragma unroll
-                    for (int v = 0; v < 8; ++v) X[v] = P[v] & row[v];
+            for (int v = 0; v < 8; ++v) P[v] = valid ? ~0ull : 0ull;
+            const cptr<uint32_t> pre = lv.prefix + (uint64_t)t * kMaxDepth;
+            if (valid) {
+                for (uint32_t j = 0; j < lv.d; ++j) {
+                    const gptr<uint64_t> row = store + (uint64_t)col_of(lv.fcol, pre[j]) * stride + w0;
 #pragma unroll
-                    for (int v = 0; v < 8; ++v)
-                        pp_scan_word(m, X[v], (w0 + v) * 64, to_global(w.db->ts), w.db->n);
+                    for (int v = 0; v < 8; ++v) P[v] &= row[v];
                 }
-                m = pp_wave_reduce(m, lane);
-                if (lane == 0) wpart[wave][e - eb] = m;
             }
-            __syncthreads();
-            if (tid < ee - eb) {
-                const PeriodPart m = pp_combine(pp_combine(wpart[0][tid], wpart[1][tid]),
-                                                pp_combine(wpart[2][tid], wpart[3][tid]));
-                a.parts[(uint64_t)(eb + tid) * a.parts_tiles + w.tile] = m;
+            for (uint32_t eb = e0; eb < e1; eb += 16) {
+                const uint32_t ee = min(e1, eb + 16);
+                for (uint32_t e = eb; e < ee; ++e) {
+                    PeriodPart m;
+                    pp_empty(m);
+                    if (valid) {
+                        const gptr<uint64_t> row = store + (uint64_t)col_of(lv.fcol, lv.ext[e]) * stride + w0;
+                        uint64_t X[8];
+#pragma unroll
+                        for (int v = 0; v < 8; ++v) X[v] = P[v] & row[v];
+#pragma unroll
+                        for (int v = 0; v < 8; ++v) pp_scan_word(m, X[v], (w0 + v) * 64, ts, n);
+                    }
+                    m = pp_wave_reduce(m, lane);
+                    if (lane == 0) wpart[wave][e - eb] = m;
+                }
+                __syncthreads();
+                if (tid < ee - eb) {
+                    const PeriodPart m = pp_combine(pp_combine(wpart[0][tid], wpart[1][tid]),
+                                                    pp_combine(wpart[2][tid], wpart[3][tid]));
+                    a.parts[(uint64_t)(eb + tid) * a.parts_tiles + r.tile] = m;
+                }
+                __syncthreads();
             }
-            __syncthreads();
+            e0 = e1;
         }
     }
-    if (a.clock && tid == 0) {
-        a.clock[2 * blockIdx.x] = t_start;
-        a.clock[2 * blockIdx.x + 1] = wall_clock64();
-    }
+    launch_epilogue(a, t_start, bytes, cands);
 }
 
 // Fold the tile partials of each entry in row order: support and the largest
@@ -351,7 +390,7 @@ __global__ __launch_bounds__(256) void k_intersect(const uint64_t *__restrict__ 
 }  // namespace
 
 int launch_support_count(cpm_ctx *ctx, const SupportArgs &a, uint32_t tile_words, uint32_t grid) {
-    if (a.G == 0 || a.G > 64) return fail(CPM_ERR_INVALID, "group size %u", a.G);
+    if (a.te == 0) return fail(CPM_ERR_INVALID, "zero tiles per work range");
     if (tile_words == 1024)
         hipLaunchKernelGGL(k_support_count<2>, dim3(grid), dim3(256), 0, ctx->stream, a);
     else if (tile_words == 2048)
@@ -363,7 +402,7 @@ int launch_support_count(cpm_ctx *ctx, const SupportArgs &a, uint32_t tile_words
 }
 
 int launch_support_periodic(cpm_ctx *ctx, const SupportArgs &a, uint32_t grid) {
-    if (a.G == 0 || a.G > 64) return fail(CPM_ERR_INVALID, "group size %u", a.G);
+    if (a.te == 0) return fail(CPM_ERR_INVALID, "zero tiles per work range");
     hipLaunchKernelGGL(k_support_periodic, dim3(grid), dim3(256), 0, ctx->stream, a);
     CPM_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_periodic_finalize, dim3(grid), dim3(256), 0, ctx->stream, a);

@@ -31,21 +31,20 @@ struct DbDev {
 struct LevelSizes {
     unsigned long long n_tasks, d, n_ext, n_work;
     unsigned long long n_pat, n_child, n_cext;  // totals of planning this level
+    unsigned long long n_cand;                  // entries before the subset test
+    unsigned long long group;                   // entries per support work range
     unsigned long long pad;
 };
 
 // A level's arrays.  Task t: prefix items prefix[t*kMaxDepth .. +d), database
 // db[t], entries ext[ext_off[t] .. ext_off[t+1]) (entry e is the candidate
-// prefix + {ext[e]}), support work items [work_off[t], work_off[t+1]).
-// Entries hold global frequent-item indexes (through fcol) or, at level 1,
-// columns of the task's database.
+// prefix + {ext[e]}).  Entries hold global frequent-item indexes (through
+// fcol) or, at level 1, columns of the task's database.
 struct LevelPtrs {
     uint32_t *prefix = nullptr;
     uint32_t *pid = nullptr;
     uint32_t *db = nullptr;
     uint32_t *ext_off = nullptr;
-    uint32_t *work_off = nullptr;
-    uint32_t *nwork = nullptr;
     uint32_t *ext = nullptr;
     uint32_t *sup = nullptr;
     int64_t *per = nullptr;
@@ -59,25 +58,45 @@ struct PeriodPart {
     uint32_t count, pad;
 };
 
+// Totals of the support launches, in device memory (fold_stamps).
+struct SupportAcc {
+    unsigned long long ticks;       // summed launch spans (wall-clock ticks)
+    unsigned long long launches;
+    unsigned long long bytes;       // algorithmic bitset bytes streamed
+    unsigned long long candidates;  // entries counted
+};
+
+// Support work: the level's entries are cut into ranges of G consecutive
+// entries (a range may span several tasks), and work item i covers range
+// i / te on tile i % te of every database's rows.  te is the largest
+// database's tile count, rounded up to a multiple of 8 when >= 8; as the grid
+// is a multiple of 8 too, a block -- which stays on XCD blockIdx % 8 -- only
+// touches tiles congruent to its XCD, so each XCD's L2 holds its own slice of
+// the hot rows.
 struct SupportArgs {
     LevelPtrs L;
     const DbDev *dbs = nullptr;
     const uint32_t *fcol = nullptr;  // global frequent index -> column (null: entries are columns)
-    uint32_t G = 32;                 // entries per work item
+    uint32_t te = 1;                 // tiles per work range (entries per range: L.sz->group)
     const uint32_t *stop = nullptr;  // skip the launch when *stop != 0
     PeriodPart *parts = nullptr;     // periodic: n_ext x parts_tiles partials
     uint32_t parts_tiles = 0;
-    // per-block device clock stamps of the launch (wall_clock64 ticks):
-    // clock[2*b] start, clock[2*b+1] end of block b; null to skip
-    unsigned long long *clock = nullptr;
+    // per-block stamps (start, end, bytes, candidates), one set per grid
+    // block; null to skip
+    unsigned long long *stamps = nullptr;
 };
 
-// Number of support work items of a task with k entries in a database with
-// tiles_eff tiles per row.  When rows have 8 or more tiles, tiles_eff is a
-// multiple of 8 and so is every task's work count: work item i then always
-// handles a tile congruent to i mod 8, i.e. to its block's XCD.
-__host__ __device__ inline uint32_t task_work(uint32_t k, uint32_t G, uint32_t tiles_eff) {
-    return (k + G - 1) / G * tiles_eff;
+// Entries per work range for a level of n_ext entries: at most G, and small
+// enough that the level has about `target` work items when it is short (a
+// short level's tasks are small, and a block walks the tasks of its range one
+// after another).
+__host__ __device__ inline uint32_t level_group(uint64_t n_ext, uint32_t G, uint32_t te, uint64_t target) {
+    const uint64_t g = (n_ext * te + target - 1) / target;
+    return (uint32_t)(g < 1 ? 1 : (g > G ? G : g));
+}
+
+__host__ __device__ inline uint64_t level_work(uint64_t n_ext, uint32_t g, uint32_t te) {
+    return (n_ext + g - 1) / g * te;
 }
 
 inline uint32_t tiles_eff_of(uint32_t tiles) { return tiles >= 8 ? (tiles + 7) / 8 * 8 : tiles; }
