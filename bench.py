@@ -99,6 +99,7 @@ def main():
     dev = torch.device("cuda:0")
     d_rows = torch.from_numpy(db.rows.view(np.int64)).to(dev)
     d_cols = torch.from_numpy(cols.view(np.int32)).to(dev)
+    d_ts = torch.from_numpy(db.timestamps.astype(np.int64)).to(dev) if periodic else None
     torch.cuda.synchronize()
     ctx = _hip.context(0)
     ctx.set_search_options(args.group, args.tile, not args.no_graph, True)
@@ -107,7 +108,7 @@ def main():
         bits = ctx.build_bits_device(d_rows.data_ptr(), d_cols.data_ptr(), nnz, n, items)
         try:
             if periodic:
-                return ctx.pfeclat_mine(bits, db.timestamps, end, minsup, PF_MAXPER)
+                return ctx.pfeclat_mine_device(bits, d_ts.data_ptr(), end, minsup, PF_MAXPER)
             return ctx.eclat_mine(bits, minsup, use_gram=True)
         finally:
             bits.close()

@@ -55,6 +55,8 @@ SIGNATURES = {
                                       ctypes.POINTER(_vp)]),
     "cpm_pfeclat_mine": (ctypes.c_int, [_vp, _vp, _i64p, ctypes.c_int64, ctypes.c_uint64,
                                         ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "cpm_pfeclat_mine_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_uint64,
+                                        ctypes.c_int64, ctypes.POINTER(_vp)]),
     "cpm_unique_pairs": (ctypes.c_int, [_vp, _u64p, _u32p, ctypes.c_uint64, _u64p, _u32p, _u64p]),
     "cpm_fptree_build": (ctypes.c_int, [_vp, _u64p, _u32p, ctypes.c_uint64, _u64p, ctypes.c_uint64,
                                         ctypes.c_uint32, ctypes.c_uint64, _u64p,
@@ -260,6 +262,15 @@ class Context:
         handle = _vp()
         check(load().cpm_pfeclat_mine(self.handle, bits.handle, _p(ts, _i64p), int(end),
                                       clamp_u64(minsup), int(maxper_floor), ctypes.byref(handle)))
+        return Patterns(handle, periodic=True)
+
+    def pfeclat_mine_device(self, bits, timestamps_ptr, end, minsup, maxper_floor):
+        """cpm_pfeclat_mine with device-resident timestamps (an int64 device
+        pointer, e.g. ``tensor.data_ptr()`` of a torch tensor on this device)."""
+        handle = _vp()
+        check(load().cpm_pfeclat_mine_device(self.handle, bits.handle, ctypes.c_void_p(int(timestamps_ptr)),
+                                             int(end), clamp_u64(minsup), int(maxper_floor),
+                                             ctypes.byref(handle)))
         return Patterns(handle, periodic=True)
 
 

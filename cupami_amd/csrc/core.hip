@@ -261,17 +261,21 @@ int cpm_bits_build_device(cpm_ctx *ctx, const uint64_t *d_rows, const uint32_t *
     bool tiled = false;
     CPM_CHECK(ctx->scratch[5].ensure(2 * sizeof(unsigned int)));
     unsigned int *flags = ctx->scratch[5].as<unsigned int>();
+    // pinned read-back slots: [0] flags, [8] last row, [16] bad flag
+    CPM_CHECK(ctx->host_scratch.ensure(64));
+    unsigned char *hs = ctx->host_scratch.as<unsigned char>();
     if (nnz && tileB >= 1) {
-        unsigned int hf[2] = {0, 0};
+        unsigned int *hf = reinterpret_cast<unsigned int *>(hs);
+        uint64_t *hlast = reinterpret_cast<uint64_t *>(hs + 8);
         CPM_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(unsigned int), ctx->stream));
         const uint64_t g = std::min<uint64_t>((nnz + 255) / 256, (uint64_t)ctx->cu_count * 16);
         hipLaunchKernelGGL(k_rows_sorted, dim3((unsigned)g), dim3(256), 0, ctx->stream, d_rows, nnz,
                            flags);
         CPM_HIP(hipGetLastError());
-        uint64_t last = 0;
-        CPM_HIP(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, ctx->stream));
-        CPM_HIP(hipMemcpyAsync(&last, d_rows + nnz - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+        CPM_HIP(hipMemcpyAsync(hf, flags, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, ctx->stream));
+        CPM_HIP(hipMemcpyAsync(hlast, d_rows + nnz - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
         CPM_HIP(hipStreamSynchronize(ctx->stream));
+        const uint64_t last = *hlast;
         if (hf[0] == 0 && last >= n_trans)
             return fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
                         (unsigned long long)n_trans, n_items);
@@ -286,15 +290,16 @@ int cpm_bits_build_device(cpm_ctx *ctx, const uint64_t *d_rows, const uint32_t *
                            (size_t)n_items * tileB * 8, ctx->stream, d_rows, d_cols, nnz, n_trans,
                            n_items, b->stride, tileB, b->store.as<uint64_t>(), bad);
         hipError_t e = hipGetLastError();
-        unsigned int hbad = 0;
+        unsigned int *hbad = reinterpret_cast<unsigned int *>(hs + 16);
+        *hbad = 0;
         if (e == hipSuccess)
-            e = hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, ctx->stream);
+            e = hipMemcpyAsync(hbad, bad, sizeof(unsigned int), hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) {
             cpm_bits_destroy(b);
             return fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(e));
         }
-        if (hbad) {
+        if (*hbad) {
             cpm_bits_destroy(b);
             return fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
                         (unsigned long long)n_trans, n_items);

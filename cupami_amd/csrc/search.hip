@@ -717,6 +717,9 @@ struct SearchWs {
     DBuf ctl, dbs, fcol, pairbits, gram, goff, gjobs, gdbs, l2, part, partl, pat_parent, pat_item, pat_sup,
         pat_per, ts, stamps;
     HBuf host;
+    HBuf stage;           // pinned staging of small host <-> device copies
+    size_t stage_off = 0; // in use since the last stream sync
+    HBuf down;            // pinned landing area of the pattern download
     Caps caps{};
     bool graphs_ok[2] = {false, false};
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -920,6 +923,31 @@ struct Search {
     int read_ctl(Ctl *h) {
         CPM_HIP(hipMemcpyAsync(h, w.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
         CPM_HIP(hipStreamSynchronize(ctx->stream));
+        w.stage_off = 0;
+        return CPM_OK;
+    }
+
+    // n bytes of pinned staging, valid until the next stream sync.  (A copy
+    // from or to pageable memory makes the runtime stage it synchronously.)
+    int stage(size_t n, unsigned char **h) {
+        const size_t end = (w.stage_off + n + 63) / 64 * 64;
+        if (!w.stage.p || end > w.stage.cap) {
+            CPM_HIP(hipStreamSynchronize(ctx->stream));
+            w.stage_off = 0;
+            CPM_CHECK(w.stage.ensure(std::max<size_t>(2 * w.stage.cap, std::max<size_t>(n + 64, 1 << 16))));
+        }
+        *h = w.stage.as<unsigned char>() + w.stage_off;
+        w.stage_off = (w.stage_off + n + 63) / 64 * 64;
+        return CPM_OK;
+    }
+
+    // asynchronous host -> device copy of pageable data through staging
+    int put(void *dst, const void *src, size_t n) {
+        if (n == 0) return CPM_OK;
+        unsigned char *h = nullptr;
+        CPM_CHECK(stage(n, &h));
+        memcpy(h, src, n);
+        CPM_HIP(hipMemcpyAsync(dst, h, n, hipMemcpyHostToDevice, ctx->stream));
         return CPM_OK;
     }
 
@@ -957,7 +985,7 @@ struct Search {
             total_items += b->n_items;
         }
         CPM_CHECK(w.dbs.ensure(std::max<uint32_t>(n_db, 1) * sizeof(DbDev)));
-        CPM_HIP(hipMemcpyAsync(w.dbs.p, dbs.data(), n_db * sizeof(DbDev), hipMemcpyHostToDevice, st));
+        CPM_CHECK(put(w.dbs.p, dbs.data(), n_db * sizeof(DbDev)));
 
         // ---- level 1 (set 0) ----
         if (n_db > w.caps.tasks || total_items > w.caps.ext) {
@@ -977,7 +1005,7 @@ struct Search {
                 e += in[i].bits->n_items;
             }
             off[n_db] = (uint32_t)e;
-            CPM_HIP(hipMemcpyAsync(S0.ext_off.p, off.data(), (n_db + 1) * 4, hipMemcpyHostToDevice, st));
+            CPM_CHECK(put(S0.ext_off.p, off.data(), (n_db + 1) * 4));
             LevelSizes l1;
             memset(&l1, 0, sizeof(l1));
             l1.n_tasks = n_db;
@@ -986,23 +1014,26 @@ struct Search {
             l1.n_cand = e;
             l1.group = level_group(e, G, te, work_target());
             l1.n_work = level_work(e, (uint32_t)l1.group, te);
-            CPM_HIP(hipMemcpyAsync(sz(0), &l1, sizeof(l1), hipMemcpyHostToDevice, st));
+            CPM_CHECK(put(sz(0), &l1, sizeof(l1)));
             hipLaunchKernelGGL(k_level1, dim3(std::min<uint32_t>(n_db, 4096)), dim3(256), 0, st, n_db,
                                S0.ext_off.as<uint32_t>(), S0.ext.as<uint32_t>(), S0.sup.as<uint32_t>(),
                                S0.db.as<uint32_t>(), S0.pid.as<uint32_t>());
             CPM_HIP(hipGetLastError());
             CPM_CHECK(count(0, true));
-            // the host vectors must outlive the copies
-            CPM_HIP(hipStreamSynchronize(st));
         }
         // plan level 1: the frequent items (no children)
         plan(plan_args(0, 0));
         CPM_HIP(hipGetLastError());
         nf.assign(n_db, 0);
         std::vector<uint32_t> fbase(n_db + 1);
-        CPM_HIP(hipMemcpyAsync(nf.data(), S0.cnt[0].p, n_db * 4, hipMemcpyDeviceToHost, st));
-        CPM_HIP(hipMemcpyAsync(fbase.data(), S0.off[0].p, (n_db + 1) * 4, hipMemcpyDeviceToHost, st));
+        unsigned char *h_nf = nullptr;  // one area: a second stage() call may regrow the arena
+        CPM_CHECK(stage((size_t)(2 * n_db + 1) * 4, &h_nf));
+        unsigned char *h_fb = h_nf + (size_t)n_db * 4;
+        CPM_HIP(hipMemcpyAsync(h_nf, S0.cnt[0].p, n_db * 4, hipMemcpyDeviceToHost, st));
+        CPM_HIP(hipMemcpyAsync(h_fb, S0.off[0].p, (n_db + 1) * 4, hipMemcpyDeviceToHost, st));
         CPM_CHECK(read_ctl(hc));
+        memcpy(nf.data(), h_nf, n_db * 4);
+        memcpy(fbase.data(), h_fb, (n_db + 1) * 4);
         if (hc->overflow) {
             *overflow = hc->overflow;
             return CPM_OK;
@@ -1038,7 +1069,7 @@ struct Search {
             for (auto &D : dbs) D.pair_w32 = 0;
             pair_words = 0;
         }
-        CPM_HIP(hipMemcpyAsync(w.dbs.p, dbs.data(), n_db * sizeof(DbDev), hipMemcpyHostToDevice, st));
+        CPM_CHECK(put(w.dbs.p, dbs.data(), n_db * sizeof(DbDev)));
         if (t2 == 0) return CPM_OK;
         if (t2 > w.caps.tasks || e2 > w.caps.ext || e2 >= 0xffffffffull) {
             *overflow = (t2 > w.caps.tasks ? kOvTasks : 0) | (e2 > w.caps.ext ? kOvEntries : 0);
@@ -1051,7 +1082,7 @@ struct Search {
         }
         LevelSet &S1 = w.set[1];
         CPM_CHECK(w.l2.ensure(n_db * sizeof(L2Db)));
-        CPM_HIP(hipMemcpyAsync(w.l2.p, l2.data(), n_db * sizeof(L2Db), hipMemcpyHostToDevice, st));
+        CPM_CHECK(put(w.l2.p, l2.data(), n_db * sizeof(L2Db)));
         {
             LevelSizes ls;
             memset(&ls, 0, sizeof(ls));
@@ -1061,13 +1092,12 @@ struct Search {
             ls.n_cand = e2;
             ls.group = level_group(e2, G, te, work_target());
             ls.n_work = level_work(e2, (uint32_t)ls.group, te);
-            CPM_HIP(hipMemcpyAsync(sz(1), &ls, sizeof(ls), hipMemcpyHostToDevice, st));
+            CPM_CHECK(put(sz(1), &ls, sizeof(ls)));
             const uint32_t e2u = (uint32_t)e2;
-            CPM_HIP(hipMemcpyAsync(S1.ext_off.as<uint32_t>() + t2, &e2u, 4, hipMemcpyHostToDevice, st));
+            CPM_CHECK(put(S1.ext_off.as<uint32_t>() + t2, &e2u, 4));
             hipLaunchKernelGGL(k_level2, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0, st,
                                w.l2.as<L2Db>(), n_db, (uint32_t)t2, S1.ptrs(sz(1)), w.dbs.as<DbDev>());
             CPM_HIP(hipGetLastError());
-            CPM_HIP(hipStreamSynchronize(st));
         }
         bool gram_ok = use_gram && !periodic;
         for (uint32_t i = 0; i < n_db && gram_ok; ++i)
@@ -1095,11 +1125,9 @@ struct Search {
             CPM_CHECK(w.goff.ensure(n_db * 8));
             CPM_CHECK(w.gjobs.ensure(std::max<size_t>(jobs.size(), 1) * sizeof(GramJob)));
             CPM_CHECK(w.gdbs.ensure(n_db * sizeof(GramDb)));
-            CPM_HIP(hipMemcpyAsync(w.goff.p, goff.data(), n_db * 8, hipMemcpyHostToDevice, st));
-            if (!jobs.empty())
-                CPM_HIP(hipMemcpyAsync(w.gjobs.p, jobs.data(), jobs.size() * sizeof(GramJob),
-                                       hipMemcpyHostToDevice, st));
-            CPM_HIP(hipMemcpyAsync(w.gdbs.p, gdb.data(), n_db * sizeof(GramDb), hipMemcpyHostToDevice, st));
+            CPM_CHECK(put(w.goff.p, goff.data(), n_db * 8));
+            CPM_CHECK(put(w.gjobs.p, jobs.data(), jobs.size() * sizeof(GramJob)));
+            CPM_CHECK(put(w.gdbs.p, gdb.data(), n_db * sizeof(GramDb)));
             CPM_CHECK(launch_gram_i8(ctx, w.gjobs.as<GramJob>(), jobs.size(), w.gdbs.as<GramDb>(),
                                      w.gram.as<int32_t>(), elems, ops));
             hipLaunchKernelGGL(k_gram_to_level2, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0,
@@ -1194,16 +1222,27 @@ struct Search {
         if (periodic) p->period.resize(n);
         hipStream_t st = ctx->stream;
         if (n) {
-            hipError_t e = hipMemcpyAsync(p->parent.data(), w.pat_parent.p, n * 4, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipMemcpyAsync(p->item.data(), w.pat_item.p, n * 4, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipMemcpyAsync(s32.data(), w.pat_sup.p, n * 4, hipMemcpyDeviceToHost, st);
+            const size_t row = periodic ? 20 : 12;
+            int sc = w.down.ensure(n * row);
+            if (sc != CPM_OK) {
+                delete p;
+                return sc;
+            }
+            unsigned char *h = w.down.as<unsigned char>();
+            hipError_t e = hipMemcpyAsync(h, w.pat_parent.p, n * 4, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(h + n * 4, w.pat_item.p, n * 4, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(h + n * 8, w.pat_sup.p, n * 4, hipMemcpyDeviceToHost, st);
             if (e == hipSuccess && periodic)
-                e = hipMemcpyAsync(p->period.data(), w.pat_per.p, n * 8, hipMemcpyDeviceToHost, st);
+                e = hipMemcpyAsync(h + n * 12, w.pat_per.p, n * 8, hipMemcpyDeviceToHost, st);
             if (e == hipSuccess) e = hipStreamSynchronize(st);
             if (e != hipSuccess) {
                 delete p;
                 return fail(CPM_ERR_RUNTIME, "pattern download failed: %s", hipGetErrorString(e));
             }
+            memcpy(p->parent.data(), h, n * 4);
+            memcpy(p->item.data(), h + n * 4, n * 4);
+            memcpy(s32.data(), h + n * 8, n * 4);
+            if (periodic) memcpy(p->period.data(), h + n * 12, n * 8);
         }
         // database of each pattern: singletons by range, others by parent
         uint32_t di = 0;
@@ -1286,6 +1325,19 @@ int cpm_eclat_mine(cpm_ctx *ctx, const cpm_bits *bits, uint64_t minsup, int use_
     return cpm_eclat_mine_batch(ctx, &bits, &minsup, 1, use_gram, out);
 }
 
+static int pfeclat_run(cpm_ctx *ctx, const cpm_bits *bits, const int64_t *d_ts, int64_t end, uint64_t minsup,
+                       int64_t maxper_floor, cpm_patterns **out) {
+    Search s(ctx, search_ws(ctx));
+    s.periodic = true;
+    s.use_gram = false;
+    s.G = ctx->opt_group;
+    s.tile_words = kTileWords;
+    s.use_graph = ctx->opt_graph;
+    s.prune = ctx->opt_prune;
+    s.in.push_back({bits, minsup, maxper_floor, end, d_ts});
+    return s.run(out);
+}
+
 int cpm_pfeclat_mine(cpm_ctx *ctx, const cpm_bits *bits, const int64_t *timestamps, int64_t end,
                      uint64_t minsup, int64_t maxper_floor, cpm_patterns **out) {
     clear_error();
@@ -1298,15 +1350,18 @@ int cpm_pfeclat_mine(cpm_ctx *ctx, const cpm_bits *bits, const int64_t *timestam
     const uint64_t n = bits->n_transactions;
     CPM_CHECK(w.ts.ensure((n ? n : 1) * 8));
     if (n) CPM_HIP(hipMemcpyAsync(w.ts.p, timestamps, n * 8, hipMemcpyHostToDevice, ctx->stream));
-    Search s(ctx, w);
-    s.periodic = true;
-    s.use_gram = false;
-    s.G = ctx->opt_group;
-    s.tile_words = kTileWords;
-    s.use_graph = ctx->opt_graph;
-    s.prune = ctx->opt_prune;
-    s.in.push_back({bits, minsup, maxper_floor, end, w.ts.as<int64_t>()});
-    return s.run(out);
+    return pfeclat_run(ctx, bits, w.ts.as<int64_t>(), end, minsup, maxper_floor, out);
+}
+
+int cpm_pfeclat_mine_device(cpm_ctx *ctx, const cpm_bits *bits, const int64_t *d_timestamps, int64_t end,
+                            uint64_t minsup, int64_t maxper_floor, cpm_patterns **out) {
+    clear_error();
+    CPM_CHECK(bind(ctx));
+    if (!out) return fail(CPM_ERR_INVALID, "null out");
+    *out = nullptr;
+    CPM_CHECK(check_bits(bits, ctx));
+    if (bits->n_transactions && !d_timestamps) return fail(CPM_ERR_INVALID, "null timestamps");
+    return pfeclat_run(ctx, bits, d_timestamps, end, minsup, maxper_floor, out);
 }
 
 // Complete-intersection supports (and periodicity) of explicit candidates:

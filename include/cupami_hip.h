@@ -135,6 +135,12 @@ int cpm_set_search_options(cpm_ctx *ctx, uint32_t group, uint32_t tile_words, in
 int cpm_pfeclat_mine(cpm_ctx *ctx, const cpm_bits *bits, const int64_t *timestamps, int64_t end,
                      uint64_t minsup, int64_t maxper_floor, cpm_patterns **out);
 
+/* cpm_pfeclat_mine with the timestamps already in device memory (a caller
+ * that keeps them resident, e.g. a serving loop mining one database
+ * repeatedly); d_timestamps must stay valid until the call returns. */
+int cpm_pfeclat_mine_device(cpm_ctx *ctx, const cpm_bits *bits, const int64_t *d_timestamps, int64_t end,
+                            uint64_t minsup, int64_t maxper_floor, cpm_patterns **out);
+
 /* ---- FP-tree (device arrays) -------------------------------------------- */
 
 /* Replaces _fptree.unique_pairs (_fptree.py:97-105): distinct (row, col)

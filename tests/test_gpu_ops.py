@@ -153,6 +153,24 @@ def test_pfeclat_mine_random(ctx, seed):
         assert ctx.pfeclat_mine(bits, ts, end, ms, per).to_dict() == expect, (ms, per)
 
 
+def test_pfeclat_mine_device_timestamps(ctx):
+    """Device-resident timestamps give the host path's (and the oracle's) forest."""
+    import torch
+    from helpers import temporal_coo
+    rng = random.Random(11)
+    rows = [[rng.randrange(0, 2000), *rng.sample(range(12), rng.randint(0, 12))] for _ in range(900)]
+    labels, rr, cc, n, ts, end = temporal_coo(rows)
+    bits = ctx.build_bits(rr, cc, n, len(labels))
+    host = oracle.bits_build(rr, cc, n, len(labels))
+    d_ts = torch.from_numpy(np.ascontiguousarray(ts, dtype=np.int64)).to("cuda:0")
+    torch.cuda.synchronize()
+    for ms, per in [(2, 2000), (20, 300), (60, 120)]:
+        expect = oracle.pfeclat(host, ts, n, end, ms, per)
+        got = ctx.pfeclat_mine_device(bits, d_ts.data_ptr(), end, ms, per).to_dict()
+        assert got == expect, (ms, per)
+        assert ctx.pfeclat_mine(bits, ts, end, ms, per).to_dict() == expect, (ms, per)
+
+
 @pytest.mark.parametrize("graph,prune,group,tile", [(True, True, 32, 2048), (False, True, 16, 1024),
                                                       (True, False, 64, 1024), (False, False, 8, 2048)])
 def test_search_schedules_agree(ctx, graph, prune, group, tile):
