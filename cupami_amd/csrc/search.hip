@@ -38,6 +38,7 @@ namespace {
 
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kMaskWords = 2048;   // LDS kept mask: up to 65536 frequent items per database
+constexpr uint64_t kTaskMaskBudget = 256ull << 20;  // bytes of per-task kept masks
 
 enum Overflow : uint32_t {
     kOvTasks = 1, kOvEntries = 2, kOvPatterns = 4, kOvDepth = 8, kOvLevelSets = 16, kOvParts = 32,
@@ -71,6 +72,8 @@ struct PlanArgs {
     unsigned long long *partl;  // prune/flags chunk survivor counts: (prune grid)
     unsigned long long *stamps; // the last support launch's per-block stamps
     uint32_t stamp_blocks;
+    uint32_t *tmask;            // kept-item mask of each task (tmask_w words apart), null:
+    uint32_t tmask_w;           //   plan/write writes the child entries itself
     uint32_t *pat_parent, *pat_item, *pat_sup;
     int64_t *pat_per;
     Ctl *ctl;
@@ -127,11 +130,23 @@ __device__ __forceinline__ void mask_clear(uint32_t *lmask, const uint32_t *kept
     for (uint32_t j = threadIdx.x; j < nk; j += blockDim.x) lmask[(ext[kept[a + j]] - fbase) >> 5] = 0;
 }
 
+// Pair-table rows are read 8 words per batch: the loads of a batch are in
+// flight together (a task's largest children decide the plan launches' time,
+// and their walks are latency-bound).
+constexpr uint32_t kRowBatch = 8;
+
 __device__ __forceinline__ uint32_t child_count(const uint32_t *lmask, const uint32_t *prow, uint32_t f,
                                                 uint32_t wlast) {
     uint32_t w = f >> 5;
     uint32_t c = __popc(lmask[w] & prow[w] & above(f));
-    for (++w; w <= wlast; ++w) c += __popc(lmask[w] & prow[w]);
+    for (++w; w + kRowBatch <= wlast + 1; w += kRowBatch) {
+        uint32_t r[kRowBatch];
+#pragma unroll
+        for (uint32_t i = 0; i < kRowBatch; ++i) r[i] = prow[w + i];
+#pragma unroll
+        for (uint32_t i = 0; i < kRowBatch; ++i) c += __popc(lmask[w + i] & r[i]);
+    }
+    for (; w <= wlast; ++w) c += __popc(lmask[w] & prow[w]);
     return c;
 }
 
@@ -358,6 +373,16 @@ __global__ __launch_bounds__(256) void k_plan_offsets(PlanArgs p) {
     }
 }
 
+// largest t < n with off[t] <= i
+__device__ __forceinline__ uint32_t upper_task(const uint32_t *__restrict__ off, uint32_t n, uint64_t i) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 // subset test (cuApriori.py:487-490) ----------------------------------------------
 __device__ __forceinline__ bool prune_active(const PlanArgs &p) {
     return p.prune && !p.ctl->overflow && p.emit_children && p.L.sz->d >= 2 && p.N.sz->n_tasks > 0;
@@ -404,9 +429,13 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             }
         }
         if (!(children && nk >= 2)) continue;
+        const bool fill = p.tmask && p.pairbits;  // k_child_fill writes the entries
         if (p.pairbits) {
             mask_set(lmask, p.kept, p.L.ext, a, nk, D.fbase);
             __syncthreads();
+            if (fill)
+                for (uint32_t q = tid; q < D.pair_w32; q += blockDim.x)
+                    p.tmask[(uint64_t)t * p.tmask_w + q] = lmask[q];
         }
         const uint32_t *table = p.pairbits ? p.pairbits + D.pair_off : nullptr;
         const uint32_t wlast = (p.L.ext[p.kept[a + nk - 1]] - D.fbase) >> 5;
@@ -438,16 +467,25 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
                 p.N.db[ct] = dbi;
                 uint32_t w = xb + carry + off;
                 odst[ct] = w;
-                if (table) {
+                if (fill) {
+                    // entries: k_child_fill
+                } else if (table) {
                     const uint32_t fl = fj - D.fbase;
-                    for (uint32_t q = fl >> 5; q <= wlast; ++q) {
-                        uint32_t m = lmask[q] & prow[q];
-                        if (q == (fl >> 5)) m &= above(fl);
-                        while (m) {
-                            xdst[w] = D.fbase + q * 32 + (uint32_t)__builtin_ctz(m);
-                            if (!stage) p.N.sup[w] = 0;
-                            ++w;
-                            m &= m - 1;
+                    for (uint32_t q0 = fl >> 5; q0 <= wlast; q0 += kRowBatch) {
+                        uint32_t r[kRowBatch];
+#pragma unroll
+                        for (uint32_t i = 0; i < kRowBatch; ++i) r[i] = q0 + i <= wlast ? prow[q0 + i] : 0u;
+#pragma unroll
+                        for (uint32_t i = 0; i < kRowBatch; ++i) {
+                            const uint32_t q = q0 + i;
+                            uint32_t m = q <= wlast ? lmask[q] & r[i] : 0u;
+                            if (q == (fl >> 5)) m &= above(fl);
+                            while (m) {
+                                xdst[w] = D.fbase + q * 32 + (uint32_t)__builtin_ctz(m);
+                                if (!stage) p.N.sup[w] = 0;
+                                ++w;
+                                m &= m - 1;
+                            }
                         }
                     }
                 } else {
@@ -464,6 +502,55 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             __syncthreads();
             mask_clear(lmask, p.kept, p.L.ext, a, nk, D.fbase);
             __syncthreads();
+        }
+    }
+}
+
+// plan/fill: the child entries of plan/write's children, one wave per child
+// across the whole grid (a task's children are not tied to one block: the
+// largest tasks would otherwise decide the launch's time).  Child ct of task
+// t with item f takes the kept items above f whose pair with f is kept: the
+// set bits of tmask[t] AND f's pair-table row, in ascending order.
+__global__ __launch_bounds__(256) void k_child_fill(PlanArgs p) {
+    if (p.ctl->overflow || !p.tmask || !p.pairbits || !p.emit_children) return;
+    const uint32_t n_child = (uint32_t)p.N.sz->n_tasks;
+    if (n_child == 0) return;
+    const uint32_t n_tasks = (uint32_t)p.L.sz->n_tasks;
+    const uint32_t d = (uint32_t)p.L.sz->d;
+    const bool stage = prune_active(p);
+    uint32_t *const xdst = stage ? p.ext2 : p.N.ext;
+    const uint32_t *const odst = stage ? p.off2 : p.N.ext_off;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t ct = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); ct < n_child; ct += waves) {
+        const uint32_t t = upper_task(p.off_child, n_tasks, ct);
+        const DbDev &D = p.dbs[p.L.db[t]];
+        const uint32_t fl = p.N.prefix[(uint64_t)ct * kMaxDepth + d] - D.fbase;
+        const uint32_t *mask = p.tmask + (uint64_t)t * p.tmask_w;
+        const uint32_t *prow = p.pairbits + D.pair_off + (uint64_t)fl * D.pair_w32;
+        uint32_t w = odst[ct];
+        for (uint32_t q0 = fl >> 5; q0 < D.pair_w32; q0 += 64) {
+            const uint32_t q = q0 + lane;
+            uint32_t m = 0;
+            if (q < D.pair_w32) {
+                m = mask[q] & prow[q];
+                if (q == (fl >> 5)) m &= above(fl);
+            }
+            const uint32_t c = __popc(m);
+            uint32_t x = c;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o, 64);
+                if (lane >= (uint32_t)o) x += y;
+            }
+            uint32_t pos = w + x - c;
+            while (m) {
+                xdst[pos] = D.fbase + q * 32 + (uint32_t)__builtin_ctz(m);
+                if (!stage) p.N.sup[pos] = 0;
+                ++pos;
+                m &= m - 1;
+            }
+            w += __shfl(x, 63, 64);
         }
     }
 }
@@ -485,16 +572,6 @@ __device__ __forceinline__ bool lv_contains(const uint32_t *__restrict__ lv, uin
         if (c < 0) lo = mid + 1; else hi = mid;
     }
     return false;
-}
-
-// largest t < n with off[t] <= i
-__device__ __forceinline__ uint32_t upper_task(const uint32_t *__restrict__ off, uint32_t n, uint64_t i) {
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= i) lo = mid; else hi = mid;
-    }
-    return lo;
 }
 
 // prune/flags: a contiguous chunk of staged candidates per block ---------------------
@@ -707,14 +784,14 @@ struct LevelSet {
 struct GraphKey {
     const void *ptrs[64];
     uint64_t caps[5];
-    uint32_t G, te, tile_words, parts_tiles;
+    uint32_t G, te, tmask_w, tile_words, parts_tiles;
     int periodic, prune;
     bool operator==(const GraphKey &o) const { return memcmp(this, &o, sizeof(GraphKey)) == 0; }
 };
 
 struct SearchWs {
     LevelSet set[2];
-    DBuf ctl, dbs, fcol, pairbits, gram, goff, gjobs, gdbs, l2, part, partl, pat_parent, pat_item, pat_sup,
+    DBuf ctl, dbs, fcol, pairbits, tmask, gram, goff, gjobs, gdbs, l2, part, partl, pat_parent, pat_item, pat_sup,
         pat_per, ts, stamps;
     HBuf host;
     HBuf stage;           // pinned staging of small host <-> device copies
@@ -756,6 +833,7 @@ struct Search {
     bool periodic = false, use_gram = true, prune = true, use_graph = true;
     uint32_t G = 32, tile_words = kTileWords;
     uint32_t te = 1;                 // tiles per support work range
+    uint32_t tmask_w = 0;            // words per task kept mask (0: no plan/fill)
     std::vector<DbDev> dbs;          // host copy
     std::vector<uint32_t> nf;        // frequent items per database
     uint32_t parts_tiles = 0;
@@ -790,6 +868,8 @@ struct Search {
         p.partl = w.partl.as<unsigned long long>();
         p.stamps = w.stamps.as<unsigned long long>();
         p.stamp_blocks = grid();
+        p.tmask = emit_children && tmask_w ? w.tmask.as<uint32_t>() : nullptr;
+        p.tmask_w = tmask_w;
         p.pat_parent = w.pat_parent.as<uint32_t>();
         p.pat_item = w.pat_item.as<uint32_t>();
         p.pat_sup = w.pat_sup.as<uint32_t>();
@@ -835,6 +915,7 @@ struct Search {
         hipLaunchKernelGGL(k_plan_count, dim3(grid_tasks()), dim3(256), 0, ctx->stream, p);
         hipLaunchKernelGGL(k_plan_offsets, dim3(grid_tasks()), dim3(256), 0, ctx->stream, p);
         hipLaunchKernelGGL(k_plan_write, dim3(grid_tasks()), dim3(256), 0, ctx->stream, p);
+        if (p.tmask) hipLaunchKernelGGL(k_child_fill, dim3(grid()), dim3(256), 0, ctx->stream, p);
     }
 
     // plan level c and produce + count level c^1 (one level step)
@@ -860,7 +941,8 @@ struct Search {
             add(S.pos); add(S.lvitems); add(S.parts);
             for (int q = 0; q < 3; ++q) { add(S.cnt[q]); add(S.off[q]); }
         }
-        add(w.ctl); add(w.dbs); add(w.fcol); add(w.pairbits); add(w.part); add(w.partl); add(w.stamps);
+        add(w.ctl); add(w.dbs); add(w.fcol); add(w.pairbits); add(w.tmask); add(w.part); add(w.partl);
+        add(w.stamps);
         add(w.pat_parent); add(w.pat_item); add(w.pat_sup); add(w.pat_per);
         k.caps[0] = w.caps.tasks;
         k.caps[1] = w.caps.ext;
@@ -869,6 +951,7 @@ struct Search {
         k.caps[4] = w.caps.parts;
         k.G = G;
         k.te = te;
+        k.tmask_w = tmask_w;
         k.tile_words = tile_words;
         k.parts_tiles = parts_tiles;
         k.periodic = periodic;
@@ -1138,7 +1221,15 @@ struct Search {
             CPM_CHECK(count(1, false));
         }
         // pair tables
+        tmask_w = 0;
         if (pair_words) {
+            // per-task kept masks for plan/fill, when they fit a modest budget
+            uint32_t wmax = 0;
+            for (auto &D : dbs) wmax = std::max(wmax, D.pair_w32);
+            if ((w.caps.tasks + 1) * wmax * 4 <= kTaskMaskBudget) {
+                tmask_w = wmax;
+                CPM_CHECK(w.tmask.ensure((w.caps.tasks + 1) * wmax * 4));
+            }
             CPM_CHECK(w.pairbits.ensure(pair_words * 4));
             hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<uint64_t>((pair_words + 255) / 256, 16384)),
                                dim3(256), 0, st, w.pairbits.as<uint32_t>(), pair_words, 0u);
