@@ -2,17 +2,25 @@
 //
 // With D the N x F 0/1 matrix of the F item rows, the pair supports are
 // G = D^T D: G[a][b] = |row_a & row_b|.  Each 64-bit word of a row is 64
-// consecutive K elements.  A workgroup owns a 128x128 block of G (upper
+// consecutive K elements.  A workgroup owns a kBT x kBT block of G (upper
 // block triangle only) and one slice of the K (transaction) axis; it stages
-// 16-word tiles of its 128 + 128 rows in LDS and each wave accumulates a
-// 64x64 quadrant with v_mfma_i32_16x16x64_i8 (4 x 4 tiles).  The bits become int8 0/1 in
-// registers: lane group g (lane / 16) takes 16 bits of each word and spreads
-// them into 16 bytes with shift-and-mask; A (row items) and B (column items)
-// use the same lane -> bit map, so every bit is paired with itself exactly
-// once whatever order the hardware gives the K slots.  Accumulation is int32:
-// exact while N < 2^31.  Slices are summed with integer atomics.  One launch
-// serves any number of databases: the job table lists each workgroup's
-// (database, block row, block column, K range).
+// kKW-word tiles of its 2 x kBT rows in LDS, and each of its four waves
+// accumulates a (kBT/2)^2 quadrant with v_mfma_i32_16x16x64_i8, kT x kT tiles
+// of 16 x 16.  The bits become int8 0/1 in registers: lane group g (lane / 16)
+// takes 16 bits of each word and spreads them into 16 bytes with
+// shift-and-mask; A (row items) and B (column items) use the same lane -> bit
+// map, so every bit is paired with itself exactly once whatever order the
+// hardware gives the K slots.
+//
+// The spread is VALU work proportional to the rows a wave touches (2 kT x 16
+// per word) while the MFMA work grows with their product (kT^2 tiles): at
+// kT = 8 a word costs 16 spreads for 64 MFMAs, so the matrix cores, not the
+// VALU, set the pace (at kT = 4 the VALU did).  The 256 accumulator registers
+// leave one wave per SIMD; the next K tile is loaded into registers while the
+// current one is multiplied out of LDS.  Accumulation is int32: exact while
+// N < 2^31.  Slices are summed with integer atomics.  One launch serves any
+// number of databases: the job table lists each workgroup's (database, block
+// row, block column, K range).
 #include "cpm_internal.h"
 
 #include <algorithm>
@@ -23,8 +31,11 @@ namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int kBT = 128;  // block tile (items)
-constexpr int kKW = 16;   // words per K tile
+constexpr int kT = 8;            // 16 x 16 tiles per wave side
+constexpr int kBT = 32 * kT;     // block tile (items): 2 x 2 waves
+constexpr int kKW = 8;           // words per K tile
+constexpr int kTPR = kKW / 8;    // loader threads per row (8 words each)
+static_assert(kBT * kTPR == 256, "one 256-lane block stages the block's rows");
 
 // 16 bits of `w` -> 16 int8 0/1 values for lane group g (0..3).
 __device__ __forceinline__ v4i spread_bits(uint64_t w, uint32_t g) {
@@ -38,10 +49,6 @@ __device__ __forceinline__ v4i spread_bits(uint64_t w, uint32_t g) {
     return r;
 }
 
-// One workgroup: a 128x128 block of G (bi <= bj) over one K slice of one
-// database, as listed in the job table.  Each wave owns a 64x64 quadrant as
-// 4x4 MFMA tiles; the next K tile is loaded into registers while the current
-// one is multiplied out of LDS.
 __global__ __launch_bounds__(256) void k_gram_i8(const GramJob *__restrict__ jobs,
                                                 const GramDb *__restrict__ dbs,
                                                 int32_t *__restrict__ out) {
@@ -59,8 +66,8 @@ __global__ __launch_bounds__(256) void k_gram_i8(const GramJob *__restrict__ job
     int32_t *res = out + db.out_off;
     const uint32_t ld = n;
 
-    // loader: lane pair (2r, 2r+1) stages row r, 8 words each
-    const uint32_t lrow = tid >> 1, lw = (tid & 1) * 8;
+    // loader: thread (row lrow, word group lw) stages 8 words of A and of B
+    const uint32_t lrow = tid / kTPR, lw = (tid % kTPR) * 8;
     const uint32_t ia = bi * kBT + lrow, ib = bj * kBT + lrow;
     const gptr<uint4> pa = ia < n ? (gptr<uint4>)(store + (uint64_t)cols[ia] * stride) : nullptr;
     const gptr<uint4> pb = ib < n ? (gptr<uint4>)(store + (uint64_t)cols[ib] * stride) : nullptr;
@@ -90,11 +97,11 @@ __global__ __launch_bounds__(256) void k_gram_i8(const GramJob *__restrict__ job
     const uint32_t li = lane & 15, lg = lane >> 4;
     // on a diagonal block the lower-left quadrant lies below the diagonal
     const bool idle = (bi == bj) && (wr > wc);
-    v4i acc[4][4];
+    v4i acc[kT][kT];
 #pragma unroll
-    for (int x = 0; x < 4; ++x)
+    for (int x = 0; x < kT; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) acc[x][y] = (v4i){0, 0, 0, 0};
+        for (int y = 0; y < kT; ++y) acc[x][y] = (v4i){0, 0, 0, 0};
 
     if (k_begin < k_end) fetch(k_begin);
     for (uint64_t k0 = k_begin; k0 < k_end; k0 += kKW) {
@@ -102,19 +109,41 @@ __global__ __launch_bounds__(256) void k_gram_i8(const GramJob *__restrict__ job
         __syncthreads();
         if (k0 + kKW < k_end) fetch(k0 + kKW);
         if (!idle) {
-#pragma unroll 2
-            for (int w = 0; w < kKW; ++w) {
-                v4i fa[4], fb[4];
+            // Software pipeline over the tile's words: the next word's LDS
+            // reads are issued before this word's MFMAs, and its B spreads are
+            // computed between this word's MFMA rows (one per row), so the
+            // VALU work hides under the matrix cores.  Column kKW of the LDS
+            // tiles is padding: the reads ahead of the last word stay in
+            // bounds and go unused.
+            uint64_t ca[kT], nb[kT];
+            v4i fb[kT];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    fa[t] = spread_bits(sA[wr * 64 + t * 16 + li][w], lg);
-                    fb[t] = spread_bits(sB[wc * 64 + t * 16 + li][w], lg);
+            for (int t = 0; t < kT; ++t) {
+                ca[t] = sA[wr * 16 * kT + t * 16 + li][0];
+                fb[t] = spread_bits(sB[wc * 16 * kT + t * 16 + li][0], lg);
+            }
+#pragma unroll 1
+            for (int w = 0; w < kKW; ++w) {
+                uint64_t na[kT];
+#pragma unroll
+                for (int t = 0; t < kT; ++t) {
+                    na[t] = sA[wr * 16 * kT + t * 16 + li][w + 1];
+                    nb[t] = sB[wc * 16 * kT + t * 16 + li][w + 1];
+                }
+                v4i gb[kT];
+#pragma unroll
+                for (int x = 0; x < kT; ++x) {
+                    const v4i fa = spread_bits(ca[x], lg);
+#pragma unroll
+                    for (int y = 0; y < kT; ++y)
+                        acc[x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa, fb[y], acc[x][y], 0, 0, 0);
+                    gb[x] = spread_bits(nb[x], lg);
                 }
 #pragma unroll
-                for (int x = 0; x < 4; ++x)
-#pragma unroll
-                    for (int y = 0; y < 4; ++y)
-                        acc[x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[x], fb[y], acc[x][y], 0, 0, 0);
+                for (int t = 0; t < kT; ++t) {
+                    ca[t] = na[t];
+                    fb[t] = gb[t];
+                }
             }
         }
         __syncthreads();
@@ -122,13 +151,13 @@ __global__ __launch_bounds__(256) void k_gram_i8(const GramJob *__restrict__ job
     if (idle) return;
     // D tile layout: lane holds rows 4*(lane/16) + r, column lane % 16
 #pragma unroll
-    for (int x = 0; x < 4; ++x)
+    for (int x = 0; x < kT; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) {
-            const uint32_t col = bj * kBT + wc * 64 + y * 16 + li;
+        for (int y = 0; y < kT; ++y) {
+            const uint32_t col = bj * kBT + wc * 16 * kT + y * 16 + li;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const uint32_t row = bi * kBT + wr * 64 + x * 16 + lg * 4 + r;
+                const uint32_t row = bi * kBT + wr * 16 * kT + x * 16 + lg * 4 + r;
                 const int v = acc[x][y][r];
                 if (row < n && col < n && row <= col && v) atomicAdd(res + (uint64_t)row * ld + col, v);
             }
@@ -147,8 +176,10 @@ int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<Gr
         out += (uint64_t)d.n_items * d.n_items;
     }
     *out_elems = out;
-    // enough workgroups to fill the device: split K when there are few blocks
-    const uint64_t want = (uint64_t)ctx->cu_count * 2;
+    // One workgroup per CU at a time (the accumulators take the register
+    // file): split K so the jobs fill one round of CUs without a partial
+    // second round (jobs <= CUs, rounded down).
+    const uint64_t want = (uint64_t)ctx->cu_count;
     for (uint32_t i = 0; i < dbs.size(); ++i) {
         const GramDb &d = dbs[i];
         if (d.n_items == 0 || d.words == 0) continue;
@@ -156,7 +187,7 @@ int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<Gr
             return fail(CPM_ERR_INVALID, "int32 pair counts need fewer than 2^31 transactions");
         const uint32_t nb = (d.n_items + kBT - 1) / kBT;
         const uint64_t ktiles = (d.words + kKW - 1) / kKW;
-        uint64_t slices = tri_total ? std::max<uint64_t>(1, (want + tri_total - 1) / tri_total) : 1;
+        uint64_t slices = tri_total ? std::max<uint64_t>(1, want / tri_total) : 1;
         slices = std::min<uint64_t>(slices, ktiles);
         const uint64_t slice_words = (ktiles + slices - 1) / slices * kKW;
         for (uint32_t bi = 0; bi < nb; ++bi)
