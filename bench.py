@@ -72,7 +72,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="eclat_1m_1k", choices=["eclat_1m_1k", "pfeclat_2m_500"])
+    ap.add_argument("--workload", default="eclat_1m_1k",
+                    choices=["eclat_1m_1k", "pfeclat_2m_500", "batch_256"])
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch level steps directly")
     ap.add_argument("--group", type=int, default=32, help="entries per support work item")
@@ -82,6 +83,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(each GPU would mine its own prefix classes; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch_256":
+        print(json.dumps(run_batch(args)), flush=True)
+        return
 
     import numpy as np
     import torch
@@ -198,6 +203,158 @@ def main():
     print(json.dumps(line), flush=True)
     if parity is False:
         sys.exit(1)
+
+
+def per_class_plain(ctx, labels, bits, n, minsup):
+    """The reference's schedule for one database on this GPU: level 1 and the
+    pair supports counted in batches, then one launch per prefix class
+    (cuEclat.py:168-245), driven from the host as cuEclat's host path does."""
+    from cupami_amd.frequentPattern.cuEclat import cuEclat
+    m = cuEclat.__new__(cuEclat)
+    m._minSup = minsup
+    m._transactionCount = n
+    m._finalPatterns = {}
+    m._mineHost(ctx, list(labels), bits)
+    return m._finalPatterns
+
+
+def run_batch(args):
+    """256 databases mined together (one store build call, one search)
+    against the same engine one database at a time, and against the
+    reference's per-class launch schedule on a subset."""
+    import numpy as np
+    import torch
+    from cupami_amd import _hip, datagen
+    from cupami_amd.batch import BatchMiner
+
+    t = time.perf_counter()
+    dbs = datagen.batch_256()
+    cols_all, labels_all = [], []
+    for db, _ in dbs:
+        labels, cols = datagen.label_columns(db.items)
+        labels_all.append(labels)
+        cols_all.append(cols)
+    minsups = [ms for _, ms in dbs]
+    nnz = [db.rows.size for db, _ in dbs]
+    log(f"bench: generated {len(dbs)} databases ({sum(nnz)} pairs) in {time.perf_counter() - t:.1f}s")
+
+    dev = torch.device("cuda:0")
+    d_rows = torch.from_numpy(np.concatenate([db.rows for db, _ in dbs]).view(np.int64)).to(dev)
+    d_cols = torch.from_numpy(np.concatenate(cols_all).view(np.int32)).to(dev)
+    torch.cuda.synchronize()
+    base_r, base_c = d_rows.data_ptr(), d_cols.data_ptr()
+    specs, off = [], 0
+    for (db, _), labels, k in zip(dbs, labels_all, nnz):
+        specs.append((base_r + 8 * off, base_c + 4 * off, k, db.n_transactions, len(labels)))
+        off += k
+
+    ctx = _hip.context(0)
+    miner = BatchMiner(ctx)
+    config, table = miner.tune(specs, minsups)
+    log(f"bench: autotune {({f'{g}x{w}': round(v * 1e3, 2) for (g, w), v in table.items()})} -> {config}")
+    for _ in range(args.warmup):
+        pats = miner.mine(specs, minsups)
+    ctx.reset_stats()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pats = miner.mine(specs, minsups)
+    ctx.synchronize()
+    batched_s = (time.perf_counter() - t0) / args.steps
+    stats = ctx.stats()
+    n_pat = len(pats)
+    log(f"bench: batched {batched_s * 1e3:.2f} ms/step, {n_pat} patterns")
+
+    # plain: the same engine, one database at a time
+    def sequential(idx):
+        out = []
+        for i in idx:
+            r, c, k, n, items = specs[i]
+            bits = ctx.build_bits_device(r, c, k, n, items)
+            try:
+                out.append(ctx.eclat_mine(bits, minsups[i]))
+            finally:
+                bits.close()
+        return out
+    sequential(range(len(dbs)))
+    ctx.synchronize()
+    t = time.perf_counter()
+    seq = sequential(range(len(dbs)))
+    ctx.synchronize()
+    seq_s = time.perf_counter() - t
+    log(f"bench: one database at a time {seq_s * 1e3:.1f} ms")
+
+    # the reference's per-class schedule, on every 16th database
+    subset = list(range(0, len(dbs), 16))
+    sub_specs = [specs[i] for i in subset]
+    sub_ms = [minsups[i] for i in subset]
+    miner.mine(sub_specs, sub_ms)
+    ctx.synchronize()
+    t = time.perf_counter()
+    sub_batched = miner.mine(sub_specs, sub_ms)
+    ctx.synchronize()
+    sub_batched_s = time.perf_counter() - t
+    t = time.perf_counter()
+    per_class = []
+    for i in subset:
+        r, c, k, n, items = specs[i]
+        bits = ctx.build_bits_device(r, c, k, n, items)
+        try:
+            per_class.append(per_class_plain(ctx, range(items), bits, n, minsups[i]))
+        finally:
+            bits.close()
+    ctx.synchronize()
+    per_class_s = time.perf_counter() - t
+    log(f"bench: subset of {len(subset)}: batched {sub_batched_s * 1e3:.1f} ms, per-class {per_class_s * 1e3:.1f} ms")
+
+    # parity: batched == one at a time (all), == per-class (subset), == oracle
+    parts = pats.split(len(dbs))
+    parity = all(p == s.to_dict() for p, s in zip(parts, seq))
+    sub_parts = sub_batched.split(len(subset))
+    parity = parity and all(sub_parts[j] == {tuple(k): v for k, v in per_class[j].items()}
+                            for j in range(len(subset)))
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        threads = oracle.threads()
+        hosts = [oracle.bits_build(db.rows, c, db.n_transactions, len(l))
+                 for (db, _), c, l in zip(dbs, cols_all, labels_all)]
+        t = time.perf_counter()
+        expect = [oracle.eclat(h, ms) for h, ms in zip(hosts, minsups)]
+        cpu_s = time.perf_counter() - t
+        parity = parity and all(p == e for p, e in zip(parts, expect))
+        n_exp = sum(len(e) for e in expect)
+        cpu = {"value": round(n_exp / cpu_s, 1), "unit": "itemsets/s", "cores": threads, "kind": "port",
+               "sample": f"all {len(dbs)} databases, one run ({n_exp} itemsets in {cpu_s:.2f}s): "
+                         "oracle/cpm_oracle.c vertical Eclat, OpenMP over prefix classes"}
+    log(f"bench: parity={parity}")
+
+    launches = max(stats["launches"], 1)
+    avg_ms = stats["kernel_ms"] / launches
+    bytes_per_launch = stats["bytes"] / launches
+    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9 if avg_ms > 0 else 0.0
+    line = {
+        "metric": "itemsets/s (batched Eclat, 256 databases) vs plain", "value": round(n_pat / batched_s, 1),
+        "unit": "itemsets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(batched_s * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic (numpy PCG64 seeds 303, 303000+i)",
+        "config": {"workload": "batch_256", "databases": len(dbs), "transactions": sum(db.n_transactions for db, _ in dbs),
+                   "pairs": sum(nnz), "patterns": n_pat, "schedule": {"group": config[0], "tile_words": config[1]},
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_support_count",
+                     "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": bytes_per_launch,
+                     "launches_per_step": launches / args.steps},
+        "cpu_baseline": cpu,
+        "plain": {"one_at_a_time_ms": round(seq_s * 1e3, 2),
+                  "speedup_batched": round(seq_s / batched_s, 2),
+                  "per_class_subset": {"databases": subset, "per_class_ms": round(per_class_s * 1e3, 1),
+                                       "batched_ms": round(sub_batched_s * 1e3, 2),
+                                       "speedup_batched": round(per_class_s / sub_batched_s, 1)}},
+        "autotune": {f"{g}x{w}": round(v * 1e3, 3) for (g, w), v in table.items()},
+        "parity": parity,
+    }
+    return line
 
 
 if __name__ == "__main__":

@@ -40,6 +40,9 @@ SIGNATURES = {
                                       ctypes.c_uint32, ctypes.POINTER(_vp)]),
     "cpm_bits_build_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint64,
                                              ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    "cpm_bits_build_device_batch": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(_vp),
+                                                   ctypes.POINTER(_vp), _u64p, _u64p, _u32p,
+                                                   ctypes.POINTER(_vp)]),
     "cpm_bits_destroy": (ctypes.c_int, [_vp]),
     "cpm_bits_shape": (ctypes.c_int, [_vp, _u64p, _u32p, _u64p, _u64p]),
     "cpm_bits_download": (ctypes.c_int, [_vp, _vp, _u64p]),
@@ -181,6 +184,23 @@ class Context:
                                            int(n_transactions), int(n_items),
                                            ctypes.byref(handle)))
         return Bits(self, handle)
+
+    def build_bits_device_batch(self, specs):
+        """Several stores in one call: ``specs`` is a sequence of
+        ``(d_rows, d_cols, nnz, n_transactions, n_items)`` with device
+        addresses as ints.  Returns one ``Bits`` per spec."""
+        n = len(specs)
+        if n == 0:
+            return []
+        rows = (_vp * n)(*[_vp(int(s[0])) for s in specs])
+        cols = (_vp * n)(*[_vp(int(s[1])) for s in specs])
+        nnz = np.ascontiguousarray([int(s[2]) for s in specs], dtype=np.uint64)
+        nt = np.ascontiguousarray([int(s[3]) for s in specs], dtype=np.uint64)
+        ni = np.ascontiguousarray([int(s[4]) for s in specs], dtype=np.uint32)
+        out = (_vp * n)()
+        check(load().cpm_bits_build_device_batch(self.handle, n, rows, cols, _p(nnz, _u64p), _p(nt, _u64p),
+                                                 _p(ni, _u32p), out))
+        return [Bits(self, _vp(out[i])) for i in range(n)]
 
     # ---- operators -----------------------------------------------------------
     def count_supports(self, bits, candidates):

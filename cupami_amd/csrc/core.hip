@@ -96,9 +96,11 @@ __global__ __launch_bounds__(256) void k_bits_scatter(const uint64_t *__restrict
 }
 
 // Is rows[] non-decreasing, and is its last row inside the store?  Raises
-// flags[0] when a descent is found.
+// flags[0] when a descent is found and flags[1] when the last row is
+// >= n_trans (for sorted rows the last is the largest).
 __global__ __launch_bounds__(256) void k_rows_sorted(const uint64_t *__restrict__ rows, uint64_t nnz,
-                                                    unsigned int *flags) {
+                                                    uint64_t n_trans, unsigned int *flags) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && nnz && rows[nnz - 1] >= n_trans) atomicOr(flags + 1, 1u);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i + 1 < nnz;
          i += (uint64_t)gridDim.x * blockDim.x)
         if (rows[i] > rows[i + 1]) {
@@ -206,6 +208,7 @@ int cpm_ctx_destroy(cpm_ctx *ctx) {
     if (ctx->search_ws_free) ctx->search_ws_free(ctx->search_ws);
     ctx->search_ws = nullptr;
     ctx->spare_store.release();
+    ctx->spare_arena.release();
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -248,88 +251,165 @@ static int bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits
     return CPM_OK;
 }
 
+}  // extern "C"
+
+namespace cpm {
+
+StoreArena::~StoreArena() {
+    // the stores of the batch are gone (their destroy synchronized the
+    // stream): keep the larger allocation for the next batch
+    if (ctx && buf.cap > ctx->spare_arena.cap) {
+        std::swap(buf.p, ctx->spare_arena.p);
+        std::swap(buf.cap, ctx->spare_arena.cap);
+    }
+}
+
+namespace {
+
+struct BuildIn {
+    const uint64_t *rows;
+    const uint32_t *cols;
+    uint64_t nnz, n_trans;
+    uint32_t n_items;
+};
+
+// words per workgroup of the LDS tile build (0: too many items for it)
+uint32_t tile_words_for(uint32_t n_items) {
+    return n_items ? std::min<uint32_t>(8, 8192 / std::max<uint32_t>(n_items, 1)) : 0;
+}
+
+// Build n stores from device COO.  Every check of the batch is read back in
+// two copies (sortedness / range, then the build's own column check), so a
+// batch costs two host synchronizations whatever its size.  batch = true puts
+// all stores in one arena.
+int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bits **out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        out[i] = nullptr;
+        if (in[i].nnz && (!in[i].rows || !in[i].cols)) return fail(CPM_ERR_INVALID, "rows/cols are NULL");
+        if (in[i].n_trans >= (1ull << 40)) return fail(CPM_ERR_INVALID, "too many transactions");
+    }
+    if (n == 0) return CPM_OK;
+    CPM_CHECK(ctx->scratch[5].ensure((size_t)3 * n * sizeof(unsigned int)));
+    CPM_CHECK(ctx->host_scratch.ensure((size_t)3 * n * sizeof(unsigned int)));
+    unsigned int *flags = ctx->scratch[5].as<unsigned int>();
+    unsigned int *hflags = ctx->host_scratch.as<unsigned int>();
+    hipStream_t st = ctx->stream;
+    // flags of store i: [3i] rows descend, [3i+1] last row out of range, [3i+2] bad pair
+    CPM_HIP(hipMemsetAsync(flags, 0, (size_t)3 * n * sizeof(unsigned int), st));
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!in[i].nnz || !tile_words_for(in[i].n_items)) continue;
+        const uint64_t g = std::min<uint64_t>((in[i].nnz + 255) / 256, (uint64_t)ctx->cu_count * 16);
+        hipLaunchKernelGGL(k_rows_sorted, dim3((unsigned)g), dim3(256), 0, st, in[i].rows, in[i].nnz,
+                           in[i].n_trans, flags + 3 * i);
+    }
+    CPM_HIP(hipGetLastError());
+    CPM_HIP(hipMemcpyAsync(hflags, flags, (size_t)3 * n * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    CPM_HIP(hipStreamSynchronize(st));
+    std::vector<char> tiled(n, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        tiled[i] = in[i].nnz && tile_words_for(in[i].n_items) && hflags[3 * i] == 0;
+        if (tiled[i] && hflags[3 * i + 1])
+            return fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
+                        (unsigned long long)in[i].n_trans, in[i].n_items);
+    }
+    // allocation
+    std::shared_ptr<StoreArena> arena;
+    std::vector<uint64_t> offset(n, 0);
+    if (batch) {
+        uint64_t total = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            offset[i] = total;
+            const uint64_t words = (in[i].n_trans + 63) / 64;
+            total += std::max<uint64_t>((uint64_t)in[i].n_items * row_stride(words) * 8, 256);
+        }
+        arena = std::make_shared<StoreArena>();
+        arena->ctx = ctx;
+        if (ctx->spare_arena.p && ctx->spare_arena.cap >= total) {
+            std::swap(arena->buf.p, ctx->spare_arena.p);
+            std::swap(arena->buf.cap, ctx->spare_arena.cap);
+        }
+        CPM_CHECK(arena->buf.ensure(total));
+    }
+    int status = CPM_OK;
+    for (uint32_t i = 0; i < n && status == CPM_OK; ++i) {
+        cpm_bits *b = nullptr;
+        if (batch) {
+            b = new cpm_bits();
+            b->ctx = ctx;
+            b->n_transactions = in[i].n_trans;
+            b->n_items = in[i].n_items;
+            b->words = (in[i].n_trans + 63) / 64;
+            b->stride = row_stride(b->words);
+            b->arena = arena;
+            b->view = reinterpret_cast<uint64_t *>(arena->buf.as<unsigned char>() + offset[i]);
+            const size_t bytes = (size_t)b->n_items * b->stride * 8;
+            if (!tiled[i] && bytes && hipMemsetAsync(b->view, 0, bytes, st) != hipSuccess)
+                status = fail(CPM_ERR_RUNTIME, "hipMemsetAsync failed");
+        } else {
+            status = bits_alloc(ctx, in[i].n_trans, in[i].n_items, &b, !tiled[i]);
+        }
+        if (!b) break;
+        out[i] = b;
+        if (status != CPM_OK) break;
+        if (tiled[i]) {
+            const uint32_t B = tile_words_for(in[i].n_items);
+            const uint64_t blocks = (b->stride + B - 1) / B;
+            hipLaunchKernelGGL(k_bits_tile, dim3((unsigned)blocks), dim3(256), (size_t)in[i].n_items * B * 8, st,
+                               in[i].rows, in[i].cols, in[i].nnz, in[i].n_trans, in[i].n_items, b->stride, B,
+                               b->rows(), flags + 3 * i + 2);
+        } else if (in[i].nnz) {
+            const uint64_t blocks = std::min<uint64_t>((in[i].nnz + 255) / 256, (uint64_t)ctx->cu_count * 32);
+            hipLaunchKernelGGL(k_bits_scatter, dim3((unsigned)blocks), dim3(256), 0, st, in[i].rows, in[i].cols,
+                               in[i].nnz, in[i].n_trans, in[i].n_items, b->stride,
+                               reinterpret_cast<unsigned long long *>(b->rows()), flags + 3 * i + 2);
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (status == CPM_OK && e != hipSuccess)
+        status = fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(e));
+    if (status == CPM_OK) {
+        e = hipMemcpyAsync(hflags, flags, (size_t)3 * n * sizeof(unsigned int), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) status = fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(e));
+    }
+    for (uint32_t i = 0; i < n && status == CPM_OK; ++i)
+        if (hflags[3 * i + 2])
+            status = fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
+                          (unsigned long long)in[i].n_trans, in[i].n_items);
+    if (status != CPM_OK) {
+        for (uint32_t i = 0; i < n; ++i) {
+            cpm_bits_destroy(out[i]);
+            out[i] = nullptr;
+        }
+    }
+    return status;
+}
+
+}  // namespace
+}  // namespace cpm
+
+extern "C" {
+
 int cpm_bits_build_device(cpm_ctx *ctx, const uint64_t *d_rows, const uint32_t *d_cols,
                           uint64_t nnz, uint64_t n_trans, uint32_t n_items, cpm_bits **out) {
     clear_error();
     CPM_CHECK(bind(ctx));
     if (!out) return fail(CPM_ERR_INVALID, "out is NULL");
     *out = nullptr;
-    if (nnz && (!d_rows || !d_cols)) return fail(CPM_ERR_INVALID, "rows/cols are NULL");
-    if (n_trans >= (1ull << 40)) return fail(CPM_ERR_INVALID, "too many transactions");
-    // row-sorted input with a small enough item count takes the LDS tile build
-    const uint32_t tileB = n_items ? std::min<uint32_t>(8, 8192 / std::max<uint32_t>(n_items, 1)) : 0;
-    bool tiled = false;
-    CPM_CHECK(ctx->scratch[5].ensure(2 * sizeof(unsigned int)));
-    unsigned int *flags = ctx->scratch[5].as<unsigned int>();
-    // pinned read-back slots: [0] flags, [8] last row, [16] bad flag
-    CPM_CHECK(ctx->host_scratch.ensure(64));
-    unsigned char *hs = ctx->host_scratch.as<unsigned char>();
-    if (nnz && tileB >= 1) {
-        unsigned int *hf = reinterpret_cast<unsigned int *>(hs);
-        uint64_t *hlast = reinterpret_cast<uint64_t *>(hs + 8);
-        CPM_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(unsigned int), ctx->stream));
-        const uint64_t g = std::min<uint64_t>((nnz + 255) / 256, (uint64_t)ctx->cu_count * 16);
-        hipLaunchKernelGGL(k_rows_sorted, dim3((unsigned)g), dim3(256), 0, ctx->stream, d_rows, nnz,
-                           flags);
-        CPM_HIP(hipGetLastError());
-        CPM_HIP(hipMemcpyAsync(hf, flags, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, ctx->stream));
-        CPM_HIP(hipMemcpyAsync(hlast, d_rows + nnz - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
-        CPM_HIP(hipStreamSynchronize(ctx->stream));
-        const uint64_t last = *hlast;
-        if (hf[0] == 0 && last >= n_trans)
-            return fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
-                        (unsigned long long)n_trans, n_items);
-        tiled = hf[0] == 0;
-    }
-    cpm_bits *b = nullptr;
-    CPM_CHECK(bits_alloc(ctx, n_trans, n_items, &b, !tiled));
-    if (tiled) {
-        unsigned int *bad = flags + 1;
-        const uint64_t blocks = (b->stride + tileB - 1) / tileB;
-        hipLaunchKernelGGL(k_bits_tile, dim3((unsigned)blocks), dim3(256),
-                           (size_t)n_items * tileB * 8, ctx->stream, d_rows, d_cols, nnz, n_trans,
-                           n_items, b->stride, tileB, b->store.as<uint64_t>(), bad);
-        hipError_t e = hipGetLastError();
-        unsigned int *hbad = reinterpret_cast<unsigned int *>(hs + 16);
-        *hbad = 0;
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(hbad, bad, sizeof(unsigned int), hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) {
-            cpm_bits_destroy(b);
-            return fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(e));
-        }
-        if (*hbad) {
-            cpm_bits_destroy(b);
-            return fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
-                        (unsigned long long)n_trans, n_items);
-        }
-    } else if (nnz) {
-        unsigned int *bad = flags;
-        hipError_t e = hipMemsetAsync(bad, 0, sizeof(unsigned int), ctx->stream);
-        if (e == hipSuccess) {
-            uint64_t blocks = std::min<uint64_t>((nnz + 255) / 256, (uint64_t)ctx->cu_count * 32);
-            hipLaunchKernelGGL(k_bits_scatter, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
-                               d_rows, d_cols, nnz, n_trans, n_items, b->stride,
-                               b->store.as<unsigned long long>(), bad);
-            e = hipGetLastError();
-        }
-        unsigned int hbad = 0;
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) {
-            cpm_bits_destroy(b);
-            return fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(e));
-        }
-        if (hbad) {
-            cpm_bits_destroy(b);
-            return fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
-                        (unsigned long long)n_trans, n_items);
-        }
-    }
-    *out = b;
-    return CPM_OK;
+    const BuildIn in{d_rows, d_cols, nnz, n_trans, n_items};
+    return build_stores(ctx, &in, 1, false, out);
+}
+
+int cpm_bits_build_device_batch(cpm_ctx *ctx, uint32_t n_db, const uint64_t *const *d_rows,
+                                const uint32_t *const *d_cols, const uint64_t *nnz, const uint64_t *n_trans,
+                                const uint32_t *n_items, cpm_bits **out) {
+    clear_error();
+    CPM_CHECK(bind(ctx));
+    if (!out) return fail(CPM_ERR_INVALID, "out is NULL");
+    if (n_db && (!d_rows || !d_cols || !nnz || !n_trans || !n_items)) return fail(CPM_ERR_INVALID, "null argument");
+    std::vector<BuildIn> in(n_db);
+    for (uint32_t i = 0; i < n_db; ++i) in[i] = {d_rows[i], d_cols[i], nnz[i], n_trans[i], n_items[i]};
+    return build_stores(ctx, in.data(), n_db, true, out);
 }
 
 int cpm_bits_build(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, uint64_t nnz,
@@ -352,9 +432,14 @@ int cpm_bits_build(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, uin
 
 int cpm_bits_destroy(cpm_bits *b) {
     if (!b) return CPM_OK;
+    if (b->arena && b->arena.use_count() > 1) {
+        delete b;  // a view into a batch arena that other stores still hold
+        return CPM_OK;
+    }
     if (b->ctx) {
         cpm_ctx *ctx = b->ctx;
         (void)hipSetDevice(ctx->device);
+        // queued work may still read this memory before it is reused
         (void)hipStreamSynchronize(ctx->stream);
         // keep the larger allocation for the next build on this context
         if (b->store.cap > ctx->spare_store.cap) {
@@ -381,7 +466,7 @@ int cpm_bits_download(cpm_ctx *ctx, const cpm_bits *b, uint64_t *host) {
     CPM_CHECK(bind(ctx));
     if (!b || !host) return fail(CPM_ERR_INVALID, "null argument");
     if (b->words && b->n_items)
-        CPM_HIP(hipMemcpy2DAsync(host, b->words * 8, b->store.p, b->stride * 8, b->words * 8,
+        CPM_HIP(hipMemcpy2DAsync(host, b->words * 8, b->rows(), b->stride * 8, b->words * 8,
                                  b->n_items, hipMemcpyDeviceToHost, ctx->stream));
     CPM_HIP(hipStreamSynchronize(ctx->stream));
     return CPM_OK;

@@ -5,6 +5,7 @@
 #include <stdint.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -78,10 +79,21 @@ struct cpm_ctx {
     void (*search_ws_free)(void *) = nullptr;
     // allocation of the last destroyed store, reused by the next store build
     cpm::DBuf spare_store;
+    cpm::DBuf spare_arena;   // batch-build arena kept for the next batch
     // search schedule (cpm_set_search_options)
     uint32_t opt_group = 32, opt_tile_words = 2048;
     bool opt_graph = true, opt_prune = true;
 };
+
+namespace cpm {
+// Device memory shared by the stores of one batch build; handed back to its
+// context for the next batch when the last store goes.
+struct StoreArena {
+    cpm_ctx *ctx = nullptr;
+    DBuf buf;
+    ~StoreArena();
+};
+}  // namespace cpm
 
 struct cpm_bits {
     cpm_ctx *ctx = nullptr;
@@ -89,7 +101,10 @@ struct cpm_bits {
     uint32_t n_items = 0;
     uint64_t words = 0;    // ceil(N/64)
     uint64_t stride = 0;   // padded words per row
-    cpm::DBuf store;       // n_items * stride uint64
+    cpm::DBuf store;       // n_items * stride uint64 (own rows) ...
+    std::shared_ptr<cpm::StoreArena> arena;  // ... or rows inside a batch arena
+    uint64_t *view = nullptr;
+    uint64_t *rows() const { return view ? view : store.as<uint64_t>(); }
 };
 
 struct cpm_patterns {

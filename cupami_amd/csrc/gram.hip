@@ -231,7 +231,7 @@ extern "C" int cpm_pair_gram_i8(cpm_ctx *ctx, const cpm_bits *bits, const uint32
     CPM_CHECK(dc.ensure((size_t)n * 4));
     CPM_HIP(hipMemcpyAsync(dc.p, cols, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
     std::vector<GramDb> dbs(1);
-    dbs[0].store = bits->store.as<const uint64_t>();
+    dbs[0].store = bits->rows();
     dbs[0].stride = bits->stride;
     dbs[0].words = bits->words;
     dbs[0].n_transactions = bits->n_transactions;

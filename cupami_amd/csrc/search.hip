@@ -1054,7 +1054,7 @@ struct Search {
         for (uint32_t i = 0; i < n_db; ++i) {
             const cpm_bits *b = in[i].bits;
             DbDev &D = dbs[i];
-            D.store = b->store.as<const uint64_t>();
+            D.store = b->rows();
             D.stride = b->stride;
             D.words = b->words;
             D.n = b->n_transactions;
@@ -1471,7 +1471,7 @@ static int run_candidates(cpm_ctx *ctx, const cpm_bits *b, const uint32_t *cand,
     const uint32_t tw = kTileWords;
     DbDev D;
     memset(&D, 0, sizeof(D));
-    D.store = b->store.as<const uint64_t>();
+    D.store = b->rows();
     D.stride = b->stride;
     D.words = b->words;
     D.n = b->n_transactions;

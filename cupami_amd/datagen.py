@@ -112,6 +112,26 @@ def temporal_2m_500(seed=404):
     return temporal(2_000_000, 500, 8.0, 1.05, seed)
 
 
+def batch_256(seed=303, n_db=256):
+    """Many small databases for batched mining: ``n_db`` databases with
+    T log-uniform in [10^3, 2*10^5] baskets, I uniform in [20, 500] items,
+    Poisson(lam) lengths with lam uniform in [4, 12], Zipf s=1.0, and a
+    relative minimum support log-uniform in [0.002, 0.05] (an absolute count
+    ceil(minSup * T), the reference's rule for a fraction).  The parameters
+    come from ``seed``; database i is drawn with seed ``seed * 1000 + i``.
+    Returns a list of ``(Database, minsup_count)``."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = np.exp(rng.uniform(np.log(1e3), np.log(2e5), n_db)).round().astype(np.int64)
+    items = rng.integers(20, 501, n_db)
+    lam = rng.uniform(4.0, 12.0, n_db)
+    frac = np.exp(rng.uniform(np.log(0.002), np.log(0.05), n_db))
+    out = []
+    for i in range(n_db):
+        db = transactional(int(t[i]), int(items[i]), float(lam[i]), 1.0, seed * 1000 + i)
+        out.append((db, int(np.ceil(frac[i] * t[i]))))
+    return out
+
+
 def label_columns(item_ids: np.ndarray, n_items_hint=None):
     """Map numeric item ids to column indexes in *string* label order.
 

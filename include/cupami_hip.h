@@ -61,6 +61,15 @@ int cpm_bits_build(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, uin
 int cpm_bits_build_device(cpm_ctx *ctx, const uint64_t *d_rows, const uint32_t *d_cols,
                           uint64_t nnz, uint64_t n_transactions, uint32_t n_items,
                           cpm_bits **out);
+/* Many stores at once from device COO (one (rows, cols, nnz, n_transactions,
+ * n_items) per database): _bitsets.build_bitsets (_bitsets.py:91-100) for a
+ * batch of databases mined together.  The stores share one device
+ * allocation and the batch is checked with two host synchronizations in
+ * all.  out[i] receives store i; each is destroyed with cpm_bits_destroy.
+ * On error no store is returned. */
+int cpm_bits_build_device_batch(cpm_ctx *ctx, uint32_t n_db, const uint64_t *const *d_rows,
+                                const uint32_t *const *d_cols, const uint64_t *nnz,
+                                const uint64_t *n_transactions, const uint32_t *n_items, cpm_bits **out);
 int cpm_bits_destroy(cpm_bits *bits);
 /* words per row (ceil(N/64)), and the padded row stride in words */
 int cpm_bits_shape(const cpm_bits *bits, uint64_t *n_transactions, uint32_t *n_items,

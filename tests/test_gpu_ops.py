@@ -49,6 +49,58 @@ def test_bits_build_many_items_and_reuse(ctx):
             bits.close()
 
 
+def _device_coo(rows, cols):
+    import torch
+    d_rows = torch.from_numpy(np.ascontiguousarray(rows, np.uint64).view(np.int64)).to("cuda:0")
+    d_cols = torch.from_numpy(np.ascontiguousarray(cols, np.uint32).view(np.int32)).to("cuda:0")
+    torch.cuda.synchronize()
+    return d_rows, d_cols
+
+
+def test_bits_build_device_batch(ctx):
+    """One batch call gives each database the store a single build gives."""
+    cases = []
+    for k, (n, items, sorted_rows) in enumerate([(1, 3, True), (65, 7, False), (4096, 40, True), (0, 5, True),
+                                                 (3000, 9000, True), (100_003, 200, False), (777, 1, True)]):
+        rows, cols = random_store(k, n, items, 0.05 if items < 1000 else 0.002) if n else (
+            np.zeros(0, np.uint64), np.zeros(0, np.uint32))
+        if sorted_rows and rows.size:
+            order = np.argsort(rows, kind="stable")
+            rows, cols = rows[order], cols[order]
+        cases.append((rows, cols, n, items))
+    dev = [_device_coo(r, c) for r, c, _, _ in cases]
+    specs = [(dr.data_ptr(), dc.data_ptr(), r.size, n, items) for (dr, dc), (r, _, n, items) in zip(dev, cases)]
+    for _ in range(2):  # the second batch reuses the first one's arena
+        stores = ctx.build_bits_device_batch(specs)
+        for bits, (rows, cols, n, items) in zip(stores, cases):
+            assert np.array_equal(bits.download(), oracle.bits_build(rows, cols, n, items))
+        for bits in stores[::2] + stores[1::2]:
+            bits.close()
+
+
+def test_bits_build_device_batch_rejects_and_mines(ctx):
+    rng = random.Random(5)
+    dbs = []
+    for k in range(5):
+        labels, rr, cc, n = coo(random_rows(rng, 300 + 50 * k, 12, 6))
+        dbs.append((labels, rr, cc, n))
+    dev = [_device_coo(rr, cc) for _, rr, cc, _ in dbs]
+    specs = [(dr.data_ptr(), dc.data_ptr(), rr.size, n, len(labels))
+             for (dr, dc), (labels, rr, _, n) in zip(dev, dbs)]
+    # one bad column anywhere rejects the whole batch
+    bad = list(specs)
+    bad[3] = (bad[3][0], bad[3][1], bad[3][2], bad[3][3], 1)
+    with pytest.raises(ValueError):
+        ctx.build_bits_device_batch(bad)
+    stores = ctx.build_bits_device_batch(specs)
+    minsups = [3, 5, 8, 12, 20]
+    parts = ctx.eclat_mine_batch(stores, minsups).split(len(stores))
+    for part, bits, (labels, rr, cc, n), ms in zip(parts, stores, dbs, minsups):
+        expect = oracle.eclat(oracle.bits_build(rr, cc, n, len(labels)), ms)
+        assert part == expect
+        assert ctx.eclat_mine(bits, ms).to_dict() == expect
+
+
 def test_bits_build_rejects_out_of_range(ctx):
     for sorted_rows in ([0, 5], [5, 0]):
         with pytest.raises(ValueError):
