@@ -308,11 +308,22 @@ def run_batch(args):
     log(f"bench: subset of {len(subset)}: batched {sub_batched_s * 1e3:.1f} ms, per-class {per_class_s * 1e3:.1f} ms")
 
     # parity: batched == one at a time (all), == per-class (subset), == oracle
+    def same(tag, idx, got, want):
+        if got == want:
+            return True
+        missing = [k for k in want if k not in got]
+        extra = [k for k in got if k not in want]
+        wrong = [k for k in want if k in got and got[k] != want[k]]
+        log(f"bench: MISMATCH {tag} database {idx}: {len(got)} vs {len(want)} itemsets, "
+            f"missing {len(missing)} {missing[:3]}, extra {len(extra)} {extra[:3]}, "
+            f"support differs {len(wrong)} {[(k, got[k], want[k]) for k in wrong[:3]]}")
+        return False
+
     parts = pats.split(len(dbs))
-    parity = all(p == s.to_dict() for p, s in zip(parts, seq))
+    parity = all([same("batched/one-at-a-time", i, p, s.to_dict()) for i, (p, s) in enumerate(zip(parts, seq))])
     sub_parts = sub_batched.split(len(subset))
-    parity = parity and all(sub_parts[j] == {tuple(k): v for k, v in per_class[j].items()}
-                            for j in range(len(subset)))
+    parity = all([same("batched-subset/per-class", subset[j], sub_parts[j],
+                       {tuple(k): v for k, v in per_class[j].items()}) for j in range(len(subset))]) and parity
     cpu = None
     if not args.no_cpu_baseline:
         import oracle
@@ -322,7 +333,7 @@ def run_batch(args):
         t = time.perf_counter()
         expect = [oracle.eclat(h, ms) for h, ms in zip(hosts, minsups)]
         cpu_s = time.perf_counter() - t
-        parity = parity and all(p == e for p, e in zip(parts, expect))
+        parity = all([same("batched/oracle", i, p, e) for i, (p, e) in enumerate(zip(parts, expect))]) and parity
         n_exp = sum(len(e) for e in expect)
         cpu = {"value": round(n_exp / cpu_s, 1), "unit": "itemsets/s", "cores": threads, "kind": "port",
                "sample": f"all {len(dbs)} databases, one run ({n_exp} itemsets in {cpu_s:.2f}s): "

@@ -95,6 +95,17 @@ __global__ __launch_bounds__(256) void k_bits_scatter(const uint64_t *__restrict
     }
 }
 
+// dst row i = src row idx[i] (words of each; rows `stride` words apart in src)
+__global__ __launch_bounds__(256) void k_gather_rows(const uint64_t *__restrict__ src, uint64_t stride,
+                                                    uint64_t words, const uint32_t *__restrict__ idx, uint32_t n,
+                                                    uint64_t *__restrict__ dst) {
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const uint64_t *s = src + (uint64_t)idx[i] * stride;
+        uint64_t *d = dst + (uint64_t)i * words;
+        for (uint64_t w = threadIdx.x; w < words; w += blockDim.x) d[w] = s[w];
+    }
+}
+
 // Is rows[] non-decreasing, and is its last row inside the store?  Raises
 // flags[0] when a descent is found and flags[1] when the last row is
 // >= n_trans (for sorted rows the last is the largest).
@@ -205,6 +216,7 @@ int cpm_ctx_destroy(cpm_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &b : ctx->scratch) b.release();
     ctx->host_scratch.release();
+    ctx->bounce.release();
     if (ctx->search_ws_free) ctx->search_ws_free(ctx->search_ws);
     ctx->search_ws = nullptr;
     ctx->spare_store.release();
@@ -254,6 +266,64 @@ static int bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits
 }  // extern "C"
 
 namespace cpm {
+
+constexpr size_t kBounceBytes = 8u << 20;
+
+int copy_to_host(cpm_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return CPM_OK;
+    CPM_CHECK(ctx->bounce.ensure(std::min(bytes, kBounceBytes)));
+    unsigned char *h = ctx->bounce.as<unsigned char>();
+    for (size_t off = 0; off < bytes; off += kBounceBytes) {
+        const size_t n = std::min(kBounceBytes, bytes - off);
+        CPM_HIP(hipMemcpyAsync(h, static_cast<const unsigned char *>(src) + off, n, hipMemcpyDeviceToHost,
+                               ctx->stream));
+        CPM_HIP(hipStreamSynchronize(ctx->stream));
+        memcpy(static_cast<unsigned char *>(dst) + off, h, n);
+    }
+    return CPM_OK;
+}
+
+int copy_to_host_2d(cpm_ctx *ctx, void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
+                    size_t height) {
+    if (width == 0 || height == 0) return CPM_OK;
+    if (width > kBounceBytes) {
+        for (size_t r = 0; r < height; ++r)
+            CPM_CHECK(copy_to_host(ctx, static_cast<unsigned char *>(dst) + r * dpitch,
+                                   static_cast<const unsigned char *>(src) + r * spitch, width));
+        return CPM_OK;
+    }
+    const size_t per = kBounceBytes / width;
+    CPM_CHECK(ctx->bounce.ensure(std::min(height, per) * width));
+    unsigned char *h = ctx->bounce.as<unsigned char>();
+    for (size_t r0 = 0; r0 < height; r0 += per) {
+        const size_t rows = std::min(per, height - r0);
+        CPM_HIP(hipMemcpy2DAsync(h, width, static_cast<const unsigned char *>(src) + r0 * spitch, spitch, width,
+                                 rows, hipMemcpyDeviceToHost, ctx->stream));
+        CPM_HIP(hipStreamSynchronize(ctx->stream));
+        for (size_t r = 0; r < rows; ++r)
+            memcpy(static_cast<unsigned char *>(dst) + (r0 + r) * dpitch, h + r * width, width);
+    }
+    return CPM_OK;
+}
+
+int upload_rows(cpm_ctx *ctx, uint64_t *dst, uint64_t stride, const uint64_t *src, uint64_t words,
+                uint64_t rows) {
+    if (rows == 0 || stride == 0) return CPM_OK;
+    const size_t row_bytes = (size_t)stride * 8;
+    const uint64_t per = std::max<uint64_t>(1, kBounceBytes / row_bytes);
+    CPM_CHECK(ctx->bounce.ensure(std::min<uint64_t>(rows, per) * row_bytes));
+    uint64_t *h = ctx->bounce.as<uint64_t>();
+    for (uint64_t r0 = 0; r0 < rows; r0 += per) {
+        const uint64_t n = std::min(per, rows - r0);
+        for (uint64_t r = 0; r < n; ++r) {
+            memcpy(h + r * stride, src + (r0 + r) * words, (size_t)words * 8);
+            memset(h + r * stride + words, 0, (size_t)(stride - words) * 8);
+        }
+        CPM_HIP(hipMemcpyAsync(dst + r0 * stride, h, (size_t)n * row_bytes, hipMemcpyHostToDevice, ctx->stream));
+        CPM_HIP(hipStreamSynchronize(ctx->stream));  // the bounce buffer is refilled next
+    }
+    return CPM_OK;
+}
 
 StoreArena::~StoreArena() {
     // the stores of the batch are gone (their destroy synchronized the
@@ -466,8 +536,7 @@ int cpm_bits_download(cpm_ctx *ctx, const cpm_bits *b, uint64_t *host) {
     CPM_CHECK(bind(ctx));
     if (!b || !host) return fail(CPM_ERR_INVALID, "null argument");
     if (b->words && b->n_items)
-        CPM_HIP(hipMemcpy2DAsync(host, b->words * 8, b->rows(), b->stride * 8, b->words * 8,
-                                 b->n_items, hipMemcpyDeviceToHost, ctx->stream));
+        CPM_CHECK(copy_to_host_2d(ctx, host, b->words * 8, b->rows(), b->stride * 8, b->words * 8, b->n_items));
     CPM_HIP(hipStreamSynchronize(ctx->stream));
     return CPM_OK;
 }
@@ -490,30 +559,34 @@ int cpm_intersect_extensions(cpm_ctx *ctx, const uint64_t *prefix, const uint64_
         CPM_CHECK(dext.ensure((size_t)k * stride * 8));
         CPM_CHECK(dsup.ensure((size_t)k * 8));
         if (kept_rows) CPM_CHECK(dout.ensure((size_t)k * stride * 8));
-        CPM_HIP(hipMemsetAsync(dpre.p, 0, stride * 8, ctx->stream));
-        CPM_HIP(hipMemcpyAsync(dpre.p, prefix, words * 8, hipMemcpyHostToDevice, ctx->stream));
-        CPM_HIP(hipMemsetAsync(dext.p, 0, (size_t)k * stride * 8, ctx->stream));
-        CPM_HIP(hipMemcpy2DAsync(dext.p, stride * 8, ext, words * 8, words * 8, k,
-                                 hipMemcpyHostToDevice, ctx->stream));
+        CPM_CHECK(upload_rows(ctx, dpre.as<uint64_t>(), stride, prefix, words, 1));
+        CPM_CHECK(upload_rows(ctx, dext.as<uint64_t>(), stride, ext, words, k));
         CPM_CHECK(launch_intersect(ctx, dpre.as<uint64_t>(), dext.as<uint64_t>(), k, stride,
                                    dsup.as<unsigned long long>(),
                                    kept_rows ? dout.as<uint64_t>() : nullptr));
-        CPM_HIP(hipMemcpyAsync(supports, dsup.p, (size_t)k * 8, hipMemcpyDeviceToHost,
-                               ctx->stream));
-        CPM_HIP(hipStreamSynchronize(ctx->stream));
+        CPM_CHECK(copy_to_host(ctx, supports, dsup.p, (size_t)k * 8));
     } else {
         for (uint32_t j = 0; j < k; ++j) supports[j] = 0;
     }
     // keep step (cuEclat.py:234-242): children at or above keep_min, in order
-    uint32_t nk = 0;
-    for (uint32_t j = 0; j < k; ++j) {
-        if (supports[j] < keep_min) continue;
-        if (kept_index) kept_index[nk] = j;
-        if (kept_rows && words)
-            CPM_HIP(hipMemcpyAsync(kept_rows + (uint64_t)nk * words,
-                                   ctx->scratch[2].as<uint64_t>() + (uint64_t)j * stride, words * 8,
-                                   hipMemcpyDeviceToHost, ctx->stream));
-        ++nk;
+    std::vector<uint32_t> keep;
+    for (uint32_t j = 0; j < k; ++j)
+        if (supports[j] >= keep_min) keep.push_back(j);
+    const uint32_t nk = (uint32_t)keep.size();
+    if (kept_index && nk) memcpy(kept_index, keep.data(), (size_t)nk * 4);
+    if (kept_rows && words && nk) {
+        // gather the kept rows on the device, then one copy out
+        DBuf &didx = ctx->scratch[4], &dgat = ctx->scratch[1];  // the extension rows are consumed
+        CPM_CHECK(didx.ensure((size_t)nk * 4));
+        CPM_CHECK(dgat.ensure((size_t)nk * words * 8));
+        CPM_CHECK(ctx->host_scratch.ensure((size_t)nk * 4));
+        memcpy(ctx->host_scratch.p, keep.data(), (size_t)nk * 4);
+        CPM_HIP(hipMemcpyAsync(didx.p, ctx->host_scratch.p, (size_t)nk * 4, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_gather_rows, dim3(std::min<uint32_t>(nk, 65535)), dim3(256), 0, ctx->stream,
+                           ctx->scratch[2].as<uint64_t>(), stride, words, didx.as<uint32_t>(), nk,
+                           dgat.as<uint64_t>());
+        CPM_HIP(hipGetLastError());
+        CPM_CHECK(copy_to_host(ctx, kept_rows, dgat.p, (size_t)nk * words * 8));
     }
     CPM_HIP(hipStreamSynchronize(ctx->stream));
     if (n_kept) *n_kept = nk;

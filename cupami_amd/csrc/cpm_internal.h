@@ -80,12 +80,27 @@ struct cpm_ctx {
     // allocation of the last destroyed store, reused by the next store build
     cpm::DBuf spare_store;
     cpm::DBuf spare_arena;   // batch-build arena kept for the next batch
+    cpm::HBuf bounce;        // pinned bounce buffer of copy_to_host
     // search schedule (cpm_set_search_options)
     uint32_t opt_group = 32, opt_tile_words = 2048;
     bool opt_graph = true, opt_prune = true;
 };
 
 namespace cpm {
+// Device -> host copies into any host memory (pageable included): staged
+// through the context's pinned bounce buffer in chunks, each chunk copied
+// out after a stream synchronization, so the data is in place on return
+// whatever the runtime does with pageable destinations.
+int copy_to_host(cpm_ctx *ctx, void *dst, const void *src, size_t bytes);
+int copy_to_host_2d(cpm_ctx *ctx, void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
+                    size_t height);
+// Host -> device upload of `rows` rows of `words` uint64 from any host
+// memory into rows `stride` words apart, the padding zeroed: rows are padded
+// on the host in the pinned bounce buffer and sent in chunks (no device
+// memset, no pageable source handed to the runtime).
+int upload_rows(cpm_ctx *ctx, uint64_t *dst, uint64_t stride, const uint64_t *src, uint64_t words,
+                uint64_t rows);
+
 // Device memory shared by the stores of one batch build; handed back to its
 // context for the next batch when the last store goes.
 struct StoreArena {

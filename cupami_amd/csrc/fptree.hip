@@ -328,8 +328,8 @@ int fptree_build(cpm_ctx *ctx, FpWs &ws, const uint64_t *d_rows, const uint32_t 
     // header: frequent items in ascending (support, then descending id)
     std::vector<uint32_t> order(item_count);
     std::vector<uint64_t> hsup(item_count);
-    CPM_HIP(hipMemcpyAsync(order.data(), vals, (size_t)item_count * 4, hipMemcpyDeviceToHost, st));
-    CPM_HIP(hipMemcpyAsync(hsup.data(), sup, (size_t)item_count * 8, hipMemcpyDeviceToHost, st));
+    CPM_CHECK(copy_to_host(ctx, order.data(), vals, (size_t)item_count * 4));
+    CPM_CHECK(copy_to_host(ctx, hsup.data(), sup, (size_t)item_count * 8));
 
     // baskets sorted by (row, rank) with their lengths
     CPM_CHECK(ws.lengths.ensure((n_rows + 1) * 4));
@@ -452,9 +452,7 @@ static int build_links(cpm_ctx *ctx, FpWs &ws, cpm_fptree *t) {
     if (n)
         CPM_HIP(hipMemcpyAsync(t->link_nodes.p, ws.vals.p, n * 4, hipMemcpyDeviceToDevice, st));
     t->link_off.resize((size_t)t->item_count + 1);
-    CPM_HIP(hipMemcpyAsync(t->link_off.data(), ws.off.p, ((size_t)t->item_count + 1) * 4,
-                           hipMemcpyDeviceToHost, st));
-    CPM_HIP(hipStreamSynchronize(st));
+    CPM_CHECK(copy_to_host(ctx, t->link_off.data(), ws.off.p, ((size_t)t->item_count + 1) * 4));
     t->links = true;
     return CPM_OK;
 }
@@ -479,8 +477,7 @@ int fptree_conditional(cpm_ctx *ctx, FpWs &ws, cpm_fptree *t, uint32_t item, uin
                        t->counts.as<unsigned long long>(), csup);
     CPM_HIP(hipGetLastError());
     std::vector<uint64_t> hs(I);
-    CPM_HIP(hipMemcpyAsync(hs.data(), csup, (size_t)I * 8, hipMemcpyDeviceToHost, st));
-    CPM_HIP(hipStreamSynchronize(st));
+    CPM_CHECK(copy_to_host(ctx, hs.data(), csup, (size_t)I * 8));
     std::vector<uint32_t> header;
     for (uint32_t i = 0; i < I; ++i)
         if (hs[i] >= minsup) header.push_back(i);
@@ -592,9 +589,8 @@ int cpm_unique_pairs(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, u
     uint64_t n = 0;
     CPM_CHECK(unique_pairs(ctx, ws, dr.as<uint64_t>(), dc.as<uint32_t>(), nnz, orow.as<uint64_t>(),
                            ocol.as<uint32_t>(), &n, max_row));
-    CPM_HIP(hipMemcpyAsync(out_rows, orow.p, n * 8, hipMemcpyDeviceToHost, ctx->stream));
-    CPM_HIP(hipMemcpyAsync(out_cols, ocol.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
-    CPM_HIP(hipStreamSynchronize(ctx->stream));
+    CPM_CHECK(copy_to_host(ctx, out_rows, orow.p, n * 8));
+    CPM_CHECK(copy_to_host(ctx, out_cols, ocol.p, n * 4));
     *out_n = n;
     return CPM_OK;
 }
@@ -654,9 +650,9 @@ int cpm_fptree_download(const cpm_fptree *t, int64_t *parents, int64_t *items, u
     if (!t) return fail(CPM_ERR_INVALID, "null tree");
     CPM_CHECK(bind(t->ctx));
     hipStream_t st = t->ctx->stream;
-    if (parents) CPM_HIP(hipMemcpyAsync(parents, t->parents.p, t->n_nodes * 8, hipMemcpyDeviceToHost, st));
-    if (items) CPM_HIP(hipMemcpyAsync(items, t->items.p, t->n_nodes * 8, hipMemcpyDeviceToHost, st));
-    if (counts) CPM_HIP(hipMemcpyAsync(counts, t->counts.p, t->n_nodes * 8, hipMemcpyDeviceToHost, st));
+    if (parents) CPM_CHECK(copy_to_host(t->ctx, parents, t->parents.p, t->n_nodes * 8));
+    if (items) CPM_CHECK(copy_to_host(t->ctx, items, t->items.p, t->n_nodes * 8));
+    if (counts) CPM_CHECK(copy_to_host(t->ctx, counts, t->counts.p, t->n_nodes * 8));
     CPM_HIP(hipStreamSynchronize(st));
     if (header_items && !t->header_item.empty())
         memcpy(header_items, t->header_item.data(), t->header_item.size() * 4);

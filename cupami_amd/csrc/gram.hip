@@ -251,7 +251,6 @@ extern "C" int cpm_pair_gram_i8(cpm_ctx *ctx, const cpm_bits *bits, const uint32
     const uint64_t ops = (uint64_t)bits->n_transactions * n * (uint64_t)(n + 1);
     CPM_CHECK(launch_gram_i8(ctx, djobs.as<GramJob>(), jobs.size(), ddb.as<GramDb>(),
                              dout.as<int32_t>(), elems, ops));
-    CPM_HIP(hipMemcpyAsync(gram, dout.p, (size_t)n * n * 4, hipMemcpyDeviceToHost, ctx->stream));
-    CPM_HIP(hipStreamSynchronize(ctx->stream));
+    CPM_CHECK(copy_to_host(ctx, gram, dout.p, (size_t)n * n * 4));
     return CPM_OK;
 }

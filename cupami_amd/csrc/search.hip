@@ -1538,9 +1538,8 @@ static int run_candidates(cpm_ctx *ctx, const cpm_bits *b, const uint32_t *cand,
     if (periodic) CPM_CHECK(launch_support_periodic(ctx, a, support_grid(ctx)));
     else CPM_CHECK(launch_support_count(ctx, a, tw, support_grid(ctx)));
     std::vector<uint32_t> s32(n_cand);
-    CPM_HIP(hipMemcpyAsync(s32.data(), dsup.p, n_cand * 4, hipMemcpyDeviceToHost, st));
-    if (periodic) CPM_HIP(hipMemcpyAsync(per_out, dper.p, n_cand * 8, hipMemcpyDeviceToHost, st));
-    CPM_HIP(hipStreamSynchronize(st));
+    CPM_CHECK(copy_to_host(ctx, s32.data(), dsup.p, n_cand * 4));
+    if (periodic) CPM_CHECK(copy_to_host(ctx, per_out, dper.p, n_cand * 8));
     for (uint64_t c = 0; c < n_cand; ++c) sup_out[c] = s32[c];
     return CPM_OK;
 }
