@@ -106,36 +106,57 @@ __global__ __launch_bounds__(256) void k_gather_rows(const uint64_t *__restrict_
     }
 }
 
+// One database of a (batched) store build.
+struct BuildJob {
+    const uint64_t *rows;
+    const uint32_t *cols;
+    uint64_t nnz, n_trans, stride;
+    uint64_t *store;        // tile build only
+    unsigned int *flags;    // [0] rows descend, [1] last row out of range, [2] bad pair
+    uint32_t n_items, B;    // B: words per workgroup of the tile build
+    uint64_t block0;        // first workgroup of this database in the tile launch
+};
+
 // Is rows[] non-decreasing, and is its last row inside the store?  Raises
 // flags[0] when a descent is found and flags[1] when the last row is
-// >= n_trans (for sorted rows the last is the largest).
-__global__ __launch_bounds__(256) void k_rows_sorted(const uint64_t *__restrict__ rows, uint64_t nnz,
-                                                    uint64_t n_trans, unsigned int *flags) {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && nnz && rows[nnz - 1] >= n_trans) atomicOr(flags + 1, 1u);
+// >= n_trans (for sorted rows the last is the largest).  blockIdx.y picks
+// the database.
+__global__ __launch_bounds__(256) void k_rows_sorted(const BuildJob *__restrict__ jobs) {
+    const BuildJob &j = jobs[blockIdx.y];
+    const uint64_t *__restrict__ rows = j.rows;
+    const uint64_t nnz = j.nnz;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && nnz && rows[nnz - 1] >= j.n_trans) atomicOr(j.flags + 1, 1u);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i + 1 < nnz;
          i += (uint64_t)gridDim.x * blockDim.x)
         if (rows[i] > rows[i + 1]) {
-            atomicOr(flags, 1u);
+            atomicOr(j.flags, 1u);
             return;
         }
 }
 
-// Store build for row-sorted pairs: workgroup b owns words [b*B, b*B+B) of
-// every item row.  Its pairs are one contiguous run of the input; they are
-// ORed into an LDS tile (n_items x B words) and the tile is written out once,
-// so the store needs no clearing and HBM sees no atomics.
-__global__ __launch_bounds__(256) void k_bits_tile(const uint64_t *__restrict__ rows,
-                                                  const uint32_t *__restrict__ cols, uint64_t nnz,
-                                                  uint64_t n_trans, uint32_t n_items, uint64_t stride,
-                                                  uint32_t B, uint64_t *__restrict__ store,
-                                                  unsigned int *bad) {
+// Store build for row-sorted pairs, any number of databases per launch:
+// workgroup b of database d owns words [b*B, b*B+B) of every item row of d.
+// Its pairs are one contiguous run of the input; they are ORed into an LDS
+// tile (n_items x B words) and the tile is written out once, so the store
+// needs no clearing and HBM sees no atomics.
+__global__ __launch_bounds__(256) void k_bits_tile(const BuildJob *__restrict__ jobs, uint32_t n_jobs) {
     extern __shared__ unsigned long long tile[];
+    uint32_t lo_j = 0, hi_j = n_jobs;  // last job with block0 <= blockIdx.x
+    while (hi_j - lo_j > 1) {
+        const uint32_t m = (lo_j + hi_j) >> 1;
+        if (jobs[m].block0 <= blockIdx.x) lo_j = m; else hi_j = m;
+    }
+    const BuildJob &j = jobs[lo_j];
+    const uint64_t *__restrict__ rows = j.rows;
+    const uint32_t *__restrict__ cols = j.cols;
+    const uint32_t n_items = j.n_items, B = j.B;
+    const uint64_t stride = j.stride, nnz = j.nnz;
     const uint32_t tid = threadIdx.x;
-    const uint64_t w0 = (uint64_t)blockIdx.x * B;
+    const uint64_t w0 = (blockIdx.x - j.block0) * B;
     const uint32_t cells = n_items * B;
     for (uint32_t i = tid; i < cells; i += blockDim.x) tile[i] = 0;
     // [lo, hi): pairs whose row lies in this word range
-    const uint64_t r_lo = w0 * 64, r_hi = min((w0 + B) * 64, n_trans);
+    const uint64_t r_lo = w0 * 64, r_hi = min((w0 + B) * 64, j.n_trans);
     uint64_t lo = 0, hi = nnz;
     while (lo < hi) {
         const uint64_t m = (lo + hi) >> 1;
@@ -151,12 +172,13 @@ __global__ __launch_bounds__(256) void k_bits_tile(const uint64_t *__restrict__ 
         const uint64_t r = rows[i];
         const uint32_t c = cols[i];
         if (c >= n_items) {
-            atomicOr(bad, 1u);
+            atomicOr(j.flags + 2, 1u);
             continue;
         }
         atomicOr(&tile[c * B + (uint32_t)((r >> 6) - w0)], 1ull << (r & 63));
     }
     __syncthreads();
+    uint64_t *__restrict__ store = j.store;
     for (uint32_t i = tid; i < cells; i += blockDim.x) {
         const uint32_t c = i / B, k = i % B;
         if (w0 + k < stride) store[(uint64_t)c * stride + w0 + k] = tile[i];
@@ -359,20 +381,40 @@ int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bi
         if (in[i].n_trans >= (1ull << 40)) return fail(CPM_ERR_INVALID, "too many transactions");
     }
     if (n == 0) return CPM_OK;
-    CPM_CHECK(ctx->scratch[5].ensure((size_t)3 * n * sizeof(unsigned int)));
-    CPM_CHECK(ctx->host_scratch.ensure((size_t)3 * n * sizeof(unsigned int)));
-    unsigned int *flags = ctx->scratch[5].as<unsigned int>();
-    unsigned int *hflags = ctx->host_scratch.as<unsigned int>();
     hipStream_t st = ctx->stream;
-    // flags of store i: [3i] rows descend, [3i+1] last row out of range, [3i+2] bad pair
-    CPM_HIP(hipMemsetAsync(flags, 0, (size_t)3 * n * sizeof(unsigned int), st));
+    // pinned host area: flags (3 per store) then the job table; device: the same
+    const size_t flag_bytes = ((size_t)3 * n * sizeof(unsigned int) + 255) / 256 * 256;
+    const size_t job_bytes = (size_t)n * sizeof(BuildJob);
+    CPM_CHECK(ctx->scratch[5].ensure(flag_bytes + job_bytes));
+    CPM_CHECK(ctx->host_scratch.ensure(flag_bytes + job_bytes));
+    unsigned int *flags = ctx->scratch[5].as<unsigned int>();
+    BuildJob *d_jobs = reinterpret_cast<BuildJob *>(ctx->scratch[5].as<unsigned char>() + flag_bytes);
+    unsigned int *hflags = ctx->host_scratch.as<unsigned int>();
+    BuildJob *h_jobs = reinterpret_cast<BuildJob *>(ctx->host_scratch.as<unsigned char>() + flag_bytes);
+
+    // 1. sortedness and range of every store's rows (one launch)
+    uint32_t n_check = 0;
+    uint64_t max_nnz = 0;
+    std::vector<uint32_t> check_of(n, 0);
     for (uint32_t i = 0; i < n; ++i) {
         if (!in[i].nnz || !tile_words_for(in[i].n_items)) continue;
-        const uint64_t g = std::min<uint64_t>((in[i].nnz + 255) / 256, (uint64_t)ctx->cu_count * 16);
-        hipLaunchKernelGGL(k_rows_sorted, dim3((unsigned)g), dim3(256), 0, st, in[i].rows, in[i].nnz,
-                           in[i].n_trans, flags + 3 * i);
+        BuildJob &j = h_jobs[n_check];
+        memset(&j, 0, sizeof(j));
+        j.rows = in[i].rows;
+        j.nnz = in[i].nnz;
+        j.n_trans = in[i].n_trans;
+        j.flags = flags + 3 * i;
+        check_of[i] = n_check++;
+        max_nnz = std::max(max_nnz, in[i].nnz);
     }
-    CPM_HIP(hipGetLastError());
+    CPM_HIP(hipMemsetAsync(flags, 0, (size_t)3 * n * sizeof(unsigned int), st));
+    if (n_check) {
+        CPM_HIP(hipMemcpyAsync(d_jobs, h_jobs, (size_t)n_check * sizeof(BuildJob), hipMemcpyHostToDevice, st));
+        const uint64_t gx = std::max<uint64_t>(1, std::min<uint64_t>((max_nnz + 255) / 256,
+                                                                    (uint64_t)ctx->cu_count * 16 / std::max(1u, n_check) + 1));
+        hipLaunchKernelGGL(k_rows_sorted, dim3((unsigned)gx, n_check), dim3(256), 0, st, d_jobs);
+        CPM_HIP(hipGetLastError());
+    }
     CPM_HIP(hipMemcpyAsync(hflags, flags, (size_t)3 * n * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
     CPM_HIP(hipStreamSynchronize(st));
     std::vector<char> tiled(n, 0);
@@ -382,7 +424,7 @@ int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bi
             return fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
                         (unsigned long long)in[i].n_trans, in[i].n_items);
     }
-    // allocation
+    // 2. allocation
     std::shared_ptr<StoreArena> arena;
     std::vector<uint64_t> offset(n, 0);
     if (batch) {
@@ -420,18 +462,43 @@ int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bi
         }
         if (!b) break;
         out[i] = b;
-        if (status != CPM_OK) break;
-        if (tiled[i]) {
-            const uint32_t B = tile_words_for(in[i].n_items);
-            const uint64_t blocks = (b->stride + B - 1) / B;
-            hipLaunchKernelGGL(k_bits_tile, dim3((unsigned)blocks), dim3(256), (size_t)in[i].n_items * B * 8, st,
-                               in[i].rows, in[i].cols, in[i].nnz, in[i].n_trans, in[i].n_items, b->stride, B,
-                               b->rows(), flags + 3 * i + 2);
-        } else if (in[i].nnz) {
-            const uint64_t blocks = std::min<uint64_t>((in[i].nnz + 255) / 256, (uint64_t)ctx->cu_count * 32);
-            hipLaunchKernelGGL(k_bits_scatter, dim3((unsigned)blocks), dim3(256), 0, st, in[i].rows, in[i].cols,
-                               in[i].nnz, in[i].n_trans, in[i].n_items, b->stride,
-                               reinterpret_cast<unsigned long long *>(b->rows()), flags + 3 * i + 2);
+    }
+    // 3. the tile builds of every sorted store in one launch; scatter builds
+    // (unsorted rows, or too many items for the LDS tile) one per store
+    if (status == CPM_OK) {
+        uint32_t nt = 0;
+        uint64_t blocks = 0;
+        size_t lds = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!tiled[i]) continue;
+            BuildJob &j = h_jobs[nt++];
+            memset(&j, 0, sizeof(j));
+            j.rows = in[i].rows;
+            j.cols = in[i].cols;
+            j.nnz = in[i].nnz;
+            j.n_trans = in[i].n_trans;
+            j.stride = out[i]->stride;
+            j.store = out[i]->rows();
+            j.flags = flags + 3 * i;
+            j.n_items = in[i].n_items;
+            j.B = tile_words_for(in[i].n_items);
+            j.block0 = blocks;
+            blocks += (j.stride + j.B - 1) / j.B;
+            lds = std::max(lds, (size_t)j.n_items * j.B * 8);
+        }
+        if (nt && blocks >= (1ull << 31)) status = fail(CPM_ERR_INVALID, "store batch too large");
+        if (nt && status == CPM_OK) {
+            if (hipMemcpyAsync(d_jobs, h_jobs, (size_t)nt * sizeof(BuildJob), hipMemcpyHostToDevice, st) != hipSuccess)
+                status = fail(CPM_ERR_RUNTIME, "job table upload failed");
+            else
+                hipLaunchKernelGGL(k_bits_tile, dim3((unsigned)blocks), dim3(256), lds, st, d_jobs, nt);
+        }
+        for (uint32_t i = 0; i < n && status == CPM_OK; ++i) {
+            if (tiled[i] || !in[i].nnz) continue;
+            const uint64_t g = std::min<uint64_t>((in[i].nnz + 255) / 256, (uint64_t)ctx->cu_count * 32);
+            hipLaunchKernelGGL(k_bits_scatter, dim3((unsigned)g), dim3(256), 0, st, in[i].rows, in[i].cols,
+                               in[i].nnz, in[i].n_trans, in[i].n_items, out[i]->stride,
+                               reinterpret_cast<unsigned long long *>(out[i]->rows()), flags + 3 * i + 2);
         }
     }
     hipError_t e = hipGetLastError();
@@ -447,6 +514,7 @@ int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bi
             status = fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
                           (unsigned long long)in[i].n_trans, in[i].n_items);
     if (status != CPM_OK) {
+        (void)hipStreamSynchronize(st);
         for (uint32_t i = 0; i < n; ++i) {
             cpm_bits_destroy(out[i]);
             out[i] = nullptr;
@@ -477,6 +545,7 @@ int cpm_bits_build_device_batch(cpm_ctx *ctx, uint32_t n_db, const uint64_t *con
     CPM_CHECK(bind(ctx));
     if (!out) return fail(CPM_ERR_INVALID, "out is NULL");
     if (n_db && (!d_rows || !d_cols || !nnz || !n_trans || !n_items)) return fail(CPM_ERR_INVALID, "null argument");
+    if (n_db > 65535) return fail(CPM_ERR_INVALID, "at most 65535 databases per batch build");
     std::vector<BuildIn> in(n_db);
     for (uint32_t i = 0; i < n_db; ++i) in[i] = {d_rows[i], d_cols[i], nnz[i], n_trans[i], n_items[i]};
     return build_stores(ctx, in.data(), n_db, true, out);
