@@ -73,7 +73,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="eclat_1m_1k",
-                    choices=["eclat_1m_1k", "pfeclat_2m_500", "batch_256"])
+                    choices=["eclat_1m_1k", "pfeclat_2m_500", "batch_256", "serve"])
+    ap.add_argument("--requests", type=int, default=24, help="serve: number of requests")
+    ap.add_argument("--qps", type=float, default=40.0, help="serve: mean arrival rate")
+    ap.add_argument("--hbm-budget-gb", type=float, default=8.0, help="serve: admission budget")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch level steps directly")
     ap.add_argument("--group", type=int, default=32, help="entries per support work item")
@@ -86,6 +89,9 @@ def main():
 
     if args.workload == "batch_256":
         print(json.dumps(run_batch(args)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(run_serve(args)), flush=True)
         return
 
     import numpy as np
@@ -363,6 +369,76 @@ def run_batch(args):
                                        "batched_ms": round(sub_batched_s * 1e3, 2),
                                        "speedup_batched": round(per_class_s / sub_batched_s, 1)}},
         "autotune": {f"{g}x{w}": round(v * 1e3, 3) for (g, w), v in table.items()},
+        "parity": parity,
+    }
+    return line
+
+
+def run_serve(args):
+    """Mining requests with Poisson arrivals, served with continuous batching
+    and HBM-footprint admission, against the same stream served one request
+    at a time."""
+    import numpy as np
+    from cupami_amd import _hip, datagen
+    from cupami_amd.serve import Request, Server, latency_summary, mine_one_at_a_time
+
+    t = time.perf_counter()
+    raw = datagen.serve_requests(args.requests, args.qps)
+    reqs = []
+    for i, (arrival, db, ms) in enumerate(raw):
+        labels, cols = datagen.label_columns(db.items)
+        reqs.append(Request(i, arrival, db.rows, cols, db.n_transactions, len(labels), ms))
+    pairs = sum(r.rows.size for r in reqs)
+    log(f"bench: generated {len(reqs)} requests ({pairs} pairs) in {time.perf_counter() - t:.1f}s")
+
+    ctx = _hip.context(0)
+    warm = [Request(r.rid, 0.0, r.rows, r.cols, r.n_transactions, r.n_items, r.minsup) for r in reqs[:4]]
+    Server(ctx, hbm_budget=int(args.hbm_budget_gb * (1 << 30))).run(warm)
+    mine_one_at_a_time(warm, ctx)
+
+    server = Server(ctx, hbm_budget=int(args.hbm_budget_gb * (1 << 30)))
+    t = time.perf_counter()
+    served = server.run(reqs)
+    served_s = time.perf_counter() - t
+    t = time.perf_counter()
+    alone = mine_one_at_a_time(reqs, ctx)
+    alone_s = time.perf_counter() - t
+    n_pat = sum(len(o.patterns) for o in served)
+    sizes = [len(b[2]) for b in server.batches]
+    log(f"bench: served {len(reqs)} requests in {served_s:.2f}s ({len(sizes)} batches, max {max(sizes)}), "
+        f"one at a time {alone_s:.2f}s")
+
+    parity = all(a.patterns == b.patterns for a, b in zip(served, alone) if not a.rejected)
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        threads = oracle.threads()
+        sample = reqs[: max(1, len(reqs) // 3)]
+        hosts = [oracle.bits_build(r.rows, r.cols, r.n_transactions, r.n_items) for r in sample]
+        t = time.perf_counter()
+        expect = [oracle.eclat(h, r.minsup) for h, r in zip(hosts, sample)]
+        cpu_s = time.perf_counter() - t
+        parity = parity and all(served[r.rid].patterns == e for r, e in zip(sample, expect))
+        n_exp = sum(len(e) for e in expect)
+        cpu = {"value": round(n_exp / cpu_s, 1), "unit": "itemsets/s", "cores": threads, "kind": "port",
+               "sample": f"the first {len(sample)} requests, one run ({n_exp} itemsets in {cpu_s:.2f}s): "
+                         "oracle/cpm_oracle.c vertical Eclat, OpenMP over prefix classes"}
+    log(f"bench: parity={parity}")
+    lat = latency_summary(served)
+    line = {
+        "metric": "serve p99 latency (ms), continuous batching", "value": lat["p99_ms"], "unit": "ms",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(served_s * 1e3, 1),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (numpy PCG64 seeds 505, 505000+i)",
+        "config": {"workload": "serve", "requests": len(reqs), "qps": args.qps, "pairs": pairs,
+                   "hbm_budget_gb": args.hbm_budget_gb, "patterns": n_pat, "parallelism": "single GPU"},
+        "roofline": None,
+        "cpu_baseline": cpu,
+        "served": {**lat, "itemsets_per_s": round(n_pat / served_s, 1), "makespan_s": round(served_s, 3),
+                   "batches": len(sizes), "max_batch": max(sizes), "mean_batch": round(sum(sizes) / len(sizes), 2),
+                   "rejected": sum(1 for o in served if o.rejected)},
+        "one_at_a_time": {**latency_summary(alone), "itemsets_per_s": round(n_pat / alone_s, 1),
+                          "makespan_s": round(alone_s, 3)},
         "parity": parity,
     }
     return line

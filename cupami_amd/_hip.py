@@ -43,6 +43,8 @@ SIGNATURES = {
     "cpm_bits_build_device_batch": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(_vp),
                                                    ctypes.POINTER(_vp), _u64p, _u64p, _u32p,
                                                    ctypes.POINTER(_vp)]),
+    "cpm_bits_build_batch": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                            _u64p, _u64p, _u32p, ctypes.POINTER(_vp)]),
     "cpm_bits_destroy": (ctypes.c_int, [_vp]),
     "cpm_bits_shape": (ctypes.c_int, [_vp, _u64p, _u32p, _u64p, _u64p]),
     "cpm_bits_download": (ctypes.c_int, [_vp, _vp, _u64p]),
@@ -200,6 +202,25 @@ class Context:
         out = (_vp * n)()
         check(load().cpm_bits_build_device_batch(self.handle, n, rows, cols, _p(nnz, _u64p), _p(nt, _u64p),
                                                  _p(ni, _u32p), out))
+        return [Bits(self, _vp(out[i])) for i in range(n)]
+
+    def build_bits_batch(self, dbs):
+        """Several stores from host pairs: ``dbs`` is a sequence of
+        ``(rows, cols, n_transactions, n_items)`` numpy arrays / ints.
+        Returns one ``Bits`` per database."""
+        n = len(dbs)
+        if n == 0:
+            return []
+        keep = [(np.ascontiguousarray(r, dtype=np.uint64), np.ascontiguousarray(c, dtype=np.uint32))
+                for r, c, _, _ in dbs]
+        rows = (_vp * n)(*[_vp(r.ctypes.data) if r.size else None for r, _ in keep])
+        cols = (_vp * n)(*[_vp(c.ctypes.data) if c.size else None for _, c in keep])
+        nnz = np.ascontiguousarray([r.size for r, _ in keep], dtype=np.uint64)
+        nt = np.ascontiguousarray([int(d[2]) for d in dbs], dtype=np.uint64)
+        ni = np.ascontiguousarray([int(d[3]) for d in dbs], dtype=np.uint32)
+        out = (_vp * n)()
+        check(load().cpm_bits_build_batch(self.handle, n, rows, cols, _p(nnz, _u64p), _p(nt, _u64p),
+                                          _p(ni, _u32p), out))
         return [Bits(self, _vp(out[i])) for i in range(n)]
 
     # ---- operators -----------------------------------------------------------

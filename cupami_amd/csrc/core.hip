@@ -551,6 +551,37 @@ int cpm_bits_build_device_batch(cpm_ctx *ctx, uint32_t n_db, const uint64_t *con
     return build_stores(ctx, in.data(), n_db, true, out);
 }
 
+int cpm_bits_build_batch(cpm_ctx *ctx, uint32_t n_db, const uint64_t *const *rows, const uint32_t *const *cols,
+                         const uint64_t *nnz, const uint64_t *n_trans, const uint32_t *n_items, cpm_bits **out) {
+    clear_error();
+    CPM_CHECK(bind(ctx));
+    if (!out) return fail(CPM_ERR_INVALID, "out is NULL");
+    if (n_db && (!rows || !cols || !nnz || !n_trans || !n_items)) return fail(CPM_ERR_INVALID, "null argument");
+    if (n_db > 65535) return fail(CPM_ERR_INVALID, "at most 65535 databases per batch build");
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n_db; ++i) {
+        if (nnz[i] && (!rows[i] || !cols[i])) return fail(CPM_ERR_INVALID, "rows/cols are NULL");
+        total += nnz[i];
+    }
+    // every database's pairs in one device staging area (rows, then columns)
+    DBuf &dr = ctx->scratch[0], &dc = ctx->scratch[1];
+    CPM_CHECK(dr.ensure(std::max<uint64_t>(total, 1) * sizeof(uint64_t)));
+    CPM_CHECK(dc.ensure(std::max<uint64_t>(total, 1) * sizeof(uint32_t)));
+    std::vector<BuildIn> in(n_db);
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < n_db; ++i) {
+        if (nnz[i]) {
+            CPM_HIP(hipMemcpyAsync(dr.as<uint64_t>() + off, rows[i], nnz[i] * sizeof(uint64_t),
+                                   hipMemcpyHostToDevice, ctx->stream));
+            CPM_HIP(hipMemcpyAsync(dc.as<uint32_t>() + off, cols[i], nnz[i] * sizeof(uint32_t),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        }
+        in[i] = {dr.as<uint64_t>() + off, dc.as<uint32_t>() + off, nnz[i], n_trans[i], n_items[i]};
+        off += nnz[i];
+    }
+    return build_stores(ctx, in.data(), n_db, true, out);
+}
+
 int cpm_bits_build(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, uint64_t nnz,
                    uint64_t n_trans, uint32_t n_items, cpm_bits **out) {
     clear_error();

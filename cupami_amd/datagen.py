@@ -132,6 +132,28 @@ def batch_256(seed=303, n_db=256):
     return out
 
 
+def serve_requests(n_requests, qps, seed=505):
+    """Mining requests for the serving loop: T log-uniform in [10^4, 2*10^6]
+    baskets, I uniform in [50, 2000] items, lam uniform in [4, 12], Zipf s
+    uniform in [0.9, 1.2], relative minimum support log-uniform in
+    [0.001, 0.05] (absolute ceil(minSup * T)), Poisson arrivals at ``qps``.
+    The parameters and arrival times come from ``seed``; request i is drawn
+    with seed ``seed * 1000 + i``.  Returns a list of
+    ``(arrival_seconds, Database, minsup_count)``."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = np.exp(rng.uniform(np.log(1e4), np.log(2e6), n_requests)).round().astype(np.int64)
+    items = rng.integers(50, 2001, n_requests)
+    lam = rng.uniform(4.0, 12.0, n_requests)
+    zipf = rng.uniform(0.9, 1.2, n_requests)
+    frac = np.exp(rng.uniform(np.log(0.001), np.log(0.05), n_requests))
+    arrival = np.cumsum(rng.exponential(1.0 / qps, n_requests))
+    out = []
+    for i in range(n_requests):
+        db = transactional(int(t[i]), int(items[i]), float(lam[i]), float(zipf[i]), seed * 1000 + i)
+        out.append((float(arrival[i]), db, int(np.ceil(frac[i] * t[i]))))
+    return out
+
+
 def label_columns(item_ids: np.ndarray, n_items_hint=None):
     """Map numeric item ids to column indexes in *string* label order.
 

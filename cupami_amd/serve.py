@@ -1,0 +1,155 @@
+"""Serving loop: many mining requests on one GPU, with continuous batching
+and admission by HBM footprint.
+
+Each request is one database (COO pairs in host memory) and an absolute
+minimum support.  The loop keeps a FIFO queue of arrived requests.  Whenever
+the device is free it admits, in arrival order, every waiting request whose
+footprint still fits the HBM budget (a request larger than the whole budget
+is rejected; one that does not fit waits for the next turn), builds all
+admitted stores with one batch call, mines them with one search
+(``cpm_eclat_mine_batch``) and hands each request its own patterns.
+Requests that arrive while a batch runs join the next batch.
+
+``mine_one_at_a_time`` serves the same arrival stream one request at a time
+(one store build and one search per request), the plain way to run the
+reference's miner repeatedly; both return per-request latencies.  A
+request's patterns are the same whichever way it is served.
+"""
+
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _hip
+
+SEARCH_RESERVE = 64 << 20   # bytes per request for its share of the search buffers
+
+
+def row_stride_words(n_transactions):
+    words = (n_transactions + 63) // 64
+    return (words + 63) // 64 * 64
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float            # seconds after the start of the run
+    rows: np.ndarray          # uint64 transaction index per pair
+    cols: np.ndarray          # uint32 item column per pair
+    n_transactions: int
+    n_items: int
+    minsup: int
+
+    def footprint(self):
+        """Device bytes while it is being served: its store, its staged
+        pairs, and a share of the search buffers."""
+        store = self.n_items * row_stride_words(self.n_transactions) * 8
+        return store + 12 * int(self.rows.size) + SEARCH_RESERVE
+
+
+@dataclass
+class Outcome:
+    rid: int
+    arrival: float
+    finish: float = 0.0
+    patterns: dict = field(default_factory=dict)
+    batch: int = -1
+    rejected: str = ""
+
+    @property
+    def latency(self):
+        return self.finish - self.arrival
+
+
+class Server:
+    def __init__(self, ctx=None, hbm_budget=8 << 30, max_batch=256):
+        self.ctx = ctx or _hip.context(0)
+        self.hbm_budget = int(hbm_budget)
+        self.max_batch = int(max_batch)
+        self.batches = []      # (start, end, request ids)
+
+    def _admit(self, queue):
+        """Admit in arrival order while the budget allows."""
+        batch, used = [], 0
+        while queue and len(batch) < self.max_batch:
+            fp = queue[0].footprint()
+            if used + fp > self.hbm_budget:
+                break
+            batch.append(queue.popleft())
+            used += fp
+        return batch
+
+    def _mine(self, batch):
+        stores = self.ctx.build_bits_batch([(r.rows, r.cols, r.n_transactions, r.n_items) for r in batch])
+        try:
+            forest = self.ctx.eclat_mine_batch(stores, [r.minsup for r in batch])
+        finally:
+            for s in stores:
+                s.close()
+        return forest.split(len(batch))
+
+    def run(self, requests):
+        """Serve ``requests`` (any order; arrival times relative to now).
+        Returns one ``Outcome`` per request, in request-id order."""
+        pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+        out = {r.rid: Outcome(r.rid, r.arrival) for r in requests}
+        queue = deque()
+        t0 = time.perf_counter()
+        while pending or queue:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                r = pending.popleft()
+                if r.footprint() > self.hbm_budget:
+                    out[r.rid].rejected = f"footprint {r.footprint()} B exceeds the budget {self.hbm_budget} B"
+                    out[r.rid].finish = now
+                else:
+                    queue.append(r)
+            if not queue:
+                if pending:
+                    time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+                continue
+            batch = self._admit(queue)
+            start = time.perf_counter() - t0
+            results = self._mine(batch)
+            end = time.perf_counter() - t0
+            k = len(self.batches)
+            self.batches.append((start, end, [r.rid for r in batch]))
+            for r, res in zip(batch, results):
+                o = out[r.rid]
+                o.finish, o.patterns, o.batch = end, res, k
+        return [out[rid] for rid in sorted(out)]
+
+
+def mine_one_at_a_time(requests, ctx=None):
+    """The same arrival stream, one request at a time in arrival order."""
+    ctx = ctx or _hip.context(0)
+    order = sorted(requests, key=lambda r: (r.arrival, r.rid))
+    out = {}
+    t0 = time.perf_counter()
+    for r in order:
+        now = time.perf_counter() - t0
+        if r.arrival > now:
+            time.sleep(r.arrival - now)
+        bits = ctx.build_bits(r.rows, r.cols, r.n_transactions, r.n_items)
+        try:
+            pats = ctx.eclat_mine(bits, r.minsup).to_dict()
+        finally:
+            bits.close()
+        o = Outcome(r.rid, r.arrival)
+        o.finish = time.perf_counter() - t0
+        o.patterns = pats
+        out[r.rid] = o
+    return [out[rid] for rid in sorted(out)]
+
+
+def latency_summary(outcomes):
+    lat = np.array([o.latency for o in outcomes if not o.rejected])
+    if lat.size == 0:
+        return {"p50_ms": None, "p99_ms": None, "mean_ms": None}
+    return {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+            "mean_ms": round(float(lat.mean()) * 1e3, 2)}

@@ -70,6 +70,13 @@ int cpm_bits_build_device(cpm_ctx *ctx, const uint64_t *d_rows, const uint32_t *
 int cpm_bits_build_device_batch(cpm_ctx *ctx, uint32_t n_db, const uint64_t *const *d_rows,
                                 const uint32_t *const *d_cols, const uint64_t *nnz,
                                 const uint64_t *n_transactions, const uint32_t *n_items, cpm_bits **out);
+/* cpm_bits_build_device_batch with the pairs in host memory (each
+ * database's rows/cols arrays, as cpm_bits_build takes them): the pairs are
+ * copied into one device staging area first.  For a serving loop that
+ * receives requests from the host. */
+int cpm_bits_build_batch(cpm_ctx *ctx, uint32_t n_db, const uint64_t *const *rows, const uint32_t *const *cols,
+                         const uint64_t *nnz, const uint64_t *n_transactions, const uint32_t *n_items,
+                         cpm_bits **out);
 int cpm_bits_destroy(cpm_bits *bits);
 /* words per row (ceil(N/64)), and the padded row stride in words */
 int cpm_bits_shape(const cpm_bits *bits, uint64_t *n_transactions, uint32_t *n_items,
