@@ -74,8 +74,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="eclat_1m_1k",
                     choices=["eclat_1m_1k", "pfeclat_2m_500", "batch_256", "serve"])
-    ap.add_argument("--requests", type=int, default=24, help="serve: number of requests")
-    ap.add_argument("--qps", type=float, default=40.0, help="serve: mean arrival rate")
+    ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
+    ap.add_argument("--qps", type=float, default=500.0, help="serve: mean arrival rate")
     ap.add_argument("--hbm-budget-gb", type=float, default=8.0, help="serve: admission budget")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch level steps directly")
@@ -392,17 +392,22 @@ def run_serve(args):
     log(f"bench: generated {len(reqs)} requests ({pairs} pairs) in {time.perf_counter() - t:.1f}s")
 
     ctx = _hip.context(0)
-    warm = [Request(r.rid, 0.0, r.rows, r.cols, r.n_transactions, r.n_items, r.minsup) for r in reqs[:4]]
-    Server(ctx, hbm_budget=int(args.hbm_budget_gb * (1 << 30))).run(warm)
-    mine_one_at_a_time(warm, ctx)
 
-    server = Server(ctx, hbm_budget=int(args.hbm_budget_gb * (1 << 30)))
-    t = time.perf_counter()
-    served = server.run(reqs)
-    served_s = time.perf_counter() - t
-    t = time.perf_counter()
-    alone = mine_one_at_a_time(reqs, ctx)
-    alone_s = time.perf_counter() - t
+    # both schedules twice, alternating (the first runs grow the device
+    # buffers); each keeps its faster run
+    served_s = alone_s = float("inf")
+    for _ in range(2):
+        srv = Server(ctx, hbm_budget=int(args.hbm_budget_gb * (1 << 30)))
+        t = time.perf_counter()
+        out = srv.run(reqs)
+        dt = time.perf_counter() - t
+        if dt < served_s:
+            served_s, served, server = dt, out, srv
+        t = time.perf_counter()
+        out = mine_one_at_a_time(reqs, ctx)
+        dt = time.perf_counter() - t
+        if dt < alone_s:
+            alone_s, alone = dt, out
     n_pat = sum(len(o.patterns) for o in served)
     sizes = [len(b[2]) for b in server.batches]
     log(f"bench: served {len(reqs)} requests in {served_s:.2f}s ({len(sizes)} batches, max {max(sizes)}), "

@@ -439,6 +439,11 @@ int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bi
         if (ctx->spare_arena.p && ctx->spare_arena.cap >= total) {
             std::swap(arena->buf.p, ctx->spare_arena.p);
             std::swap(arena->buf.cap, ctx->spare_arena.cap);
+        } else {
+            // a new arena, with headroom so that batches of varying size
+            // rarely allocate again (the old one is freed)
+            ctx->spare_arena.release();
+            total += total / 2;
         }
         CPM_CHECK(arena->buf.ensure(total));
     }

@@ -3,12 +3,16 @@ and admission by HBM footprint.
 
 Each request is one database (COO pairs in host memory) and an absolute
 minimum support.  The loop keeps a FIFO queue of arrived requests.  Whenever
-the device is free it admits, in arrival order, every waiting request whose
-footprint still fits the HBM budget (a request larger than the whole budget
-is rejected; one that does not fit waits for the next turn), builds all
-admitted stores with one batch call, mines them with one search
-(``cpm_eclat_mine_batch``) and hands each request its own patterns.
-Requests that arrive while a batch runs join the next batch.
+the device is free it admits, in arrival order, waiting requests while their
+footprints fit the HBM budget and their pairs fit the batch work cap (a
+request larger than the whole budget is rejected; one that does not fit
+waits for the next turn), builds all admitted stores with one batch call,
+mines them with one search (``cpm_eclat_mine_batch``) and hands each request
+its own patterns.  Requests that arrive while a batch runs join the next
+batch.  The work cap keeps batching where it pays: a request with millions
+of pairs fills the device by itself and runs alone (packing it with others
+would only make them wait for its deeper levels), while small requests,
+which leave the device mostly idle one at a time, share a search.
 
 ``mine_one_at_a_time`` serves the same arrival stream one request at a time
 (one store build and one search per request), the plain way to run the
@@ -66,24 +70,35 @@ class Outcome:
 
 
 class Server:
-    def __init__(self, ctx=None, hbm_budget=8 << 30, max_batch=256):
+    def __init__(self, ctx=None, hbm_budget=8 << 30, max_batch=256, max_batch_pairs=4 << 20):
         self.ctx = ctx or _hip.context(0)
         self.hbm_budget = int(hbm_budget)
         self.max_batch = int(max_batch)
+        self.max_batch_pairs = int(max_batch_pairs)
         self.batches = []      # (start, end, request ids)
 
     def _admit(self, queue):
-        """Admit in arrival order while the budget allows."""
-        batch, used = [], 0
+        """Admit in arrival order while the HBM budget and the work cap allow
+        (the first request always fits: oversize ones were rejected)."""
+        batch, used, pairs = [], 0, 0
         while queue and len(batch) < self.max_batch:
             fp = queue[0].footprint()
-            if used + fp > self.hbm_budget:
+            n = int(queue[0].rows.size)
+            if batch and (used + fp > self.hbm_budget or pairs + n > self.max_batch_pairs):
                 break
             batch.append(queue.popleft())
             used += fp
+            pairs += n
         return batch
 
     def _mine(self, batch):
+        if len(batch) == 1:  # nothing to share: the single-store path
+            r = batch[0]
+            bits = self.ctx.build_bits(r.rows, r.cols, r.n_transactions, r.n_items)
+            try:
+                return [self.ctx.eclat_mine(bits, r.minsup).to_dict()]
+            finally:
+                bits.close()
         stores = self.ctx.build_bits_batch([(r.rows, r.cols, r.n_transactions, r.n_items) for r in batch])
         try:
             forest = self.ctx.eclat_mine_batch(stores, [r.minsup for r in batch])

@@ -26,6 +26,18 @@ class StubCtx:
     def eclat_mine_batch(self, stores, minsups):
         return StubForest(len(stores))
 
+    def build_bits(self, rows, cols, n_trans, n_items):
+        self.batches.append(1)
+        return StubStore()
+
+    def eclat_mine(self, bits, minsup):
+        return StubPatterns()
+
+
+class StubPatterns:
+    def to_dict(self):
+        return {("alone",): 1}
+
 
 class StubStore:
     closed = 0
@@ -51,7 +63,7 @@ def test_admission_rejects_defers_and_batches():
             req(3, 0.0, 10_000_000, 5000, 10),  # larger than the whole budget
             req(4, 0.01, 64, 1, 10)]
     ctx = StubCtx()
-    server = Server(ctx, hbm_budget=budget, max_batch=8)
+    server = Server(ctx, hbm_budget=budget, max_batch=8, max_batch_pairs=1 << 20)
     out = server.run(reqs)
     assert [o.rid for o in out] == [0, 1, 2, 3, 4]
     assert out[3].rejected and not out[3].patterns
@@ -64,3 +76,11 @@ def test_admission_rejects_defers_and_batches():
     assert StubStore.closed >= 4
     lat = latency_summary(out)
     assert lat["p50_ms"] is not None and lat["p99_ms"] >= lat["p50_ms"]
+
+
+def test_work_cap_runs_large_requests_alone():
+    reqs = [req(0, 0.0, 64, 1, 10), req(1, 0.0, 64, 1, 3000), req(2, 0.0, 64, 1, 10), req(3, 0.0, 64, 1, 10)]
+    server = Server(StubCtx(), hbm_budget=1 << 40, max_batch=8, max_batch_pairs=2000)
+    out = server.run(reqs)
+    assert [b[2] for b in server.batches] == [[0], [1], [2, 3]]
+    assert all(o.patterns for o in out)
