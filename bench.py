@@ -69,14 +69,16 @@ def csrc_digest():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="eclat_1m_1k",
                     choices=["eclat_1m_1k", "pfeclat_2m_500", "batch_256", "serve"])
     ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
     ap.add_argument("--qps", type=float, default=500.0, help="serve: mean arrival rate")
     ap.add_argument("--hbm-budget-gb", type=float, default=8.0, help="serve: admission budget")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="eclat_1m_1k: skip the batched and serving summaries")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch level steps directly")
     ap.add_argument("--group", type=int, default=32, help="entries per support work item")
@@ -206,9 +208,36 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "gram": gram,
         "levels_per_step": stats["levels"], "parity": parity,
     }
+    if not periodic and not args.no_extras:
+        line["batched"], line["serve"] = extras(args)
+        if line["batched"]["parity"] is False or line["serve"]["parity"] is False:
+            parity = False
     print(json.dumps(line), flush=True)
     if parity is False:
         sys.exit(1)
+
+
+def extras(args):
+    """Compact summaries of the batched (256 databases) and serving
+    workloads for the headline line (their own workloads report in full)."""
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup, sub.no_cpu_baseline = 3, 1, True
+    b = run_batch(sub)
+    batched = {"workload": "batch_256", "databases": b["config"]["databases"],
+               "patterns": b["config"]["patterns"], "itemsets_per_s": b["value"],
+               "ms_per_step": b["ms_per_step"], "one_at_a_time_ms": b["plain"]["one_at_a_time_ms"],
+               "speedup_vs_one_at_a_time": b["plain"]["speedup_batched"],
+               "speedup_vs_per_class_subset": b["plain"]["per_class_subset"]["speedup_batched"],
+               "schedule": b["config"]["schedule"], "parity": b["parity"]}
+    v = run_serve(sub)
+    serve = {"workload": "serve", "requests": v["config"]["requests"], "qps": v["config"]["qps"],
+             "p50_ms": v["served"]["p50_ms"], "p99_ms": v["served"]["p99_ms"],
+             "itemsets_per_s": v["served"]["itemsets_per_s"],
+             "one_at_a_time_p50_ms": v["one_at_a_time"]["p50_ms"],
+             "one_at_a_time_p99_ms": v["one_at_a_time"]["p99_ms"],
+             "one_at_a_time_itemsets_per_s": v["one_at_a_time"]["itemsets_per_s"],
+             "batches": v["served"]["batches"], "max_batch": v["served"]["max_batch"], "parity": v["parity"]}
+    return batched, serve
 
 
 def per_class_plain(ctx, labels, bits, n, minsup):
