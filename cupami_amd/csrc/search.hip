@@ -660,6 +660,17 @@ __global__ void k_level1(uint32_t n_db, const uint32_t *ext_off, uint32_t *ext, 
 }
 
 // fcol[fbase_i + j] = column of database i's j-th frequent item
+// Level-1 results for the host, placed right after the control block so
+// that one read returns both: frequent items per database, then their
+// exclusive offsets (n_db + 1 words).
+constexpr size_t kCtlBytes = (sizeof(Ctl) + 255) / 256 * 256;
+
+__global__ void k_level1_pack(uint32_t n_db, const uint32_t *cnt, const uint32_t *off, Ctl *ctl) {
+    uint32_t *dst = reinterpret_cast<uint32_t *>(reinterpret_cast<unsigned char *>(ctl) + kCtlBytes);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n_db + 1; i += gridDim.x * blockDim.x)
+        dst[i] = i < n_db ? cnt[i] : off[i - n_db];
+}
+
 __global__ void k_fcol(uint32_t n_db, const uint32_t *ext_off, const uint32_t *kept, const uint32_t *ext,
                        const uint32_t *off_pat, const uint32_t *cnt_pat, uint32_t *fcol) {
     for (uint32_t t = blockIdx.x; t < n_db; t += gridDim.x)
@@ -731,6 +742,30 @@ __global__ void k_pair_table(PlanArgs p, uint32_t *bits) {
     }
 }
 
+// Queued host -> device copies, applied by one launch: record r copies
+// `bytes` (a multiple of 4) from the blob at `src` to `dst`, or zeroes them
+// when src == kZeroFill.
+constexpr uint64_t kZeroFill = ~0ull;
+struct PutRec {
+    uint32_t *dst;
+    uint64_t src;
+    uint64_t bytes;
+};
+
+__global__ __launch_bounds__(256) void k_scatter(const PutRec *__restrict__ recs, uint32_t n,
+                                                const uint32_t *__restrict__ blob) {
+    for (uint32_t r = blockIdx.x; r < n; r += gridDim.x) {
+        const PutRec rec = recs[r];
+        const uint64_t words = rec.bytes / 4;
+        if (rec.src == kZeroFill) {
+            for (uint64_t i = threadIdx.x; i < words; i += blockDim.x) rec.dst[i] = 0;
+        } else {
+            const uint32_t *src = blob + rec.src / 4;
+            for (uint64_t i = threadIdx.x; i < words; i += blockDim.x) rec.dst[i] = src[i];
+        }
+    }
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -791,8 +826,9 @@ struct GraphKey {
 
 struct SearchWs {
     LevelSet set[2];
-    DBuf ctl, dbs, fcol, pairbits, tmask, gram, goff, gjobs, gdbs, l2, part, partl, pat_parent, pat_item, pat_sup,
-        pat_per, ts, stamps;
+    DBuf ctl, dbs, fcol, pairbits, tmask, gram, goff, gjobs, gdbs, l2, part, partl, scatter, ts, stamps;
+    DBuf pat;             // pattern forest: parent, item and support columns, pat_pitch() bytes apart
+    DBuf pat_per;
     HBuf host;
     HBuf stage;           // pinned staging of small host <-> device copies
     size_t stage_off = 0; // in use since the last stream sync
@@ -834,6 +870,7 @@ struct Search {
     uint32_t G = 32, tile_words = kTileWords;
     uint32_t te = 1;                 // tiles per support work range
     uint32_t tmask_w = 0;            // words per task kept mask (0: no plan/fill)
+    bool ctl_current = false;        // the host copy of Ctl is final (read after the last step)
     std::vector<DbDev> dbs;          // host copy
     std::vector<uint32_t> nf;        // frequent items per database
     uint32_t parts_tiles = 0;
@@ -841,6 +878,10 @@ struct Search {
     Search(cpm_ctx *c, SearchWs &ws) : ctx(c), w(ws) {}
 
     Ctl *ctl() const { return w.ctl.as<Ctl>(); }
+    size_t pat_pitch() const { return ((w.caps.patterns + 1) * 4 + 255) / 256 * 256; }
+    uint32_t *pat_col(int k) const {
+        return reinterpret_cast<uint32_t *>(w.pat.as<unsigned char>() + (size_t)k * pat_pitch());
+    }
     LevelSizes *sz(int i) const { return &ctl()->sz[i]; }
 
     PlanArgs plan_args(int c, int emit_children) const {
@@ -870,9 +911,9 @@ struct Search {
         p.stamp_blocks = grid();
         p.tmask = emit_children && tmask_w ? w.tmask.as<uint32_t>() : nullptr;
         p.tmask_w = tmask_w;
-        p.pat_parent = w.pat_parent.as<uint32_t>();
-        p.pat_item = w.pat_item.as<uint32_t>();
-        p.pat_sup = w.pat_sup.as<uint32_t>();
+        p.pat_parent = pat_col(0);
+        p.pat_item = pat_col(1);
+        p.pat_sup = pat_col(2);
         p.pat_per = periodic ? w.pat_per.as<int64_t>() : nullptr;
         p.ctl = ctl();
         p.caps = w.caps;
@@ -943,7 +984,7 @@ struct Search {
         }
         add(w.ctl); add(w.dbs); add(w.fcol); add(w.pairbits); add(w.tmask); add(w.part); add(w.partl);
         add(w.stamps);
-        add(w.pat_parent); add(w.pat_item); add(w.pat_sup); add(w.pat_per);
+        add(w.pat); add(w.pat_per);
         k.caps[0] = w.caps.tasks;
         k.caps[1] = w.caps.ext;
         k.caps[2] = w.caps.patterns;
@@ -991,20 +1032,20 @@ struct Search {
 
     int reserve() {
         for (int i = 0; i < 2; ++i) CPM_CHECK(w.set[i].reserve(w.caps, periodic, parts_tiles));
-        CPM_CHECK(w.pat_parent.ensure((w.caps.patterns + 1) * 4));
-        CPM_CHECK(w.pat_item.ensure((w.caps.patterns + 1) * 4));
-        CPM_CHECK(w.pat_sup.ensure((w.caps.patterns + 1) * 4));
+        CPM_CHECK(w.pat.ensure(3 * pat_pitch()));
         if (periodic) CPM_CHECK(w.pat_per.ensure((w.caps.patterns + 1) * 8));
         CPM_CHECK(w.part.ensure((size_t)3 * grid_tasks() * 8));
         CPM_CHECK(w.partl.ensure((size_t)grid() * 8));
-        CPM_CHECK(w.ctl.ensure(sizeof(Ctl)));
+        CPM_CHECK(w.ctl.ensure(kCtlBytes + (2 * in.size() + 1) * 4));
         CPM_CHECK(w.stamps.ensure((size_t)grid() * 4 * 8));
-        CPM_CHECK(w.host.ensure(std::max<size_t>(sizeof(Ctl), 4096)));
+        CPM_CHECK(w.host.ensure(std::max<size_t>(kCtlBytes + (2 * in.size() + 1) * 4, 4096)));
         return CPM_OK;
     }
 
-    int read_ctl(Ctl *h) {
-        CPM_HIP(hipMemcpyAsync(h, w.ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
+    // the control block (and `tail` bytes after it) into pinned host memory
+    int read_ctl(Ctl *h, size_t tail = 0) {
+        CPM_HIP(hipMemcpyAsync(h, w.ctl.p, tail ? kCtlBytes + tail : sizeof(Ctl), hipMemcpyDeviceToHost,
+                               ctx->stream));
         CPM_HIP(hipStreamSynchronize(ctx->stream));
         w.stage_off = 0;
         return CPM_OK;
@@ -1024,13 +1065,60 @@ struct Search {
         return CPM_OK;
     }
 
-    // asynchronous host -> device copy of pageable data through staging
-    int put(void *dst, const void *src, size_t n) {
-        if (n == 0) return CPM_OK;
+    // Queued host -> device copies (qput / qzero), applied by flush() with
+    // one copy of the packed records and data and one k_scatter launch
+    // instead of a copy launch each.  Large records are cut into 64 KB
+    // pieces so the launch spreads them over many blocks.
+    std::vector<PutRec> recs;
+    std::vector<unsigned char> blob;
+    static constexpr uint64_t kPiece = 64 << 10;
+
+    void qput(void *dst, const void *src, size_t n) {
+        for (size_t off = 0; off < n; off += kPiece) {
+            const size_t m = std::min<size_t>(kPiece, n - off);
+            recs.push_back({reinterpret_cast<uint32_t *>(static_cast<unsigned char *>(dst) + off),
+                            (uint64_t)blob.size(), (uint64_t)m});
+            blob.insert(blob.end(), static_cast<const unsigned char *>(src) + off,
+                        static_cast<const unsigned char *>(src) + off + m);
+            blob.resize((blob.size() + 15) / 16 * 16, 0);
+        }
+    }
+
+    void qzero(void *dst, size_t n) {
+        for (size_t off = 0; off < n; off += kPiece)
+            recs.push_back({reinterpret_cast<uint32_t *>(static_cast<unsigned char *>(dst) + off), kZeroFill,
+                            (uint64_t)std::min<size_t>(kPiece, n - off)});
+    }
+
+    int flush() {
+        if (recs.empty()) return CPM_OK;
+        for (const PutRec &r : recs)
+            if (r.bytes % 4 || reinterpret_cast<uintptr_t>(r.dst) % 4)
+                return fail(CPM_ERR_INVALID, "queued copy of %llu bytes is not word aligned",
+                            (unsigned long long)r.bytes);
+        // the records are applied in no particular order: none may overlap
+        std::vector<std::pair<uintptr_t, uintptr_t>> span(recs.size());
+        for (size_t i = 0; i < recs.size(); ++i)
+            span[i] = {reinterpret_cast<uintptr_t>(recs[i].dst), reinterpret_cast<uintptr_t>(recs[i].dst) + recs[i].bytes};
+        std::sort(span.begin(), span.end());
+        for (size_t i = 1; i < span.size(); ++i)
+            if (span[i].first < span[i - 1].second)
+                return fail(CPM_ERR_INVALID, "queued copies overlap");
+        const size_t head = (recs.size() * sizeof(PutRec) + 255) / 256 * 256;
+        const size_t total = head + blob.size();
         unsigned char *h = nullptr;
-        CPM_CHECK(stage(n, &h));
-        memcpy(h, src, n);
-        CPM_HIP(hipMemcpyAsync(dst, h, n, hipMemcpyHostToDevice, ctx->stream));
+        CPM_CHECK(stage(total, &h));
+        memcpy(h, recs.data(), recs.size() * sizeof(PutRec));
+        if (!blob.empty()) memcpy(h + head, blob.data(), blob.size());
+        CPM_CHECK(w.scatter.ensure(total));
+        CPM_HIP(hipMemcpyAsync(w.scatter.p, h, total, hipMemcpyHostToDevice, ctx->stream));
+        const uint32_t n = (uint32_t)recs.size();
+        hipLaunchKernelGGL(k_scatter, dim3(std::min<uint32_t>(n, 4096)), dim3(256), 0, ctx->stream,
+                           w.scatter.as<PutRec>(), n,
+                           reinterpret_cast<const uint32_t *>(w.scatter.as<unsigned char>() + head));
+        CPM_HIP(hipGetLastError());
+        recs.clear();
+        blob.clear();
         return CPM_OK;
     }
 
@@ -1042,10 +1130,11 @@ struct Search {
         hipStream_t st = ctx->stream;
         CPM_CHECK(reserve());
         Ctl *hc = w.host.as<Ctl>();
-        memset(hc, 0, sizeof(Ctl));
-        CPM_HIP(hipMemcpyAsync(w.ctl.p, hc, sizeof(Ctl), hipMemcpyHostToDevice, st));
-        // stamps left by an attempt that overflowed (outside any graph)
-        CPM_HIP(hipMemsetAsync(w.stamps.p, 0, (size_t)grid() * 4 * 8, st));
+        recs.clear();
+        blob.clear();
+        ctl_current = false;
+        // stamps left by an attempt that overflowed
+        qzero(w.stamps.p, (size_t)grid() * 4 * 8);
 
         // database table (frequent ranges filled after level 1)
         dbs.assign(n_db, DbDev{});
@@ -1068,7 +1157,7 @@ struct Search {
             total_items += b->n_items;
         }
         CPM_CHECK(w.dbs.ensure(std::max<uint32_t>(n_db, 1) * sizeof(DbDev)));
-        CPM_CHECK(put(w.dbs.p, dbs.data(), n_db * sizeof(DbDev)));
+        qput(w.dbs.p, dbs.data(), n_db * sizeof(DbDev));
 
         // ---- level 1 (set 0) ----
         if (n_db > w.caps.tasks || total_items > w.caps.ext) {
@@ -1088,7 +1177,7 @@ struct Search {
                 e += in[i].bits->n_items;
             }
             off[n_db] = (uint32_t)e;
-            CPM_CHECK(put(S0.ext_off.p, off.data(), (n_db + 1) * 4));
+            qput(S0.ext_off.p, off.data(), (n_db + 1) * 4);
             LevelSizes l1;
             memset(&l1, 0, sizeof(l1));
             l1.n_tasks = n_db;
@@ -1097,7 +1186,12 @@ struct Search {
             l1.n_cand = e;
             l1.group = level_group(e, G, te, work_target());
             l1.n_work = level_work(e, (uint32_t)l1.group, te);
-            CPM_CHECK(put(sz(0), &l1, sizeof(l1)));
+            // the whole control block, zeroed but for level 1's sizes
+            Ctl c0;
+            memset(&c0, 0, sizeof(c0));
+            c0.sz[0] = l1;
+            qput(w.ctl.p, &c0, sizeof(c0));
+            CPM_CHECK(flush());
             hipLaunchKernelGGL(k_level1, dim3(std::min<uint32_t>(n_db, 4096)), dim3(256), 0, st, n_db,
                                S0.ext_off.as<uint32_t>(), S0.ext.as<uint32_t>(), S0.sup.as<uint32_t>(),
                                S0.db.as<uint32_t>(), S0.pid.as<uint32_t>());
@@ -1109,18 +1203,22 @@ struct Search {
         CPM_HIP(hipGetLastError());
         nf.assign(n_db, 0);
         std::vector<uint32_t> fbase(n_db + 1);
-        unsigned char *h_nf = nullptr;  // one area: a second stage() call may regrow the arena
-        CPM_CHECK(stage((size_t)(2 * n_db + 1) * 4, &h_nf));
-        unsigned char *h_fb = h_nf + (size_t)n_db * 4;
-        CPM_HIP(hipMemcpyAsync(h_nf, S0.cnt[0].p, n_db * 4, hipMemcpyDeviceToHost, st));
-        CPM_HIP(hipMemcpyAsync(h_fb, S0.off[0].p, (n_db + 1) * 4, hipMemcpyDeviceToHost, st));
-        CPM_CHECK(read_ctl(hc));
-        memcpy(nf.data(), h_nf, n_db * 4);
-        memcpy(fbase.data(), h_fb, (n_db + 1) * 4);
+        hipLaunchKernelGGL(k_level1_pack, dim3(std::min<uint32_t>((2 * n_db + 256) / 256, 64)), dim3(256), 0, st,
+                           n_db, S0.cnt[0].as<uint32_t>(), S0.off[0].as<uint32_t>(), ctl());
+        CPM_HIP(hipGetLastError());
+        CPM_CHECK(read_ctl(hc, (size_t)(2 * n_db + 1) * 4));
+        {
+            const unsigned char *tail = reinterpret_cast<const unsigned char *>(hc) + kCtlBytes;
+            memcpy(nf.data(), tail, n_db * 4);
+            memcpy(fbase.data(), tail + (size_t)n_db * 4, (n_db + 1) * 4);
+        }
         if (hc->overflow) {
             *overflow = hc->overflow;
             return CPM_OK;
         }
+        // the support stamps are folded and nothing below changes the
+        // control block unless a level-2 search follows
+        ctl_current = true;
         const uint64_t nf_total = fbase[n_db];
         if (nf_total == 0) return CPM_OK;
         CPM_CHECK(w.fcol.ensure(nf_total * 4));
@@ -1152,8 +1250,9 @@ struct Search {
             for (auto &D : dbs) D.pair_w32 = 0;
             pair_words = 0;
         }
-        CPM_CHECK(put(w.dbs.p, dbs.data(), n_db * sizeof(DbDev)));
-        if (t2 == 0) return CPM_OK;
+        qput(w.dbs.p, dbs.data(), n_db * sizeof(DbDev));
+        if (t2 == 0) return flush();
+        ctl_current = false;
         if (t2 > w.caps.tasks || e2 > w.caps.ext || e2 >= 0xffffffffull) {
             *overflow = (t2 > w.caps.tasks ? kOvTasks : 0) | (e2 > w.caps.ext ? kOvEntries : 0);
             if (e2 >= 0xffffffffull) return fail(CPM_ERR_NOMEM, "too many frequent item pairs");
@@ -1165,7 +1264,7 @@ struct Search {
         }
         LevelSet &S1 = w.set[1];
         CPM_CHECK(w.l2.ensure(n_db * sizeof(L2Db)));
-        CPM_CHECK(put(w.l2.p, l2.data(), n_db * sizeof(L2Db)));
+        qput(w.l2.p, l2.data(), n_db * sizeof(L2Db));
         {
             LevelSizes ls;
             memset(&ls, 0, sizeof(ls));
@@ -1175,9 +1274,10 @@ struct Search {
             ls.n_cand = e2;
             ls.group = level_group(e2, G, te, work_target());
             ls.n_work = level_work(e2, (uint32_t)ls.group, te);
-            CPM_CHECK(put(sz(1), &ls, sizeof(ls)));
+            qput(sz(1), &ls, sizeof(ls));
             const uint32_t e2u = (uint32_t)e2;
-            CPM_CHECK(put(S1.ext_off.as<uint32_t>() + t2, &e2u, 4));
+            qput(S1.ext_off.as<uint32_t>() + t2, &e2u, 4);
+            CPM_CHECK(flush());
             hipLaunchKernelGGL(k_level2, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0, st,
                                w.l2.as<L2Db>(), n_db, (uint32_t)t2, S1.ptrs(sz(1)), w.dbs.as<DbDev>());
             CPM_HIP(hipGetLastError());
@@ -1208,9 +1308,10 @@ struct Search {
             CPM_CHECK(w.goff.ensure(n_db * 8));
             CPM_CHECK(w.gjobs.ensure(std::max<size_t>(jobs.size(), 1) * sizeof(GramJob)));
             CPM_CHECK(w.gdbs.ensure(n_db * sizeof(GramDb)));
-            CPM_CHECK(put(w.goff.p, goff.data(), n_db * 8));
-            CPM_CHECK(put(w.gjobs.p, jobs.data(), jobs.size() * sizeof(GramJob)));
-            CPM_CHECK(put(w.gdbs.p, gdb.data(), n_db * sizeof(GramDb)));
+            qput(w.goff.p, goff.data(), n_db * 8);
+            qput(w.gjobs.p, jobs.data(), jobs.size() * sizeof(GramJob));
+            qput(w.gdbs.p, gdb.data(), n_db * sizeof(GramDb));
+            CPM_CHECK(flush());
             CPM_CHECK(launch_gram_i8(ctx, w.gjobs.as<GramJob>(), jobs.size(), w.gdbs.as<GramDb>(),
                                      w.gram.as<int32_t>(), elems, ops));
             hipLaunchKernelGGL(k_gram_to_level2, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0,
@@ -1253,7 +1354,12 @@ struct Search {
                 *overflow = hc->overflow;
                 return CPM_OK;
             }
-            if (hc->sz[c].n_tasks == 0) break;
+            if (hc->sz[c].n_tasks == 0) {
+                // level c is empty: its support launch wrote no stamps, and
+                // the plan that made it folded the previous launch's
+                ctl_current = true;
+                break;
+            }
         }
         return CPM_OK;
     }
@@ -1297,10 +1403,12 @@ struct Search {
 
     int download(cpm_patterns **out) {
         Ctl *hc = w.host.as<Ctl>();
-        hipLaunchKernelGGL(k_fold_stamps, dim3(1), dim3(256), 0, ctx->stream, w.stamps.as<unsigned long long>(),
-                           grid(), ctl());
-        CPM_HIP(hipGetLastError());
-        CPM_CHECK(read_ctl(hc));
+        if (!ctl_current) {
+            hipLaunchKernelGGL(k_fold_stamps, dim3(1), dim3(256), 0, ctx->stream,
+                               w.stamps.as<unsigned long long>(), grid(), ctl());
+            CPM_HIP(hipGetLastError());
+            CPM_CHECK(read_ctl(hc));
+        }
         const uint64_t n = hc->n_patterns;
         cpm_patterns *p = new cpm_patterns();
         p->n = n;
@@ -1320,9 +1428,8 @@ struct Search {
                 return sc;
             }
             unsigned char *h = w.down.as<unsigned char>();
-            hipError_t e = hipMemcpyAsync(h, w.pat_parent.p, n * 4, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipMemcpyAsync(h + n * 4, w.pat_item.p, n * 4, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipMemcpyAsync(h + n * 8, w.pat_sup.p, n * 4, hipMemcpyDeviceToHost, st);
+            // parent, item and support columns in one strided copy
+            hipError_t e = hipMemcpy2DAsync(h, n * 4, w.pat.p, pat_pitch(), n * 4, 3, hipMemcpyDeviceToHost, st);
             if (e == hipSuccess && periodic)
                 e = hipMemcpyAsync(h + n * 12, w.pat_per.p, n * 8, hipMemcpyDeviceToHost, st);
             if (e == hipSuccess) e = hipStreamSynchronize(st);
