@@ -153,9 +153,10 @@ def main():
                 "bytes_per_launch": bytes_per_launch, "launches_per_step": launches / args.steps}
     gram_ms = stats["gram_ms"] / max(stats["gram_launches"], 1)
     gram_ops = stats["gram_ops"] / max(stats["gram_launches"], 1)
-    gram = {"kernel": "k_gram_i8", "avg_launch_ms": round(gram_ms, 4),
+    # dense FP4 MFMA peak of the MI355X (the Gram's operands are E2M1 0 / 1)
+    gram = {"kernel": "k_gram", "dtype": "fp4", "avg_launch_ms": round(gram_ms, 4),
             "tops": round(gram_ops / (gram_ms / 1e3) / 1e12, 2) if gram_ms > 0 else 0.0,
-            "peak_tops": 5000.0}
+            "peak_tops": 10100.0}
 
     cpu = None
     parity = None

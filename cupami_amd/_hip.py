@@ -53,7 +53,7 @@ SIGNATURES = {
     "cpm_intersect_extensions": (ctypes.c_int, [_vp, _u64p, _u64p, ctypes.c_uint32,
                                                 ctypes.c_uint64, ctypes.c_uint64, _u64p, _u64p,
                                                 _u32p, _u32p]),
-    "cpm_pair_gram_i8": (ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint32, _u32p]),
+    "cpm_pair_gram": (ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint32, _u32p]),
     "cpm_periodic_eval": (ctypes.c_int, [_vp, _vp, _i64p, ctypes.c_int64, _u32p, ctypes.c_uint64,
                                          ctypes.c_uint32, _u64p, _i64p]),
     "cpm_eclat_mine": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_int,
@@ -258,7 +258,7 @@ class Context:
         n = cols.size
         out = np.zeros((n, n), dtype=np.uint32)
         if n:
-            check(load().cpm_pair_gram_i8(self.handle, bits.handle, _p(cols, _u32p), n,
+            check(load().cpm_pair_gram(self.handle, bits.handle, _p(cols, _u32p), n,
                                           _p(out, _u32p)))
         return out
 

@@ -210,7 +210,7 @@ struct GramJob {
 int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<GramJob> &jobs,
               uint64_t *out_elems);
 // Zero `out` and run the jobs (device tables); n_ops feeds the statistics.
-int launch_gram_i8(cpm_ctx *ctx, const GramJob *d_jobs, uint64_t n_jobs, const GramDb *d_dbs,
+int launch_gram(cpm_ctx *ctx, const GramJob *d_jobs, uint64_t n_jobs, const GramDb *d_dbs,
                    int32_t *d_out, uint64_t out_elems, uint64_t n_ops);
 
 }  // namespace cpm

@@ -1,26 +1,29 @@
-// Level-2 pair supports as a dense contraction on the INT8 matrix cores.
+// Level-2 pair supports as a dense contraction on the matrix cores.
 //
 // With D the N x F 0/1 matrix of the F item rows, the pair supports are
 // G = D^T D: G[a][b] = |row_a & row_b|.  Each 64-bit word of a row is 64
 // consecutive K elements.  A workgroup owns a kBT x kBT block of G (upper
 // block triangle only) and one slice of the K (transaction) axis; it stages
 // kKW-word tiles of its 2 x kBT rows in LDS, and each of its four waves
-// accumulates a (kBT/2)^2 quadrant with v_mfma_i32_16x16x64_i8, kT x kT tiles
-// of 16 x 16.  The bits become int8 0/1 in registers: lane group g (lane / 16)
-// takes 16 bits of each word and spreads them into 16 bytes with
-// shift-and-mask; A (row items) and B (column items) use the same lane -> bit
-// map, so every bit is paired with itself exactly once whatever order the
-// hardware gives the K slots.
+// accumulates a (kBT/2)^2 quadrant with v_mfma_scale_f32_16x16x128_f8f6f4 in
+// its FP4 (E2M1) form, kT x kT tiles of 16 x 16.  0 and 1 are exact E2M1
+// values (1.0 is the nibble 0b0010) and every product is 0 or 1, so the FP32
+// accumulators hold exact integers while a slice spans fewer than 2^24
+// transactions; the planner cuts K so that it does.  The FP4 form runs at
+// twice the INT8 rate and takes 128 K elements (two words) per instruction.
 //
-// The spread is VALU work proportional to the rows a wave touches (2 kT x 16
-// per word) while the MFMA work grows with their product (kT^2 tiles): at
-// kT = 8 a word costs 16 spreads for 64 MFMAs, so the matrix cores, not the
-// VALU, set the pace (at kT = 4 the VALU did).  The 256 accumulator registers
-// leave one wave per SIMD; the next K tile is loaded into registers while the
-// current one is multiplied out of LDS.  Accumulation is int32: exact while
-// N < 2^31.  Slices are summed with integer atomics.  One launch serves any
-// number of databases: the job table lists each workgroup's (database, block
-// row, block column, K range).
+// The bits become nibbles in registers: lane group g (lane / 16) takes 32
+// bits of each word pair (word 2s + g / 2, half g % 2) and spreads them into
+// 16 bytes with shift-and-mask; A (row items) and B (column items) use the
+// same lane -> bit map, so every bit is paired with itself exactly once
+// whatever order the hardware gives the K slots.  The spread is VALU work
+// proportional to the rows a wave touches (2 kT x 16 per word pair) while
+// the MFMA work grows with their product (kT^2 tiles), so at kT = 8 the
+// matrix cores set the pace.  The 256 accumulator registers leave one wave
+// per SIMD; the next K tile is loaded into registers while the current one is
+// multiplied out of LDS.  Slices are summed with integer atomics.  One launch
+// serves any number of databases: the job table lists each workgroup's
+// (database, block row, block column, K range).
 #include "cpm_internal.h"
 
 #include <algorithm>
@@ -30,30 +33,30 @@ namespace cpm {
 namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int kT = 8;            // 16 x 16 tiles per wave side
-constexpr int kBT = 32 * kT;     // block tile (items): 2 x 2 waves
-constexpr int kKW = 8;           // words per K tile
-constexpr int kTPR = kKW / 8;    // loader threads per row (8 words each)
-static_assert(kBT * kTPR == 256, "one 256-lane block stages the block's rows");
+constexpr int kT = 8;                 // 16 x 16 tiles per wave side
+constexpr int kBT = 32 * kT;          // block tile (items): 2 x 2 waves
+constexpr int kKW = 8;                // words per K tile
+constexpr int kSteps = kKW / 2;       // MFMA steps (word pairs) per K tile
+constexpr int kLdsW = 2 * kKW + 4;    // u32 per LDS row: the tile and one step of read-ahead
+constexpr uint32_t kE2M1One = 0x22222222u;  // bit 1 of each nibble: 1.0 in E2M1
+constexpr int kFmtFp4 = 4;            // f8f6f4 operand format code of E2M1
+constexpr int kScaleOne = 0x7f7f7f7f; // E8M0 block scales of 2^0
+static_assert(kBT == 256, "one 256-lane block stages the block's rows");
+static_assert(kKW % 2 == 0, "K tiles hold whole word pairs");
 
-// 16 bits of `w` -> 16 int8 0/1 values for lane group g (0..3).
-__device__ __forceinline__ v4i spread_bits(uint64_t w, uint32_t g) {
-    const uint32_t half = (g & 2) ? (uint32_t)(w >> 32) : (uint32_t)w;
-    const uint32_t q = (g & 1) * 4;
-    v4i r;
-    r.x = (int)((half >> (q + 0)) & 0x01010101u);
-    r.y = (int)((half >> (q + 1)) & 0x01010101u);
-    r.z = (int)((half >> (q + 2)) & 0x01010101u);
-    r.w = (int)((half >> (q + 3)) & 0x01010101u);
-    return r;
+// 32 bits -> 32 E2M1 values 0 / 1.0: bit 4j + k lands in nibble j of dword k.
+__device__ __forceinline__ v8i spread_fp4(uint32_t x) {
+    return (v8i){(int)((x << 1) & kE2M1One), (int)(x & kE2M1One), (int)((x >> 1) & kE2M1One),
+                 (int)((x >> 2) & kE2M1One), 0, 0, 0, 0};
 }
 
-__global__ __launch_bounds__(256) void k_gram_i8(const GramJob *__restrict__ jobs,
-                                                const GramDb *__restrict__ dbs,
-                                                int32_t *__restrict__ out) {
-    __shared__ uint64_t sA[kBT][kKW + 1];
-    __shared__ uint64_t sB[kBT][kKW + 1];
+__global__ __launch_bounds__(256) void k_gram(const GramJob *__restrict__ jobs, const GramDb *__restrict__ dbs,
+                                              int32_t *__restrict__ out) {
+    __shared__ uint32_t sA[kBT][kLdsW];
+    __shared__ uint32_t sB[kBT][kLdsW];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const GramJob job = jobs[blockIdx.x];
     const GramDb db = dbs[job.db];
@@ -66,16 +69,15 @@ __global__ __launch_bounds__(256) void k_gram_i8(const GramJob *__restrict__ job
     int32_t *res = out + db.out_off;
     const uint32_t ld = n;
 
-    // loader: thread (row lrow, word group lw) stages 8 words of A and of B
-    const uint32_t lrow = tid / kTPR, lw = (tid % kTPR) * 8;
-    const uint32_t ia = bi * kBT + lrow, ib = bj * kBT + lrow;
+    // loader: thread tid stages the kKW words of A row tid and B row tid
+    const uint32_t ia = bi * kBT + tid, ib = bj * kBT + tid;
     const gptr<uint4> pa = ia < n ? (gptr<uint4>)(store + (uint64_t)cols[ia] * stride) : nullptr;
     const gptr<uint4> pb = ib < n ? (gptr<uint4>)(store + (uint64_t)cols[ib] * stride) : nullptr;
-    uint4 ra[4], rb[4];
+    uint4 ra[kKW / 2], rb[kKW / 2];
     auto fetch = [&](uint64_t k0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint64_t w = k0 + lw + 2 * q;  // even; w + 1 < stride (zero padded)
+        for (int q = 0; q < kKW / 2; ++q) {
+            const uint64_t w = k0 + 2 * q;  // even; w + 1 < stride (zero padded)
             const bool in = w < k_end;
             ra[q] = make_uint4(0, 0, 0, 0);
             rb[q] = make_uint4(0, 0, 0, 0);
@@ -85,11 +87,9 @@ __global__ __launch_bounds__(256) void k_gram_i8(const GramJob *__restrict__ job
     };
     auto stash = [&]() {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            sA[lrow][lw + 2 * q] = (uint64_t)ra[q].x | ((uint64_t)ra[q].y << 32);
-            sA[lrow][lw + 2 * q + 1] = (uint64_t)ra[q].z | ((uint64_t)ra[q].w << 32);
-            sB[lrow][lw + 2 * q] = (uint64_t)rb[q].x | ((uint64_t)rb[q].y << 32);
-            sB[lrow][lw + 2 * q + 1] = (uint64_t)rb[q].z | ((uint64_t)rb[q].w << 32);
+        for (int q = 0; q < kKW / 2; ++q) {
+            *reinterpret_cast<uint4 *>(&sA[tid][4 * q]) = ra[q];
+            *reinterpret_cast<uint4 *>(&sB[tid][4 * q]) = rb[q];
         }
     };
 
@@ -97,47 +97,49 @@ __global__ __launch_bounds__(256) void k_gram_i8(const GramJob *__restrict__ job
     const uint32_t li = lane & 15, lg = lane >> 4;
     // on a diagonal block the lower-left quadrant lies below the diagonal
     const bool idle = (bi == bj) && (wr > wc);
-    v4i acc[kT][kT];
+    v4f acc[kT][kT];
 #pragma unroll
     for (int x = 0; x < kT; ++x)
 #pragma unroll
-        for (int y = 0; y < kT; ++y) acc[x][y] = (v4i){0, 0, 0, 0};
+        for (int y = 0; y < kT; ++y) acc[x][y] = (v4f){0.f, 0.f, 0.f, 0.f};
 
+    const uint32_t *rowA = &sA[wr * 16 * kT + li][lg];
+    const uint32_t *rowB = &sB[wc * 16 * kT + li][lg];
     if (k_begin < k_end) fetch(k_begin);
     for (uint64_t k0 = k_begin; k0 < k_end; k0 += kKW) {
         stash();
         __syncthreads();
         if (k0 + kKW < k_end) fetch(k0 + kKW);
         if (!idle) {
-            // Software pipeline over the tile's words: the next word's LDS
-            // reads are issued before this word's MFMAs, and its B spreads are
-            // computed between this word's MFMA rows (one per row), so the
-            // VALU work hides under the matrix cores.  Column kKW of the LDS
-            // tiles is padding: the reads ahead of the last word stay in
-            // bounds and go unused.
-            uint64_t ca[kT], nb[kT];
-            v4i fb[kT];
+            // Software pipeline over the tile's word pairs: the next pair's
+            // LDS reads are issued before this pair's MFMAs, and its B spreads
+            // are computed between this pair's MFMA rows (one per row), so the
+            // VALU work hides under the matrix cores.  The last four u32 of an
+            // LDS row are read ahead of the last pair and go unused.
+            uint32_t ca[kT], nb[kT];
+            v8i fb[kT];
 #pragma unroll
             for (int t = 0; t < kT; ++t) {
-                ca[t] = sA[wr * 16 * kT + t * 16 + li][0];
-                fb[t] = spread_bits(sB[wc * 16 * kT + t * 16 + li][0], lg);
+                ca[t] = rowA[t * 16 * kLdsW];
+                fb[t] = spread_fp4(rowB[t * 16 * kLdsW]);
             }
 #pragma unroll 1
-            for (int w = 0; w < kKW; ++w) {
-                uint64_t na[kT];
+            for (int s = 0; s < kSteps; ++s) {
+                uint32_t na[kT];
 #pragma unroll
                 for (int t = 0; t < kT; ++t) {
-                    na[t] = sA[wr * 16 * kT + t * 16 + li][w + 1];
-                    nb[t] = sB[wc * 16 * kT + t * 16 + li][w + 1];
+                    na[t] = rowA[t * 16 * kLdsW + 4 * (s + 1)];
+                    nb[t] = rowB[t * 16 * kLdsW + 4 * (s + 1)];
                 }
-                v4i gb[kT];
+                v8i gb[kT];
 #pragma unroll
                 for (int x = 0; x < kT; ++x) {
-                    const v4i fa = spread_bits(ca[x], lg);
+                    const v8i fa = spread_fp4(ca[x]);
 #pragma unroll
                     for (int y = 0; y < kT; ++y)
-                        acc[x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa, fb[y], acc[x][y], 0, 0, 0);
-                    gb[x] = spread_bits(nb[x], lg);
+                        acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                            fa, fb[y], acc[x][y], kFmtFp4, kFmtFp4, 0, kScaleOne, 0, kScaleOne);
+                    gb[x] = spread_fp4(nb[x]);
                 }
 #pragma unroll
                 for (int t = 0; t < kT; ++t) {
@@ -158,13 +160,16 @@ __global__ __launch_bounds__(256) void k_gram_i8(const GramJob *__restrict__ job
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const uint32_t row = bi * kBT + wr * 16 * kT + x * 16 + lg * 4 + r;
-                const int v = acc[x][y][r];
+                const int v = (int)acc[x][y][r];
                 if (row < n && col < n && row <= col && v) atomicAdd(res + (uint64_t)row * ld + col, v);
             }
         }
 }
 
 }  // namespace
+
+// transactions per K slice: every partial sum is at most this, below 2^24
+constexpr uint64_t kMaxSliceBits = 1ull << 23;
 
 int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<GramJob> &jobs,
               uint64_t *out_elems) {
@@ -189,6 +194,8 @@ int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<Gr
         const uint64_t ktiles = (d.words + kKW - 1) / kKW;
         uint64_t slices = tri_total ? std::max<uint64_t>(1, want / tri_total) : 1;
         slices = std::min<uint64_t>(slices, ktiles);
+        // FP32 sums stay exact below 2^24 transactions per slice
+        slices = std::max<uint64_t>(slices, (ktiles * kKW * 64 + kMaxSliceBits - 1) / kMaxSliceBits);
         const uint64_t slice_words = (ktiles + slices - 1) / slices * kKW;
         for (uint32_t bi = 0; bi < nb; ++bi)
             for (uint32_t bj = bi; bj < nb; ++bj)
@@ -199,12 +206,12 @@ int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<Gr
     return CPM_OK;
 }
 
-int launch_gram_i8(cpm_ctx *ctx, const GramJob *d_jobs, uint64_t n_jobs, const GramDb *d_dbs,
+int launch_gram(cpm_ctx *ctx, const GramJob *d_jobs, uint64_t n_jobs, const GramDb *d_dbs,
                    int32_t *d_out, uint64_t out_elems, uint64_t n_ops) {
     CPM_HIP(hipMemsetAsync(d_out, 0, out_elems * sizeof(int32_t), ctx->stream));
     if (n_jobs == 0) return CPM_OK;
     Timed timer(ctx, 5);
-    hipLaunchKernelGGL(k_gram_i8, dim3((unsigned)n_jobs), dim3(256), 0, ctx->stream, d_jobs, d_dbs,
+    hipLaunchKernelGGL(k_gram, dim3((unsigned)n_jobs), dim3(256), 0, ctx->stream, d_jobs, d_dbs,
                        d_out);
     CPM_HIP(hipGetLastError());
     CPM_CHECK(timer.finish());
@@ -215,8 +222,8 @@ int launch_gram_i8(cpm_ctx *ctx, const GramJob *d_jobs, uint64_t n_jobs, const G
 
 }  // namespace cpm
 
-extern "C" int cpm_pair_gram_i8(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t *cols,
-                                uint32_t n, uint32_t *gram) {
+extern "C" int cpm_pair_gram(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t *cols, uint32_t n,
+                             uint32_t *gram) {
     using namespace cpm;
     clear_error();
     CPM_CHECK(bind(ctx));
@@ -249,7 +256,7 @@ extern "C" int cpm_pair_gram_i8(cpm_ctx *ctx, const cpm_bits *bits, const uint32
                                hipMemcpyHostToDevice, ctx->stream));
     CPM_HIP(hipMemcpyAsync(ddb.p, dbs.data(), sizeof(GramDb), hipMemcpyHostToDevice, ctx->stream));
     const uint64_t ops = (uint64_t)bits->n_transactions * n * (uint64_t)(n + 1);
-    CPM_CHECK(launch_gram_i8(ctx, djobs.as<GramJob>(), jobs.size(), ddb.as<GramDb>(),
+    CPM_CHECK(launch_gram(ctx, djobs.as<GramJob>(), jobs.size(), ddb.as<GramDb>(),
                              dout.as<int32_t>(), elems, ops));
     CPM_CHECK(copy_to_host(ctx, gram, dout.p, (size_t)n * n * 4));
     return CPM_OK;

@@ -1312,7 +1312,7 @@ struct Search {
             qput(w.gjobs.p, jobs.data(), jobs.size() * sizeof(GramJob));
             qput(w.gdbs.p, gdb.data(), n_db * sizeof(GramDb));
             CPM_CHECK(flush());
-            CPM_CHECK(launch_gram_i8(ctx, w.gjobs.as<GramJob>(), jobs.size(), w.gdbs.as<GramDb>(),
+            CPM_CHECK(launch_gram(ctx, w.gjobs.as<GramJob>(), jobs.size(), w.gdbs.as<GramDb>(),
                                      w.gram.as<int32_t>(), elems, ops));
             hipLaunchKernelGGL(k_gram_to_level2, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0,
                                st, S1.ptrs(sz(1)), w.dbs.as<DbDev>(), w.gram.as<int32_t>(),

@@ -105,12 +105,14 @@ int cpm_intersect_extensions(cpm_ctx *ctx, const uint64_t *prefix, const uint64_
                              uint64_t *supports, uint64_t *kept_rows, uint32_t *kept_index,
                              uint32_t *n_kept);
 
-/* Level-2 pair supports as an INT8 D^T D on MFMA (exact int32 accumulation):
+/* Level-2 pair supports as a D^T D on the matrix cores (FP4 MFMA over 0/1
+ * operands, exact: partial sums stay below 2^24 and are summed in int32):
  * gram[a*n + b] = |rows(cols[a]) & rows(cols[b])| for a <= b; entries below
  * the diagonal are left 0.  n <= 16384, n_transactions < 2^31.  Host output
- * of n*n uint32. */
-int cpm_pair_gram_i8(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t *cols, uint32_t n,
-                     uint32_t *gram);
+ * of n*n uint32.  Replaces the level-2 count_supports over all pairs
+ * (cuEclat.py:184-193). */
+int cpm_pair_gram(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t *cols, uint32_t n,
+                  uint32_t *gram);
 
 /* Replaces TemporalInput._evaluate's periodic_counts (_temporal.py:28-55,
  * 151-161): support and periodicity (largest gap of [0, ts of each
@@ -126,7 +128,7 @@ int cpm_periodic_eval(cpm_ctx *ctx, const cpm_bits *bits, const int64_t *timesta
  * cuApriori.py:501-535): every itemset whose support >= minsup, found by a
  * level-synchronous search whose candidate generation, pruning and support
  * counting all run on the device.  Level-2 supports come from
- * cpm_pair_gram_i8 when use_gram != 0. */
+ * cpm_pair_gram when use_gram != 0. */
 int cpm_eclat_mine(cpm_ctx *ctx, const cpm_bits *bits, uint64_t minsup, int use_gram,
                    cpm_patterns **out);
 
