@@ -41,7 +41,7 @@ constexpr uint32_t kMaskWords = 2048;   // LDS kept mask: up to 65536 frequent i
 constexpr uint64_t kTaskMaskBudget = 256ull << 20;  // bytes of per-task kept masks
 
 enum Overflow : uint32_t {
-    kOvTasks = 1, kOvEntries = 2, kOvPatterns = 4, kOvDepth = 8, kOvLevelSets = 16, kOvParts = 32,
+    kOvTasks = 1, kOvEntries = 2, kOvPatterns = 4, kOvDepth = 8, kOvParts = 32,
     kOvIndex = 64  // a level with 2^32 or more entries
 };
 
@@ -57,7 +57,7 @@ struct Ctl {
 };
 
 struct Caps {
-    uint64_t tasks, ext, patterns, lv, parts;
+    uint64_t tasks, ext, patterns, parts;
 };
 
 struct PlanArgs {
@@ -66,7 +66,7 @@ struct PlanArgs {
     const uint32_t *fcol;       // null at level 1 (entries are columns)
     const uint32_t *pairbits;   // pair tables, null for none
     uint32_t *kept, *cnt_pat, *cnt_child, *cnt_cext, *off_pat, *off_child, *off_cext;
-    uint32_t *lvitems;          // this level's itemsets for the subset test
+    unsigned long long *filter; // Bloom filter of this level's itemsets for the subset test
     uint32_t *off2, *ext2, *live, *pos;  // next level's staging for the subset test
     unsigned long long *part;   // plan/count chunk sums: 3 x (plan grid)
     unsigned long long *partl;  // prune/flags chunk survivor counts: (prune grid)
@@ -314,7 +314,6 @@ __device__ void alloc_next(const PlanArgs &p) {
     if (n_cext >= 0xffffffffull || n_child >= 0xffffffffull) ov |= kOvIndex;
     if (c->n_patterns + n_pat > p.caps.patterns) ov |= kOvPatterns;
     if (n_child && d1 > kMaxDepth) ov |= kOvDepth;
-    if (p.prune && n_child && L->d >= 2 && n_pat * d1 > p.caps.lv) ov |= kOvLevelSets;
     if (p.periodic && n_cext * p.parts_tiles > p.caps.parts) ov |= kOvParts;
     if (ov) {
         c->overflow |= ov;
@@ -384,6 +383,44 @@ __device__ __forceinline__ uint32_t upper_task(const uint32_t *__restrict__ off,
 }
 
 // subset test (cuApriori.py:487-490) ----------------------------------------------
+//
+// The itemsets of the level being planned go into a blocked Bloom filter:
+// a 64-bit hash of the itemset picks one 64-bit word and four bits in it,
+// set with one atomicOr (nothing waits for it).  The filter has 32 bits per
+// itemset (a power of two), which keeps false positives near 0.1%.  A child
+// candidate survives when all its subsets test present.  A false positive can
+// only keep a candidate that should have been pruned; its support, counted
+// exactly, is then below minsup (it is at most the missing subset's), so
+// results never change.
+
+// finalizer of splitmix64 (Steele, Lea and Flood, 2014)
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+constexpr uint64_t kSetSeed = 0x243f6a8885a308d3ull;
+__device__ __forceinline__ uint64_t set_step(uint64_t h, uint32_t item) {
+    return mix64(h ^ (item + 0x9e3779b97f4a7c15ull));
+}
+// words of the filter for n_pat itemsets, less one: next_pow2(32 n_pat) bits
+__device__ __forceinline__ uint64_t filter_mask(uint64_t n_pat) {
+    const uint64_t bits = n_pat <= 2 ? 64 : 1ull << (64 - __clzll(32 * n_pat - 1));
+    return bits / 64 - 1;
+}
+// word index from the low bits, four bit positions from the top 24
+__device__ __forceinline__ unsigned long long filter_bits(uint64_t h) {
+    return (1ull << ((h >> 40) & 63)) | (1ull << ((h >> 46) & 63)) | (1ull << ((h >> 52) & 63)) |
+           (1ull << (h >> 58));
+}
+__device__ __forceinline__ void filter_insert(unsigned long long *f, uint64_t mask, uint64_t h) {
+    atomicOr(f + (h & mask), filter_bits(h));
+}
+__device__ __forceinline__ bool filter_test(const unsigned long long *f, uint64_t mask, uint64_t h) {
+    const unsigned long long b = filter_bits(h);
+    return (f[h & mask] & b) == b;
+}
+
 __device__ __forceinline__ bool prune_active(const PlanArgs &p) {
     return p.prune && !p.ctl->overflow && p.emit_children && p.L.sz->d >= 2 && p.N.sz->n_tasks > 0;
 }
@@ -400,9 +437,10 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
     const uint64_t pat0 = p.ctl->pat_base;
     const bool children = p.emit_children && p.N.sz->n_tasks > 0;
     // With the subset test, the children's entries go to staging and this
-    // level's itemsets feed the test (the allocator sized lvitems for exactly
-    // this case); without it, straight into the next level.
+    // level's itemsets go into the filter; without it, straight into the
+    // next level.
     const bool stage = prune_active(p);
+    const uint64_t fmask = filter_mask(p.L.sz->n_pat);
     uint32_t *const xdst = stage ? p.ext2 : p.N.ext;
     uint32_t *const odst = stage ? p.off2 : p.N.ext_off;
     for (uint32_t w = tid; w < kMaskWords; w += blockDim.x) lmask[w] = 0;
@@ -423,9 +461,9 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             p.pat_sup[pb + j] = p.L.sup[e];
             if (p.pat_per) p.pat_per[pb + j] = p.L.per[e];
             if (stage) {
-                uint32_t *dst = p.lvitems + (uint64_t)(p.off_pat[t] + j) * d1;
-                for (uint32_t i = 0; i < d; ++i) dst[i] = src[i];
-                dst[d] = f;
+                uint64_t h = kSetSeed;
+                for (uint32_t i = 0; i < d; ++i) h = set_step(h, src[i]);
+                filter_insert(p.filter, fmask, set_step(h, f));
             }
         }
         if (!(children && nk >= 2)) continue;
@@ -556,32 +594,13 @@ __global__ __launch_bounds__(256) void k_child_fill(PlanArgs p) {
 }
 
 // Is (prefix without item `drop`) + {x} -- d1 items, ascending -- in lv?
-__device__ __forceinline__ bool lv_contains(const uint32_t *__restrict__ lv, uint64_t n_lv, uint32_t d1,
-                                            const uint32_t *__restrict__ pre, uint32_t drop,
-                                            uint32_t x) {
-    uint64_t lo = 0, hi = n_lv;
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        const uint32_t *m = lv + mid * d1;
-        int c = 0;
-        for (uint32_t i = 0; i < d1 && c == 0; ++i) {
-            const uint32_t k = i < drop ? pre[i] : (i + 1 < d1 ? pre[i + 1] : x);
-            c = m[i] < k ? -1 : (m[i] > k ? 1 : 0);
-        }
-        if (c == 0) return true;
-        if (c < 0) lo = mid + 1; else hi = mid;
-    }
-    return false;
-}
-
-// prune/flags: a contiguous chunk of staged candidates per block ---------------------
 __global__ __launch_bounds__(256) void k_prune_flags(PlanArgs p) {
     __shared__ unsigned long long red[4];
     if (!prune_active(p)) return;
     const uint32_t n_child = (uint32_t)p.N.sz->n_tasks;
     const uint64_t n_cand = p.N.sz->n_cand;
     const uint32_t d1 = (uint32_t)p.N.sz->d;  // prefix length of the children
-    const uint64_t n_lv = p.L.sz->n_pat;
+    const uint64_t fmask = filter_mask(p.L.sz->n_pat);
     uint64_t lo, hi;
     chunk_of(n_cand, blockIdx.x, gridDim.x, &lo, &hi);
     unsigned long long c = 0;
@@ -592,7 +611,12 @@ __global__ __launch_bounds__(256) void k_prune_flags(PlanArgs p) {
         bool ok = true;
         // the subsets without the last prefix item or without x are kept by
         // construction; test the ones without each earlier prefix item
-        for (uint32_t drop = 0; ok && drop + 1 < d1; ++drop) ok = lv_contains(p.lvitems, n_lv, d1, pre, drop, x);
+        for (uint32_t drop = 0; ok && drop + 1 < d1; ++drop) {
+            uint64_t h = kSetSeed;
+            for (uint32_t i = 0; i < d1; ++i)
+                if (i != drop) h = set_step(h, pre[i]);
+            ok = filter_test(p.filter, fmask, set_step(h, x));
+        }
         p.live[w] = ok ? 1u : 0u;
         c += ok ? 1u : 0u;
     }
@@ -640,6 +664,12 @@ __global__ __launch_bounds__(256) void k_prune_compact(PlanArgs p) {
         p.N.sz->group = level_group(base, p.G, p.te, p.work_target);
         p.N.sz->n_work = level_work(base, (uint32_t)p.N.sz->group, p.te);
     }
+    // k_prune_flags is done with the filter: clear the words this level used
+    // for the next one
+    const uint64_t fwords = filter_mask(p.L.sz->n_pat) + 1;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < fwords;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p.filter[i] = 0;
 }
 
 // ---- level 1 and level 2 setup -----------------------------------------------------
@@ -776,7 +806,7 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
 
 struct LevelSet {
     DBuf prefix, pid, db, ext_off, ext, sup, per;
-    DBuf kept, cnt[3], off[3], off2, ext2, live, pos, lvitems, parts;
+    DBuf kept, cnt[3], off[3], off2, ext2, live, pos, parts;
     LevelPtrs ptrs(LevelSizes *sz) const {
         LevelPtrs p;
         p.prefix = prefix.as<uint32_t>();
@@ -806,7 +836,6 @@ struct LevelSet {
         CPM_CHECK(kept.ensure(e1 * 4));
         CPM_CHECK(live.ensure(e1 * 4));
         CPM_CHECK(pos.ensure(e1 * 4));
-        CPM_CHECK(lvitems.ensure((c.lv + 1) * 4));
         if (periodic) {
             CPM_CHECK(per.ensure(e1 * 8));
             CPM_CHECK(parts.ensure((c.parts + 1) * sizeof(PeriodPart)));
@@ -818,7 +847,7 @@ struct LevelSet {
 
 struct GraphKey {
     const void *ptrs[64];
-    uint64_t caps[5];
+    uint64_t caps[4];
     uint32_t G, te, tmask_w, tile_words, parts_tiles;
     int periodic, prune;
     bool operator==(const GraphKey &o) const { return memcmp(this, &o, sizeof(GraphKey)) == 0; }
@@ -828,6 +857,8 @@ struct SearchWs {
     LevelSet set[2];
     DBuf ctl, dbs, fcol, pairbits, tmask, gram, goff, gjobs, gdbs, l2, part, partl, scatter, ts, stamps;
     DBuf pat;             // pattern forest: parent, item and support columns, pat_pitch() bytes apart
+    DBuf filter;          // subset-test Bloom filter (filter_words() words)
+    bool filter_dirty = true;  // holds bits of an attempt that stopped early
     DBuf pat_per;
     HBuf host;
     HBuf stage;           // pinned staging of small host <-> device copies
@@ -878,6 +909,13 @@ struct Search {
     Search(cpm_ctx *c, SearchWs &ws) : ctx(c), w(ws) {}
 
     Ctl *ctl() const { return w.ctl.as<Ctl>(); }
+    // words of the subset-test filter: 32 bits per itemset of the pattern
+    // capacity, rounded up to a power of two (a level holds at most that many)
+    uint64_t filter_words() const {
+        uint64_t bits = 64;
+        while (bits < 32 * w.caps.patterns) bits *= 2;
+        return bits / 64;
+    }
     size_t pat_pitch() const { return ((w.caps.patterns + 1) * 4 + 255) / 256 * 256; }
     uint32_t *pat_col(int k) const {
         return reinterpret_cast<uint32_t *>(w.pat.as<unsigned char>() + (size_t)k * pat_pitch());
@@ -900,7 +938,7 @@ struct Search {
         p.off_pat = L.off[0].as<uint32_t>();
         p.off_child = L.off[1].as<uint32_t>();
         p.off_cext = L.off[2].as<uint32_t>();
-        p.lvitems = L.lvitems.as<uint32_t>();
+        p.filter = w.filter.as<unsigned long long>();
         p.off2 = N.off2.as<uint32_t>();
         p.ext2 = N.ext2.as<uint32_t>();
         p.live = N.live.as<uint32_t>();
@@ -979,17 +1017,16 @@ struct Search {
         for (int j = 0; j < 2; ++j) {
             const LevelSet &S = w.set[(c + j) & 1];
             add(S.prefix); add(S.pid); add(S.db); add(S.ext_off); add(S.ext); add(S.sup); add(S.per); add(S.kept); add(S.off2); add(S.ext2); add(S.live);
-            add(S.pos); add(S.lvitems); add(S.parts);
+            add(S.pos); add(S.parts);
             for (int q = 0; q < 3; ++q) { add(S.cnt[q]); add(S.off[q]); }
         }
         add(w.ctl); add(w.dbs); add(w.fcol); add(w.pairbits); add(w.tmask); add(w.part); add(w.partl);
         add(w.stamps);
-        add(w.pat); add(w.pat_per);
+        add(w.pat); add(w.pat_per); add(w.filter);
         k.caps[0] = w.caps.tasks;
         k.caps[1] = w.caps.ext;
         k.caps[2] = w.caps.patterns;
-        k.caps[3] = w.caps.lv;
-        k.caps[4] = w.caps.parts;
+        k.caps[3] = w.caps.parts;
         k.G = G;
         k.te = te;
         k.tmask_w = tmask_w;
@@ -1033,6 +1070,11 @@ struct Search {
     int reserve() {
         for (int i = 0; i < 2; ++i) CPM_CHECK(w.set[i].reserve(w.caps, periodic, parts_tiles));
         CPM_CHECK(w.pat.ensure(3 * pat_pitch()));
+        {
+            void *before = w.filter.p;
+            CPM_CHECK(w.filter.ensure(filter_words() * 8));
+            if (w.filter.p != before) w.filter_dirty = true;
+        }
         if (periodic) CPM_CHECK(w.pat_per.ensure((w.caps.patterns + 1) * 8));
         CPM_CHECK(w.part.ensure((size_t)3 * grid_tasks() * 8));
         CPM_CHECK(w.partl.ensure((size_t)grid() * 8));
@@ -1135,6 +1177,10 @@ struct Search {
         ctl_current = false;
         // stamps left by an attempt that overflowed
         qzero(w.stamps.p, (size_t)grid() * 4 * 8);
+        if (w.filter_dirty) {
+            qzero(w.filter.p, filter_words() * 8);
+            w.filter_dirty = false;
+        }
 
         // database table (frequent ranges filled after level 1)
         dbs.assign(n_db, DbDev{});
@@ -1378,16 +1424,17 @@ struct Search {
         want.tasks = std::max<uint64_t>(std::max<uint64_t>(items, in.size()), 1 << 16);
         want.ext = std::max<uint64_t>(std::max<uint64_t>(e2, items), 1 << 18);
         want.patterns = std::max<uint64_t>(items * 4, 1 << 18);
-        want.lv = std::max<uint64_t>(want.ext * 2, 1 << 18);
         want.parts = periodic ? want.ext * std::max<uint32_t>(parts_tiles, 1) : 0;
-        for (int i = 0; i < 5; ++i) {
+        for (int i = 0; i < 4; ++i) {
             uint64_t *a = &w.caps.tasks + i;
             const uint64_t *b = &want.tasks + i;
             if (*a < *b) *a = *b;
         }
         for (int tries = 0;; ++tries) {
             uint32_t ov = 0;
-            CPM_CHECK(attempt(&ov));
+            const int rc = attempt(&ov);
+            if (rc != CPM_OK || ov) w.filter_dirty = true;  // stopped between insert and clear
+            CPM_CHECK(rc);
             if (!ov) break;
             if (tries > 24) return fail(CPM_ERR_NOMEM, "search buffers keep overflowing (%u)", ov);
             if (ov & kOvDepth) return fail(CPM_ERR_NOMEM, "an itemset longer than %u items", kMaxDepth + 1);
@@ -1395,7 +1442,6 @@ struct Search {
             if (ov & kOvTasks) w.caps.tasks *= 2;
             if (ov & kOvEntries) w.caps.ext *= 2;
             if (ov & kOvPatterns) w.caps.patterns *= 2;
-            if (ov & kOvLevelSets) w.caps.lv *= 2;
             if (ov & kOvParts) w.caps.parts = std::max<uint64_t>(w.caps.parts * 2, 1 << 20);
         }
         return download(out);
