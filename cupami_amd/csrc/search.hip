@@ -167,6 +167,35 @@ __device__ __forceinline__ void chunk_of(uint64_t n, uint32_t b, uint32_t nb, ui
     *hi = min(n, *lo + c);
 }
 
+// Tasks with at most kSmallTask entries are planned by one wave each (no
+// block barriers); larger ones by the whole block.
+constexpr uint32_t kSmallTask = 64;
+
+// orders a wave's LDS writes before its later LDS reads in other lanes
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// exclusive scan over the 64 lanes of a wave; *total gets the wave sum
+__device__ __forceinline__ uint32_t wave_scan(uint32_t v, uint32_t *total) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    *total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+// bit f of a pair-table row: the pair (row item, f) is frequent
+__device__ __forceinline__ uint32_t pair_bit(const uint32_t *prow, uint32_t f) {
+    return (prow[f >> 5] >> (f & 31)) & 1u;
+}
+
 // sum of part[0 .. b), in every lane of the block
 __device__ __forceinline__ unsigned long long sum_before(const unsigned long long *part, uint32_t b,
                                                          unsigned long long *red) {
@@ -241,19 +270,62 @@ __global__ __launch_bounds__(256) void k_fold_stamps(unsigned long long *stamps,
 // the support launch before it)
 __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
     __shared__ uint32_t lds[8];
+    __shared__ unsigned long long red[4];
     __shared__ uint32_t lmask[kMaskWords];
+    __shared__ uint32_t wf[4][kSmallTask];  // per wave: the task's kept items
     if (p.stamps && blockIdx.x == gridDim.x - 1) fold_stamps(p.stamps, p.stamp_blocks, &p.ctl->acc);
     if (p.ctl->overflow) return;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t n_tasks = (uint32_t)p.L.sz->n_tasks;
     uint64_t lo, hi;
     chunk_of(n_tasks, blockIdx.x, gridDim.x, &lo, &hi);
     for (uint32_t w = tid; w < kMaskWords; w += blockDim.x) lmask[w] = 0;
     __syncthreads();
-    unsigned long long s_pat = 0, s_child = 0, s_cext = 0;
-    for (uint32_t t = (uint32_t)lo; t < hi; ++t) {
-        const DbDev &D = p.dbs[p.L.db[t]];
+    unsigned long long s_pat = 0, s_child = 0, s_cext = 0;  // this lane's share of the block sums
+    // small tasks, one wave each: keep flags by ballot, child counts from the
+    // pair table bit by bit
+    for (uint32_t t = (uint32_t)lo + wave; t < hi; t += 4) {
         const uint32_t a = p.L.ext_off[t], b = p.L.ext_off[t + 1];
+        if (b - a > kSmallTask) continue;
+        const DbDev &D = p.dbs[p.L.db[t]];
+        const uint32_t e = a + lane;
+        const bool k = e < b && keep_entry(p, D, e);
+        const unsigned long long bal = __ballot(k);
+        const uint32_t nk = (uint32_t)__popcll(bal);
+        wave_sync();  // the previous task's reads of wf are done
+        if (k) {
+            const uint32_t pos = (uint32_t)__popcll(bal & ((1ull << lane) - 1));
+            p.kept[a + pos] = e;
+            wf[wave][pos] = p.L.ext[e];
+        }
+        wave_sync();
+        const uint32_t nc = (p.emit_children && nk >= 2) ? nk - 1 : 0;
+        uint32_t c = 0;
+        if (lane < nc) {
+            if (p.pairbits) {
+                const uint32_t fj = wf[wave][lane] - D.fbase;
+                const uint32_t *prow = p.pairbits + D.pair_off + (uint64_t)fj * D.pair_w32;
+                for (uint32_t l = lane + 1; l < nk; ++l) c += pair_bit(prow, wf[wave][l] - D.fbase);
+            } else {
+                c = nk - 1 - lane;
+            }
+        }
+        uint32_t cext;
+        (void)wave_scan(c, &cext);
+        if (lane == 0) {
+            p.cnt_pat[t] = nk;
+            p.cnt_child[t] = nc;
+            p.cnt_cext[t] = cext;
+            s_pat += nk;
+            s_child += nc;
+            s_cext += cext;
+        }
+    }
+    // large tasks: the whole block
+    for (uint32_t t = (uint32_t)lo; t < hi; ++t) {
+        const uint32_t a = p.L.ext_off[t], b = p.L.ext_off[t + 1];
+        if (b - a <= kSmallTask) continue;
+        const DbDev &D = p.dbs[p.L.db[t]];
         uint32_t nk = 0;
         for (uint32_t i0 = a; i0 < b; i0 += 256) {
             const uint32_t e = i0 + tid;
@@ -287,12 +359,15 @@ __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
             p.cnt_pat[t] = nk;
             p.cnt_child[t] = nc;
             p.cnt_cext[t] = cext;
+            s_pat += nk;
+            s_child += nc;
+            s_cext += cext;
         }
-        s_pat += nk;
-        s_child += nc;
-        s_cext += cext;
         __syncthreads();
     }
+    s_pat = block_sum64(s_pat, red);
+    s_child = block_sum64(s_child, red);
+    s_cext = block_sum64(s_cext, red);
     if (tid == 0) {
         p.part[blockIdx.x] = s_pat;
         p.part[gridDim.x + blockIdx.x] = s_child;
@@ -429,11 +504,11 @@ __device__ __forceinline__ bool prune_active(const PlanArgs &p) {
 __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
     __shared__ uint32_t lds[8];
     __shared__ uint32_t lmask[kMaskWords];
+    __shared__ uint32_t wf[4][kSmallTask];  // per wave: the task's kept items
     if (p.ctl->overflow) return;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t n_tasks = (uint32_t)p.L.sz->n_tasks;
     const uint32_t d = (uint32_t)p.L.sz->d;
-    const uint32_t d1 = d + 1;
     const uint64_t pat0 = p.ctl->pat_base;
     const bool children = p.emit_children && p.N.sz->n_tasks > 0;
     // With the subset test, the children's entries go to staging and this
@@ -443,13 +518,90 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
     const uint64_t fmask = filter_mask(p.L.sz->n_pat);
     uint32_t *const xdst = stage ? p.ext2 : p.N.ext;
     uint32_t *const odst = stage ? p.off2 : p.N.ext_off;
+    const bool fill = p.tmask && p.pairbits;  // k_child_fill writes the child entries
     for (uint32_t w = tid; w < kMaskWords; w += blockDim.x) lmask[w] = 0;
     __syncthreads();
-    for (uint32_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {
+    // small tasks, one wave each: lane j writes kept entry j and its child
+    for (uint32_t t = blockIdx.x * 4 + wave; t < n_tasks; t += gridDim.x * 4) {
+        const uint32_t nk = p.cnt_pat[t];
+        if (nk > kSmallTask) continue;
         const uint32_t dbi = p.L.db[t];
         const DbDev &D = p.dbs[dbi];
         const uint32_t a = p.L.ext_off[t];
+        const uint64_t pb = pat0 + p.off_pat[t];
+        const uint32_t *src = p.L.prefix + (uint64_t)t * kMaxDepth;
+        uint32_t f = 0;
+        if (lane < nk) {
+            const uint32_t e = p.kept[a + lane];
+            f = p.L.ext[e];
+            p.pat_parent[pb + lane] = p.L.pid[t];
+            p.pat_item[pb + lane] = p.fcol ? p.fcol[f] : f;
+            p.pat_sup[pb + lane] = p.L.sup[e];
+            if (p.pat_per) p.pat_per[pb + lane] = p.L.per[e];
+            if (stage) {
+                uint64_t h = kSetSeed;
+                for (uint32_t i = 0; i < d; ++i) h = set_step(h, src[i]);
+                filter_insert(p.filter, fmask, set_step(h, f));
+            }
+        }
+        if (!(children && nk >= 2)) continue;
+        wave_sync();  // the previous task's reads of wf are done
+        if (lane < nk) wf[wave][lane] = f - D.fbase;
+        wave_sync();
+        if (fill) {
+            // kept-item mask, word q: the run of kept items in that word
+            for (uint32_t q = lane; q < D.pair_w32; q += 64) {
+                uint32_t lo = 0, hi = nk;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if ((wf[wave][mid] >> 5) < q) lo = mid + 1; else hi = mid;
+                }
+                uint32_t m = 0;
+                for (uint32_t l = lo; l < nk && (wf[wave][l] >> 5) == q; ++l) m |= 1u << (wf[wave][l] & 31);
+                p.tmask[(uint64_t)t * p.tmask_w + q] = m;
+            }
+        }
+        const uint32_t *table = p.pairbits ? p.pairbits + D.pair_off : nullptr;
+        const bool live = lane + 1 < nk;
+        const uint32_t fl = f - D.fbase;
+        const uint32_t *prow = table && live ? table + (uint64_t)fl * D.pair_w32 : nullptr;
+        uint32_t cj = 0;
+        if (live) {
+            if (table) {
+                for (uint32_t l = lane + 1; l < nk; ++l) cj += pair_bit(prow, wf[wave][l]);
+            } else {
+                cj = nk - 1 - lane;
+            }
+        }
+        uint32_t tot;
+        const uint32_t off = wave_scan(cj, &tot);
+        if (live) {
+            const uint32_t ct = p.off_child[t] + lane;
+            uint32_t *dst = p.N.prefix + (uint64_t)ct * kMaxDepth;
+            for (uint32_t i = 0; i < d; ++i) dst[i] = src[i];
+            dst[d] = f;
+            p.N.pid[ct] = (uint32_t)(pb + lane);
+            p.N.db[ct] = dbi;
+            uint32_t w = p.off_cext[t] + off;
+            odst[ct] = w;
+            if (!fill) {
+                for (uint32_t l = lane + 1; l < nk; ++l) {
+                    const uint32_t g = wf[wave][l];
+                    if (table && !pair_bit(prow, g)) continue;
+                    xdst[w] = D.fbase + g;
+                    if (!stage) p.N.sup[w] = 0;
+                    ++w;
+                }
+            }
+        }
+    }
+    // large tasks: the whole block
+    for (uint32_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {
         const uint32_t nk = p.cnt_pat[t];
+        if (nk <= kSmallTask) continue;
+        const uint32_t dbi = p.L.db[t];
+        const DbDev &D = p.dbs[dbi];
+        const uint32_t a = p.L.ext_off[t];
         const uint64_t pb = pat0 + p.off_pat[t];
         const uint32_t parent = p.L.pid[t];
         const uint32_t *src = p.L.prefix + (uint64_t)t * kMaxDepth;
@@ -467,7 +619,6 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             }
         }
         if (!(children && nk >= 2)) continue;
-        const bool fill = p.tmask && p.pairbits;  // k_child_fill writes the entries
         if (p.pairbits) {
             mask_set(lmask, p.kept, p.L.ext, a, nk, D.fbase);
             __syncthreads();
