@@ -117,21 +117,26 @@ struct BuildJob {
     uint64_t block0;        // first workgroup of this database in the tile launch
 };
 
-// Is rows[] non-decreasing, and is its last row inside the store?  Raises
-// flags[0] when a descent is found and flags[1] when the last row is
-// >= n_trans (for sorted rows the last is the largest).  blockIdx.y picks
-// the database.
+// Is rows[] non-decreasing, and does every pair lie inside the store?
+// Raises flags[0] when a descent is found, and flags[1] when the last row is
+// >= n_trans (for sorted rows the last is the largest) or a column is >=
+// n_items.  blockIdx.y picks the database.
 __global__ __launch_bounds__(256) void k_rows_sorted(const BuildJob *__restrict__ jobs) {
     const BuildJob &j = jobs[blockIdx.y];
     const uint64_t *__restrict__ rows = j.rows;
+    const uint32_t *__restrict__ cols = j.cols;
     const uint64_t nnz = j.nnz;
     if (blockIdx.x == 0 && threadIdx.x == 0 && nnz && rows[nnz - 1] >= j.n_trans) atomicOr(j.flags + 1, 1u);
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i + 1 < nnz;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        if (rows[i] > rows[i + 1]) {
-            atomicOr(j.flags, 1u);
+    bool bad_col = false;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        bad_col |= cols[i] >= j.n_items;
+        if (i + 1 < nnz && rows[i] > rows[i + 1]) {
+            atomicOr(j.flags, 1u);  // unsorted: the scatter build checks its own pairs
             return;
         }
+    }
+    if (bad_col) atomicOr(j.flags + 1, 1u);
 }
 
 // Store build for row-sorted pairs, any number of databases per launch:
@@ -222,8 +227,6 @@ int cpm_ctx_create(int device, cpm_ctx **out) {
     ctx->device = device;
     ctx->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreate(&ctx->ev_start);
-    if (e == hipSuccess) e = hipEventCreate(&ctx->ev_stop);
     if (e != hipSuccess) {
         cpm_ctx_destroy(ctx);
         return fail(CPM_ERR_RUNTIME, "stream/event creation failed: %s", hipGetErrorString(e));
@@ -243,8 +246,11 @@ int cpm_ctx_destroy(cpm_ctx *ctx) {
     ctx->search_ws = nullptr;
     ctx->spare_store.release();
     ctx->spare_arena.release();
-    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
-    if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    for (auto &t : ctx->ev_pending) {
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    for (hipEvent_t ev : ctx->ev_pool) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CPM_OK;
@@ -290,6 +296,29 @@ static int bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits
 namespace cpm {
 
 constexpr size_t kBounceBytes = 8u << 20;
+
+hipEvent_t take_event(cpm_ctx *ctx) {
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t ev = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    return hipEventCreate(&ev) == hipSuccess ? ev : nullptr;
+}
+
+int collect_timing(cpm_ctx *ctx) {
+    if (ctx->ev_pending.empty()) return CPM_OK;
+    CPM_HIP(hipStreamSynchronize(ctx->stream));
+    for (const auto &t : ctx->ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) ctx->stats[t.slot] += ms;
+        ctx->ev_pool.push_back(t.start);
+        ctx->ev_pool.push_back(t.stop);
+    }
+    ctx->ev_pending.clear();
+    return CPM_OK;
+}
 
 int copy_to_host(cpm_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (bytes == 0) return CPM_OK;
@@ -401,8 +430,10 @@ int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bi
         BuildJob &j = h_jobs[n_check];
         memset(&j, 0, sizeof(j));
         j.rows = in[i].rows;
+        j.cols = in[i].cols;
         j.nnz = in[i].nnz;
         j.n_trans = in[i].n_trans;
+        j.n_items = in[i].n_items;
         j.flags = flags + 3 * i;
         check_of[i] = n_check++;
         max_nnz = std::max(max_nnz, in[i].nnz);
@@ -470,6 +501,7 @@ int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bi
     }
     // 3. the tile builds of every sorted store in one launch; scatter builds
     // (unsorted rows, or too many items for the LDS tile) one per store
+    bool scattered = false;  // a scatter build ran: its range flag must be read
     if (status == CPM_OK) {
         uint32_t nt = 0;
         uint64_t blocks = 0;
@@ -504,18 +536,21 @@ int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bi
             hipLaunchKernelGGL(k_bits_scatter, dim3((unsigned)g), dim3(256), 0, st, in[i].rows, in[i].cols,
                                in[i].nnz, in[i].n_trans, in[i].n_items, out[i]->stride,
                                reinterpret_cast<unsigned long long *>(out[i]->rows()), flags + 3 * i + 2);
+            scattered = true;
         }
     }
     hipError_t e = hipGetLastError();
     if (status == CPM_OK && e != hipSuccess)
         status = fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(e));
-    if (status == CPM_OK) {
+    // tile builds had their range checked in step 1: only scatter builds
+    // report out-of-range pairs afterwards
+    if (status == CPM_OK && scattered) {
         e = hipMemcpyAsync(hflags, flags, (size_t)3 * n * sizeof(unsigned int), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) status = fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(e));
     }
-    for (uint32_t i = 0; i < n && status == CPM_OK; ++i)
-        if (hflags[3 * i + 2])
+    for (uint32_t i = 0; i < n && status == CPM_OK && scattered; ++i)
+        if (!tiled[i] && hflags[3 * i + 2])
             status = fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
                           (unsigned long long)in[i].n_trans, in[i].n_items);
     if (status != CPM_OK) {
@@ -729,12 +764,16 @@ int cpm_patterns_destroy(cpm_patterns *p) {
 
 int cpm_stats(cpm_ctx *ctx, double *out8) {
     if (!ctx || !out8) return fail(CPM_ERR_INVALID, "null argument");
+    CPM_CHECK(bind(ctx));
+    CPM_CHECK(collect_timing(ctx));
     memcpy(out8, ctx->stats, sizeof(ctx->stats));
     return CPM_OK;
 }
 
 int cpm_stats_reset(cpm_ctx *ctx) {
     if (!ctx) return fail(CPM_ERR_INVALID, "null context");
+    CPM_CHECK(bind(ctx));
+    CPM_CHECK(collect_timing(ctx));  // launches before the reset count for nothing
     memset(ctx->stats, 0, sizeof(ctx->stats));
     return CPM_OK;
 }

@@ -70,7 +70,14 @@ struct cpm_ctx {
     int cu_count = 256;
     bool timing = true;
     double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    // event pairs of timed launches not read yet (read by cpm_stats), and
+    // spare events
+    struct TimedPair {
+        int slot;
+        hipEvent_t start, stop;
+    };
+    std::vector<TimedPair> ev_pending;
+    std::vector<hipEvent_t> ev_pool;
     // scratch used by the operator entry points
     cpm::DBuf scratch[6];
     cpm::HBuf host_scratch;
@@ -136,20 +143,32 @@ namespace cpm {
 int bind(cpm_ctx *ctx);
 
 // Event-timed launch bracket: accumulates milliseconds into stats[slot].
+// Event timing of one launch: two events on the stream, read later by
+// collect_timing (no synchronization here).
+hipEvent_t take_event(cpm_ctx *ctx);
+int collect_timing(cpm_ctx *ctx);
 struct Timed {
     cpm_ctx *ctx;
     int slot;
-    bool on;
-    Timed(cpm_ctx *c, int s) : ctx(c), slot(s), on(c->timing) {
-        if (on) (void)hipEventRecord(ctx->ev_start, ctx->stream);
+    hipEvent_t start = nullptr;
+    Timed(cpm_ctx *c, int s) : ctx(c), slot(s) {
+        if (c->timing && (start = take_event(c)) != nullptr && hipEventRecord(start, c->stream) != hipSuccess) {
+            ctx->ev_pool.push_back(start);
+            start = nullptr;
+        }
     }
     int finish() {
-        if (!on) return CPM_OK;
-        CPM_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
-        CPM_HIP(hipEventSynchronize(ctx->ev_stop));
-        float ms = 0.f;
-        CPM_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
-        ctx->stats[slot] += ms;
+        if (!start) return CPM_OK;
+        hipEvent_t stop = take_event(ctx);
+        if (!stop) {
+            ctx->ev_pool.push_back(start);
+            return CPM_OK;
+        }
+        CPM_HIP(hipEventRecord(stop, ctx->stream));
+        ctx->ev_pending.push_back({slot, start, stop});
+        start = nullptr;
+        // bound the backlog of a context whose statistics are never read
+        if (ctx->ev_pending.size() >= 256) return collect_timing(ctx);
         return CPM_OK;
     }
 };

@@ -14,13 +14,14 @@
 //                 checks the capacities and sizes the next level
 //   plan/write    write the patterns (parent id, item, support) and the next
 //                 level (to staging when the subset test follows)
-//   prune/flags   test each staged candidate's subsets (binary search in this
-//                 level's sorted itemsets); per-block survivor counts
+//   prune/flags   test each staged candidate's subsets (Bloom filter of this
+//                 level's itemsets); per-block survivor counts
 //   prune/compact move the survivors into place, child entry offsets
 //   support       count the next level (support.hip)
 //
-// so the step can be captured once as a hipGraph and replayed level after
-// level with no host round trip, and no block ever waits for another.  The
+// so a chunk of steps can be captured once as a hipGraph and replayed
+// level after level with no host round trip, and no block ever waits for
+// another.  The
 // children of a task are cuEclat's child prefix classes (cuEclat.py:211-245);
 // the join/prune is cuApriori's (cuApriori.py:473-492).  Pruning removes only
 // candidates that cannot be frequent, so the itemsets are the reference's.
@@ -1188,8 +1189,16 @@ struct Search {
         return k;
     }
 
-    int launch_step(int c) {
-        if (!use_graph) return step(c);
+    // kChunkSteps levels, starting with set c, as one graph launch; an even
+    // count leaves the next level in set c again
+    static constexpr int kChunkSteps = 4;
+    static_assert(kChunkSteps % 2 == 0, "a chunk ends on the set it starts from");
+    int chunk(int c) {
+        for (int k = 0; k < kChunkSteps; ++k) CPM_CHECK(step(c ^ (k & 1)));
+        return CPM_OK;
+    }
+    int launch_chunk(int c) {
+        if (!use_graph) return chunk(c);
         const GraphKey k = graph_key(c);
         if (!w.graphs_ok[c] || !(w.key[c] == k)) {
             if (w.graph[c]) {
@@ -1198,7 +1207,7 @@ struct Search {
             }
             hipGraph_t g = nullptr;
             CPM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            int s = step(c);
+            int s = chunk(c);
             hipError_t e = hipStreamEndCapture(ctx->stream, &g);
             if (s != CPM_OK) {
                 if (g) (void)hipGraphDestroy(g);
@@ -1540,12 +1549,9 @@ struct Search {
         }
 
         // ---- levels >= 2: replayed steps ----
-        int c = 1;
+        const int c = 1;
         for (;;) {
-            for (int k = 0; k < 4; ++k) {
-                CPM_CHECK(launch_step(c));
-                c ^= 1;
-            }
+            CPM_CHECK(launch_chunk(c));
             CPM_CHECK(read_ctl(hc));
             if (hc->overflow) {
                 *overflow = hc->overflow;

@@ -209,15 +209,17 @@ int cpm_patterns_databases(const cpm_patterns *p, uint32_t *db);
 
 /* ---- measurement -------------------------------------------------------- */
 
-/* Totals since the last cpm_stats_reset on this context, from HIP events on
- * its stream around each launch of the support kernel:
- * [0] launches, [1] kernel milliseconds, [2] algorithmic bitset bytes
- * (8 * words per row read), [3] candidates counted, [4] gram launches,
- * [5] gram milliseconds, [6] gram int8 ops (2 per multiply-add),
- * [7] search levels. */
+/* Totals since the last cpm_stats_reset on this context:
+ * [0] support launches, [1] their kernel milliseconds (device clock stamps),
+ * [2] algorithmic bitset bytes (8 * words per row read), [3] candidates
+ * counted, [4] gram launches, [5] gram milliseconds (HIP events around each
+ * launch), [6] gram ops (2 per multiply-add), [7] search levels.
+ * Event times are read here: cpm_stats and cpm_stats_reset wait for the
+ * context's stream when timed launches are outstanding. */
 int cpm_stats(cpm_ctx *ctx, double *out8);
 int cpm_stats_reset(cpm_ctx *ctx);
-/* Enable (1) / disable (0) per-launch event timing (default on). */
+/* Enable (1) / disable (0) per-launch event timing of the gram (default on;
+ * it adds no synchronization). */
 int cpm_set_timing(cpm_ctx *ctx, int on);
 
 #ifdef __cplusplus
