@@ -144,8 +144,28 @@ __global__ __launch_bounds__(256) void k_rows_sorted(const BuildJob *__restrict_
 // Its pairs are one contiguous run of the input; they are ORed into an LDS
 // tile (n_items x B words) and the tile is written out once, so the store
 // needs no clearing and HBM sees no atomics.
+// First i in [0, n) with rows[i] >= x (n if none), for non-decreasing rows,
+// by one wave: each step probes 64 positions that split the interval into 65
+// parts, so 10^7 rows take four dependent loads.
+__device__ uint64_t wave_lower_bound(const uint64_t *__restrict__ rows, uint64_t n, uint64_t x) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t lo = 0, hi = n;  // the answer lies in [lo, hi]
+    while (hi - lo > 64) {
+        const uint64_t span = hi - lo;
+        const bool less = rows[lo + (uint64_t)(lane + 1) * span / 65] < x;
+        const uint32_t cnt = (uint32_t)__popcll(__ballot(less));  // a prefix of the lanes
+        const uint64_t nlo = cnt ? lo + (uint64_t)cnt * span / 65 + 1 : lo;
+        const uint64_t nhi = cnt < 64 ? lo + (uint64_t)(cnt + 1) * span / 65 : hi;
+        lo = nlo;
+        hi = nhi;
+    }
+    const uint64_t q = lo + lane;
+    return lo + (uint64_t)__popcll(__ballot(q < hi && rows[q] < x));
+}
+
 __global__ __launch_bounds__(256) void k_bits_tile(const BuildJob *__restrict__ jobs, uint32_t n_jobs) {
     extern __shared__ unsigned long long tile[];
+    __shared__ uint64_t range[2];
     uint32_t lo_j = 0, hi_j = n_jobs;  // last job with block0 <= blockIdx.x
     while (hi_j - lo_j > 1) {
         const uint32_t m = (lo_j + hi_j) >> 1;
@@ -160,27 +180,36 @@ __global__ __launch_bounds__(256) void k_bits_tile(const BuildJob *__restrict__ 
     const uint64_t w0 = (blockIdx.x - j.block0) * B;
     const uint32_t cells = n_items * B;
     for (uint32_t i = tid; i < cells; i += blockDim.x) tile[i] = 0;
-    // [lo, hi): pairs whose row lies in this word range
+    // [lo, hi): pairs whose row lies in this word range, the two ends found
+    // by waves 0 and 1 at once
     const uint64_t r_lo = w0 * 64, r_hi = min((w0 + B) * 64, j.n_trans);
-    uint64_t lo = 0, hi = nnz;
-    while (lo < hi) {
-        const uint64_t m = (lo + hi) >> 1;
-        if (rows[m] < r_lo) lo = m + 1; else hi = m;
-    }
-    uint64_t lo2 = lo, hi2 = nnz;
-    while (lo2 < hi2) {
-        const uint64_t m = (lo2 + hi2) >> 1;
-        if (rows[m] < r_hi) lo2 = m + 1; else hi2 = m;
+    if (tid < 128) {
+        const uint64_t b = wave_lower_bound(rows, nnz, tid < 64 ? r_lo : r_hi);
+        if ((tid & 63) == 0) range[tid >> 6] = b;
     }
     __syncthreads();
-    for (uint64_t i = lo + tid; i < lo2; i += blockDim.x) {
-        const uint64_t r = rows[i];
-        const uint32_t c = cols[i];
-        if (c >= n_items) {
-            atomicOr(j.flags + 2, 1u);
-            continue;
+    const uint64_t lo = range[0], lo2 = range[1];
+    constexpr int kPairBatch = 4;  // loads in flight per lane before its LDS atomics
+    for (uint64_t i0 = lo; i0 < lo2; i0 += (uint64_t)kPairBatch * blockDim.x) {
+        uint64_t r[kPairBatch];
+        uint32_t c[kPairBatch];
+        bool v[kPairBatch];
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) {
+            const uint64_t i = i0 + (uint64_t)q * blockDim.x + tid;
+            v[q] = i < lo2;
+            r[q] = v[q] ? rows[i] : 0;
+            c[q] = v[q] ? cols[i] : 0;
         }
-        atomicOr(&tile[c * B + (uint32_t)((r >> 6) - w0)], 1ull << (r & 63));
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) {
+            if (!v[q]) continue;
+            if (c[q] >= n_items) {
+                atomicOr(j.flags + 2, 1u);
+                continue;
+            }
+            atomicOr(&tile[c[q] * B + (uint32_t)((r[q] >> 6) - w0)], 1ull << (r[q] & 63));
+        }
     }
     __syncthreads();
     uint64_t *__restrict__ store = j.store;
@@ -399,10 +428,11 @@ uint32_t tile_words_for(uint32_t n_items) {
     return n_items ? std::min<uint32_t>(8, 8192 / std::max<uint32_t>(n_items, 1)) : 0;
 }
 
-// Build n stores from device COO.  Every check of the batch is read back in
-// two copies (sortedness / range, then the build's own column check), so a
-// batch costs two host synchronizations whatever its size.  batch = true puts
-// all stores in one arena.
+// Build n stores from device COO.  The sortedness and range checks of the
+// whole batch are read back in one copy, so a batch costs one host
+// synchronization whatever its size (two when a store has unsorted rows: the
+// scatter build's own range check is read back at the end).  batch = true
+// puts all stores in one arena.
 int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bits **out) {
     for (uint32_t i = 0; i < n; ++i) {
         out[i] = nullptr;
