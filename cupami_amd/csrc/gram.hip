@@ -2,28 +2,28 @@
 //
 // With D the N x F 0/1 matrix of the F item rows, the pair supports are
 // G = D^T D: G[a][b] = |row_a & row_b|.  Each 64-bit word of a row is 64
-// consecutive K elements.  A workgroup owns a kBT x kBT block of G (upper
-// block triangle only) and one slice of the K (transaction) axis; it stages
-// kKW-word tiles of its 2 x kBT rows in LDS, and each of its four waves
-// accumulates a (kBT/2)^2 quadrant with v_mfma_scale_f32_16x16x128_f8f6f4 in
-// its FP4 (E2M1) form, kT x kT tiles of 16 x 16.  0 and 1 are exact E2M1
-// values (1.0 is the nibble 0b0010) and every product is 0 or 1, so the FP32
-// accumulators hold exact integers while a slice spans fewer than 2^24
-// transactions; the planner cuts K so that it does.  The FP4 form runs at
-// twice the INT8 rate and takes 128 K elements (two words) per instruction.
+// consecutive K elements.  A 512-lane workgroup owns a 256 x 256 block of G
+// (upper block triangle only) and one slice of the K (transaction) axis.  It
+// stages kKW-word tiles of its 2 x 256 rows in LDS -- eight lanes per row, so
+// one load instruction reads eight whole 128-byte lines -- and each of its
+// eight waves accumulates a 128 x 64 piece with
+// v_mfma_scale_f32_16x16x128_f8f6f4 in its FP4 (E2M1) form, 8 x 4 tiles of
+// 16 x 16.  0 and 1 are exact E2M1 values (1.0 is the nibble 0b0010) and
+// every product is 0 or 1, so the FP32 accumulators hold exact integers
+// while a slice spans fewer than 2^24 transactions; the planner cuts K so
+// that it does.  The FP4 form runs at twice the INT8 rate and takes 128 K
+// elements (two words) per instruction.
 //
 // The bits become nibbles in registers: lane group g (lane / 16) takes 32
-// bits of each word pair (word 2s + g / 2, half g % 2) and spreads them into
-// 16 bytes with shift-and-mask; A (row items) and B (column items) use the
-// same lane -> bit map, so every bit is paired with itself exactly once
-// whatever order the hardware gives the K slots.  The spread is VALU work
-// proportional to the rows a wave touches (2 kT x 16 per word pair) while
-// the MFMA work grows with their product (kT^2 tiles), so at kT = 8 the
-// matrix cores set the pace.  The 256 accumulator registers leave one wave
-// per SIMD; the next K tile is loaded into registers while the current one is
-// multiplied out of LDS.  Slices are summed with integer atomics.  One launch
-// serves any number of databases: the job table lists each workgroup's
-// (database, block row, block column, K range).
+// bits of each word pair (word 2s + g / 2, half g % 2) and spreads them with
+// shift-and-mask; A (row items) and B (column items) use the same lane -> bit
+// map, so every bit is paired with itself exactly once whatever order the
+// hardware gives the K slots.  128 accumulator registers per wave leave room
+// for two waves per SIMD, so one wave's barrier and LDS waits hide under the
+// other's MFMAs.  The next K tile is loaded into registers while the current
+// one is multiplied out of LDS.  Slices are summed with integer atomics.  One
+// launch serves any number of databases: the job table lists each
+// workgroup's (database, block row, block column, K range).
 #include "cpm_internal.h"
 
 #include <algorithm>
@@ -32,19 +32,20 @@ namespace cpm {
 
 namespace {
 
-typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int kT = 8;                 // 16 x 16 tiles per wave side
-constexpr int kBT = 32 * kT;          // block tile (items): 2 x 2 waves
-constexpr int kKW = 8;                // words per K tile
-constexpr int kSteps = kKW / 2;       // MFMA steps (word pairs) per K tile
-constexpr int kLdsW = 2 * kKW + 4;    // u32 per LDS row: the tile and one step of read-ahead
-constexpr uint32_t kE2M1One = 0x22222222u;  // bit 1 of each nibble: 1.0 in E2M1
-constexpr int kFmtFp4 = 4;            // f8f6f4 operand format code of E2M1
-constexpr int kScaleOne = 0x7f7f7f7f; // E8M0 block scales of 2^0
-static_assert(kBT == 256, "one 256-lane block stages the block's rows");
+constexpr int kTX = 8, kTY = 4;             // 16 x 16 tiles per wave: rows, columns
+constexpr int kBT = 256;                     // block tile (items)
+constexpr int kWR = kBT / (16 * kTX);        // waves along rows
+constexpr int kWC = kBT / (16 * kTY);        // waves along columns
+constexpr int kThreads = 64 * kWR * kWC;     // 512: two waves per SIMD
+constexpr int kKW = 16;                      // words per K tile: one 128-byte line per row
+constexpr int kSteps = kKW / 2;              // MFMA steps (word pairs) per K tile
+constexpr int kLdsW = 2 * kKW + 4;           // u32 per LDS row: the tile and one step of read-ahead
+constexpr uint32_t kE2M1One = 0x22222222u;   // bit 1 of each nibble: 1.0 in E2M1
+constexpr int kFmtFp4 = 4;                   // f8f6f4 operand format code of E2M1
+constexpr int kScaleOne = 0x7f7f7f7f;        // E8M0 block scales of 2^0
 static_assert(kKW % 2 == 0, "K tiles hold whole word pairs");
 
 // 32 bits -> 32 E2M1 values 0 / 1.0: bit 4j + k lands in nibble j of dword k.
@@ -53,8 +54,8 @@ __device__ __forceinline__ v8i spread_fp4(uint32_t x) {
                  (int)((x >> 2) & kE2M1One), 0, 0, 0, 0};
 }
 
-__global__ __launch_bounds__(256) void k_gram(const GramJob *__restrict__ jobs, const GramDb *__restrict__ dbs,
-                                              int32_t *__restrict__ out) {
+__global__ __launch_bounds__(kThreads) void k_gram(const GramJob *__restrict__ jobs, const GramDb *__restrict__ dbs,
+                                                   int32_t *__restrict__ out) {
     __shared__ uint32_t sA[kBT][kLdsW];
     __shared__ uint32_t sB[kBT][kLdsW];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -69,83 +70,85 @@ __global__ __launch_bounds__(256) void k_gram(const GramJob *__restrict__ jobs, 
     int32_t *res = out + db.out_off;
     const uint32_t ld = n;
 
-    // loader: thread tid stages the kKW words of A row tid and B row tid
-    const uint32_t ia = bi * kBT + tid, ib = bj * kBT + tid;
-    const gptr<uint4> pa = ia < n ? (gptr<uint4>)(store + (uint64_t)cols[ia] * stride) : nullptr;
-    const gptr<uint4> pb = ib < n ? (gptr<uint4>)(store + (uint64_t)cols[ib] * stride) : nullptr;
-    uint4 ra[kKW / 2], rb[kKW / 2];
-    auto fetch = [&](uint64_t k0) {
+    // loader: lanes 4r .. 4r+3 of a wave stage the 64 bytes (kKW words) of
+    // one row, so one load instruction touches 16 rows; wave w's pass q
+    // covers rows 16 (4w + q) .. +15 of the 512 (A then B)
+    constexpr int kLanesPerRow = kKW / 2;            // uint4 per row per tile
+    constexpr int kRowsPerPass = 64 / kLanesPerRow;  // rows per load instruction
+    constexpr int kPasses = 2 * kBT / (kRowsPerPass * (kThreads / 64));
+    static_assert(kPasses * kRowsPerPass * (kThreads / 64) == 2 * kBT, "passes cover A and B");
+    const uint32_t part = lane % kLanesPerRow;
+    gptr<uint4> pr[kPasses];
+    uint32_t *srow[kPasses];
 #pragma unroll
-        for (int q = 0; q < kKW / 2; ++q) {
-            const uint64_t w = k0 + 2 * q;  // even; w + 1 < stride (zero padded)
-            const bool in = w < k_end;
-            ra[q] = make_uint4(0, 0, 0, 0);
-            rb[q] = make_uint4(0, 0, 0, 0);
-            if (pa && in) ra[q] = pa[w / 2];
-            if (pb && in) rb[q] = pb[w / 2];
+    for (int q = 0; q < kPasses; ++q) {
+        const uint32_t R = (wave * kPasses + q) * kRowsPerPass + lane / kLanesPerRow;  // 0 .. 511
+        const bool isB = R >= (uint32_t)kBT;
+        const uint32_t lr = R % kBT;
+        const uint32_t item = (isB ? bj : bi) * kBT + lr;
+        pr[q] = item < n ? (gptr<uint4>)(store + (uint64_t)cols[item] * stride) : nullptr;
+        srow[q] = (isB ? &sB[lr][0] : &sA[lr][0]) + 4 * part;
+    }
+    uint4 rr[kPasses];
+    auto fetch = [&](uint64_t k0) {
+        const uint64_t w = k0 + 2 * part;  // even; w + 1 < stride (zero padded)
+#pragma unroll
+        for (int q = 0; q < kPasses; ++q) {
+            rr[q] = make_uint4(0, 0, 0, 0);
+            if (pr[q] && w < k_end) rr[q] = pr[q][w / 2];
         }
     };
     auto stash = [&]() {
 #pragma unroll
-        for (int q = 0; q < kKW / 2; ++q) {
-            *reinterpret_cast<uint4 *>(&sA[tid][4 * q]) = ra[q];
-            *reinterpret_cast<uint4 *>(&sB[tid][4 * q]) = rb[q];
-        }
+        for (int q = 0; q < kPasses; ++q) *reinterpret_cast<uint4 *>(srow[q]) = rr[q];
     };
 
-    const uint32_t wr = wave >> 1, wc = wave & 1;
+    const uint32_t wr = wave / kWC, wc = wave % kWC;
     const uint32_t li = lane & 15, lg = lane >> 4;
-    // on a diagonal block the lower-left quadrant lies below the diagonal
-    const bool idle = (bi == bj) && (wr > wc);
-    v4f acc[kT][kT];
+    // on a diagonal block, waves whose rows all lie below their columns idle
+    const bool idle = (bi == bj) && (wr * 16 * kTX > wc * 16 * kTY + 16 * kTY - 1);
+    v4f acc[kTX][kTY];
 #pragma unroll
-    for (int x = 0; x < kT; ++x)
+    for (int x = 0; x < kTX; ++x)
 #pragma unroll
-        for (int y = 0; y < kT; ++y) acc[x][y] = (v4f){0.f, 0.f, 0.f, 0.f};
+        for (int y = 0; y < kTY; ++y) acc[x][y] = (v4f){0.f, 0.f, 0.f, 0.f};
 
-    const uint32_t *rowA = &sA[wr * 16 * kT + li][lg];
-    const uint32_t *rowB = &sB[wc * 16 * kT + li][lg];
+    const uint32_t *rowA = &sA[wr * 16 * kTX + li][lg];
+    const uint32_t *rowB = &sB[wc * 16 * kTY + li][lg];
     if (k_begin < k_end) fetch(k_begin);
     for (uint64_t k0 = k_begin; k0 < k_end; k0 += kKW) {
         stash();
         __syncthreads();
         if (k0 + kKW < k_end) fetch(k0 + kKW);
         if (!idle) {
-            // Software pipeline over the tile's word pairs: the next pair's
-            // LDS reads are issued before this pair's MFMAs, and its B spreads
-            // are computed between this pair's MFMA rows (one per row), so the
-            // VALU work hides under the matrix cores.  The last four u32 of an
-            // LDS row are read ahead of the last pair and go unused.
-            uint32_t ca[kT], nb[kT];
-            v8i fb[kT];
+            // The next word pair's LDS reads are issued before this pair's
+            // MFMAs; the last four u32 of an LDS row are read ahead of the
+            // last pair and go unused.
+            uint32_t ca[kTX], nb[kTY];
+            v8i fb[kTY];
 #pragma unroll
-            for (int t = 0; t < kT; ++t) {
-                ca[t] = rowA[t * 16 * kLdsW];
-                fb[t] = spread_fp4(rowB[t * 16 * kLdsW]);
-            }
+            for (int t = 0; t < kTX; ++t) ca[t] = rowA[t * 16 * kLdsW];
+#pragma unroll
+            for (int t = 0; t < kTY; ++t) fb[t] = spread_fp4(rowB[t * 16 * kLdsW]);
 #pragma unroll 1
             for (int s = 0; s < kSteps; ++s) {
-                uint32_t na[kT];
+                uint32_t na[kTX];
 #pragma unroll
-                for (int t = 0; t < kT; ++t) {
-                    na[t] = rowA[t * 16 * kLdsW + 4 * (s + 1)];
-                    nb[t] = rowB[t * 16 * kLdsW + 4 * (s + 1)];
-                }
-                v8i gb[kT];
+                for (int t = 0; t < kTX; ++t) na[t] = rowA[t * 16 * kLdsW + 4 * (s + 1)];
 #pragma unroll
-                for (int x = 0; x < kT; ++x) {
+                for (int t = 0; t < kTY; ++t) nb[t] = rowB[t * 16 * kLdsW + 4 * (s + 1)];
+#pragma unroll
+                for (int x = 0; x < kTX; ++x) {
                     const v8i fa = spread_fp4(ca[x]);
 #pragma unroll
-                    for (int y = 0; y < kT; ++y)
+                    for (int y = 0; y < kTY; ++y)
                         acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
                             fa, fb[y], acc[x][y], kFmtFp4, kFmtFp4, 0, kScaleOne, 0, kScaleOne);
-                    gb[x] = spread_fp4(nb[x]);
                 }
 #pragma unroll
-                for (int t = 0; t < kT; ++t) {
-                    ca[t] = na[t];
-                    fb[t] = gb[t];
-                }
+                for (int t = 0; t < kTX; ++t) ca[t] = na[t];
+#pragma unroll
+                for (int t = 0; t < kTY; ++t) fb[t] = spread_fp4(nb[t]);
             }
         }
         __syncthreads();
@@ -153,13 +156,13 @@ __global__ __launch_bounds__(256) void k_gram(const GramJob *__restrict__ jobs, 
     if (idle) return;
     // D tile layout: lane holds rows 4*(lane/16) + r, column lane % 16
 #pragma unroll
-    for (int x = 0; x < kT; ++x)
+    for (int x = 0; x < kTX; ++x)
 #pragma unroll
-        for (int y = 0; y < kT; ++y) {
-            const uint32_t col = bj * kBT + wc * 16 * kT + y * 16 + li;
+        for (int y = 0; y < kTY; ++y) {
+            const uint32_t col = bj * kBT + wc * 16 * kTY + y * 16 + li;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const uint32_t row = bi * kBT + wr * 16 * kT + x * 16 + lg * 4 + r;
+                const uint32_t row = bi * kBT + wr * 16 * kTX + x * 16 + lg * 4 + r;
                 const int v = (int)acc[x][y][r];
                 if (row < n && col < n && row <= col && v) atomicAdd(res + (uint64_t)row * ld + col, v);
             }
@@ -171,20 +174,33 @@ __global__ __launch_bounds__(256) void k_gram(const GramJob *__restrict__ jobs, 
 // transactions per K slice: every partial sum is at most this, below 2^24
 constexpr uint64_t kMaxSliceBits = 1ull << 23;
 
+// waves of a diagonal block with work (the others lie below the diagonal)
+constexpr int diagonal_active_waves() {
+    int a = 0;
+    for (int wr = 0; wr < kWR; ++wr)
+        for (int wc = 0; wc < kWC; ++wc) a += !(wr * 16 * kTX > wc * 16 * kTY + 16 * kTY - 1);
+    return a;
+}
+
 int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<GramJob> &jobs,
               uint64_t *out_elems) {
     jobs.clear();
-    uint64_t tri_total = 0, out = 0;
+    // work of a block: its K tiles x the share of its waves with work
+    const double diag = (double)diagonal_active_waves() / (kWR * kWC);
+    double total = 0;
+    uint64_t out = 0;
     for (const GramDb &d : dbs) {
-        const uint64_t nb = (d.n_items + kBT - 1) / kBT;
-        tri_total += nb * (nb + 1) / 2;
         out += (uint64_t)d.n_items * d.n_items;
+        if (d.n_items == 0 || d.words == 0) continue;
+        const uint64_t nb = (d.n_items + kBT - 1) / kBT;
+        const double kt = (double)((d.words + kKW - 1) / kKW);
+        total += kt * ((double)nb * (nb - 1) / 2 + diag * nb);
     }
     *out_elems = out;
-    // One workgroup per CU at a time (the accumulators take the register
-    // file): split K so the jobs fill one round of CUs without a partial
-    // second round (jobs <= CUs, rounded down).
-    const uint64_t want = (uint64_t)ctx->cu_count;
+    // One workgroup per CU at a time (the accumulators take half the
+    // register file): cut each block's K range in proportion to its work so
+    // that the jobs fill one round of CUs evenly.
+    const double per_job = total / (double)std::max(1, ctx->cu_count);
     for (uint32_t i = 0; i < dbs.size(); ++i) {
         const GramDb &d = dbs[i];
         if (d.n_items == 0 || d.words == 0) continue;
@@ -192,15 +208,17 @@ int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<Gr
             return fail(CPM_ERR_INVALID, "int32 pair counts need fewer than 2^31 transactions");
         const uint32_t nb = (d.n_items + kBT - 1) / kBT;
         const uint64_t ktiles = (d.words + kKW - 1) / kKW;
-        uint64_t slices = tri_total ? std::max<uint64_t>(1, want / tri_total) : 1;
-        slices = std::min<uint64_t>(slices, ktiles);
         // FP32 sums stay exact below 2^24 transactions per slice
-        slices = std::max<uint64_t>(slices, (ktiles * kKW * 64 + kMaxSliceBits - 1) / kMaxSliceBits);
-        const uint64_t slice_words = (ktiles + slices - 1) / slices * kKW;
+        const uint64_t exact = (ktiles * kKW * 64 + kMaxSliceBits - 1) / kMaxSliceBits;
         for (uint32_t bi = 0; bi < nb; ++bi)
-            for (uint32_t bj = bi; bj < nb; ++bj)
+            for (uint32_t bj = bi; bj < nb; ++bj) {
+                const double work = (double)ktiles * (bi == bj ? diag : 1.0);
+                uint64_t slices = per_job > 0 ? (uint64_t)(work / per_job) : 1;
+                slices = std::max<uint64_t>(std::min<uint64_t>(std::max<uint64_t>(slices, 1), ktiles), exact);
+                const uint64_t slice_words = (ktiles + slices - 1) / slices * kKW;
                 for (uint64_t k0 = 0; k0 < d.words; k0 += slice_words)
                     jobs.push_back({i, bi, bj, 0, k0, std::min<uint64_t>(d.words, k0 + slice_words)});
+            }
     }
     if (jobs.size() >= (1ull << 31)) return fail(CPM_ERR_INVALID, "pair block grid too large");
     return CPM_OK;
@@ -211,7 +229,7 @@ int launch_gram(cpm_ctx *ctx, const GramJob *d_jobs, uint64_t n_jobs, const Gram
     CPM_HIP(hipMemsetAsync(d_out, 0, out_elems * sizeof(int32_t), ctx->stream));
     if (n_jobs == 0) return CPM_OK;
     Timed timer(ctx, 5);
-    hipLaunchKernelGGL(k_gram, dim3((unsigned)n_jobs), dim3(256), 0, ctx->stream, d_jobs, d_dbs,
+    hipLaunchKernelGGL(k_gram, dim3((unsigned)n_jobs), dim3(kThreads), 0, ctx->stream, d_jobs, d_dbs,
                        d_out);
     CPM_HIP(hipGetLastError());
     CPM_CHECK(timer.finish());
