@@ -82,7 +82,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch level steps directly")
     ap.add_argument("--group", type=int, default=32, help="entries per support work item")
-    ap.add_argument("--tile", type=int, default=2048, help="words per support tile (1024|2048)")
+    ap.add_argument("--tile", type=int, default=1024, help="words per support tile (1024|2048)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

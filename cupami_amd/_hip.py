@@ -292,7 +292,7 @@ class Context:
                                           1 if use_gram else 0, ctypes.byref(handle)))
         return Patterns(handle, databases=True)
 
-    def set_search_options(self, group=32, tile_words=2048, use_graph=True, subset_prune=True):
+    def set_search_options(self, group=32, tile_words=1024, use_graph=True, subset_prune=True):
         check(load().cpm_set_search_options(self.handle, int(group), int(tile_words),
                                             1 if use_graph else 0, 1 if subset_prune else 0))
 

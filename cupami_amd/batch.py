@@ -58,7 +58,7 @@ class BatchMiner:
                 self._cache = {k: tuple(v) for k, v in json.loads(self.cache_path.read_text()).items()}
             except (OSError, ValueError):
                 self._cache = {}
-        self.config = (32, 2048)
+        self.config = (32, 1024)
 
     def build(self, specs):
         return self.ctx.build_bits_device_batch(specs)

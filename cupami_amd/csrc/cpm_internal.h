@@ -89,7 +89,7 @@ struct cpm_ctx {
     cpm::DBuf spare_arena;   // batch-build arena kept for the next batch
     cpm::HBuf bounce;        // pinned bounce buffer of copy_to_host
     // search schedule (cpm_set_search_options)
-    uint32_t opt_group = 32, opt_tile_words = 2048;
+    uint32_t opt_group = 32, opt_tile_words = 1024;
     bool opt_graph = true, opt_prune = true;
 };
 

@@ -141,7 +141,7 @@ int cpm_eclat_mine_batch(cpm_ctx *ctx, const cpm_bits *const *bits, const uint64
                          uint32_t n_db, int use_gram, cpm_patterns **out);
 
 /* Search schedule of this context: entries per support work item (1..64),
- * words per tile (1024 or 2048), replay of the level step as a hipGraph
+ * words per tile (1024, the default, or 2048), replay of the level step as a hipGraph
  * (use_graph), and the subset test of candidates before counting. */
 int cpm_set_search_options(cpm_ctx *ctx, uint32_t group, uint32_t tile_words, int use_graph,
                            int subset_prune);
