@@ -78,6 +78,9 @@ SIGNATURES = {
     "cpm_set_search_options": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                               ctypes.c_int]),
     "cpm_patterns_databases": (ctypes.c_int, [_vp, _u32p]),
+    "cpm_patterns_view": (ctypes.c_int, [_vp, ctypes.POINTER(_u32p), ctypes.POINTER(_u32p),
+                                         ctypes.POINTER(_u64p), ctypes.POINTER(_i64p),
+                                         ctypes.POINTER(_u32p)]),
     "cpm_patterns_count": (ctypes.c_int, [_vp, _u64p]),
     "cpm_patterns_download": (ctypes.c_int, [_vp, _u32p, _u32p, _u64p, _i64p]),
     "cpm_patterns_destroy": (ctypes.c_int, [_vp]),
@@ -434,29 +437,57 @@ class Bits:
         return out
 
 
+class _PatternsHandle:
+    """Owns a ``cpm_patterns`` result; the arrays viewed from it keep it alive."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def __del__(self):
+        if self.handle and _lib is not None:
+            _lib.cpm_patterns_destroy(self.handle)
+            self.handle = None
+
+
+def _view(owner, address, n, ctype, dtype):
+    """numpy view of n elements at `address`, keeping `owner` alive."""
+    buf = (ctype * n).from_address(address)
+    buf._owner = owner
+    return np.frombuffer(buf, dtype=dtype)
+
+
 class Patterns:
-    """Pattern forest returned by a search (``cpm_patterns``)."""
+    """Pattern forest returned by a search (``cpm_patterns``): numpy views of
+    the library's result arrays (no copy; the result is released when the
+    last view goes)."""
 
     def __init__(self, handle, periodic=False, databases=False):
         lib = load()
+        owner = _PatternsHandle(handle)
         n = ctypes.c_uint64()
-        try:
-            check(lib.cpm_patterns_count(handle, ctypes.byref(n)))
-            self.n = n.value
-            self.parent = np.zeros(self.n, dtype=np.uint32)
-            self.item = np.zeros(self.n, dtype=np.uint32)
-            self.support = np.zeros(self.n, dtype=np.uint64)
-            self.period = np.zeros(self.n, dtype=np.int64) if periodic else None
-            self.db = np.zeros(self.n, dtype=np.uint32)
-            if self.n:
-                check(lib.cpm_patterns_download(
-                    handle, _p(self.parent, _u32p), _p(self.item, _u32p),
-                    _p(self.support, _u64p), _p(self.period, _i64p) if periodic else None))
-                if databases:
-                    check(lib.cpm_patterns_databases(handle, _p(self.db, _u32p)))
-        finally:
-            lib.cpm_patterns_destroy(handle)
+        check(lib.cpm_patterns_count(handle, ctypes.byref(n)))
+        self.n = n.value
         self.periodic = periodic
+        parent, item, db = _u32p(), _u32p(), _u32p()
+        support, period = _u64p(), _i64p()
+        check(lib.cpm_patterns_view(handle, ctypes.byref(parent), ctypes.byref(item),
+                                    ctypes.byref(support), ctypes.byref(period),
+                                    ctypes.byref(db)))
+        if self.n:
+            addr = lambda p: ctypes.cast(p, ctypes.c_void_p).value
+            self.parent = _view(owner, addr(parent), self.n, ctypes.c_uint32, np.uint32)
+            self.item = _view(owner, addr(item), self.n, ctypes.c_uint32, np.uint32)
+            self.support = _view(owner, addr(support), self.n, ctypes.c_uint64, np.uint64)
+            self.period = (_view(owner, addr(period), self.n, ctypes.c_int64, np.int64)
+                           if periodic else None)
+            self.db = (_view(owner, addr(db), self.n, ctypes.c_uint32, np.uint32)
+                       if databases and db else np.zeros(self.n, dtype=np.uint32))
+        else:
+            self.parent = np.zeros(0, dtype=np.uint32)
+            self.item = np.zeros(0, dtype=np.uint32)
+            self.support = np.zeros(0, dtype=np.uint64)
+            self.period = np.zeros(0, dtype=np.int64) if periodic else None
+            self.db = np.zeros(0, dtype=np.uint32)
 
     def __len__(self):
         return self.n

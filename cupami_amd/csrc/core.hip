@@ -1,6 +1,8 @@
 // Context, errors, device buffers, the vertical store and the operator-level
 // entry points (count_supports, intersect_extensions, periodic_eval).
 #include "cpm_internal.h"
+
+#include <mutex>
 #include "support_kernels.h"
 
 #include <string.h>
@@ -325,6 +327,63 @@ static int bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits
 namespace cpm {
 
 constexpr size_t kBounceBytes = 8u << 20;
+
+// ---- host block cache ------------------------------------------------------
+// A large block carries a 64-byte header with its capacity.  The cache keeps
+// up to kHostCacheBlocks blocks and kHostCacheBytes in all (held until the
+// process ends); a request takes the smallest cached block that fits and is
+// at most twice its size.
+namespace {
+constexpr size_t kHostBlockMin = 1 << 20;
+constexpr size_t kHostCacheBlocks = 16;
+constexpr size_t kHostCacheBytes = (size_t)512 << 20;
+constexpr size_t kHostHeader = 64;
+std::mutex g_host_mu;
+std::vector<std::pair<size_t, void *>> g_host_cache;  // (capacity, block)
+size_t g_host_cached = 0;
+}  // namespace
+
+void *host_block_take(size_t bytes) {
+    if (bytes < kHostBlockMin) return malloc(bytes ? bytes : 1);
+    {
+        std::lock_guard<std::mutex> lock(g_host_mu);
+        size_t best = g_host_cache.size();
+        for (size_t i = 0; i < g_host_cache.size(); ++i) {
+            const size_t cap = g_host_cache[i].first;
+            if (cap >= bytes && cap <= 2 * bytes && (best == g_host_cache.size() || cap < g_host_cache[best].first))
+                best = i;
+        }
+        if (best < g_host_cache.size()) {
+            void *b = g_host_cache[best].second;
+            g_host_cached -= g_host_cache[best].first;
+            g_host_cache.erase(g_host_cache.begin() + best);
+            return static_cast<unsigned char *>(b) + kHostHeader;
+        }
+    }
+    void *b = malloc(bytes + kHostHeader);
+    if (!b) return nullptr;
+    *static_cast<size_t *>(b) = bytes;
+    return static_cast<unsigned char *>(b) + kHostHeader;
+}
+
+void host_block_give(void *p, size_t bytes) {
+    if (!p) return;
+    if (bytes < kHostBlockMin) {  // the malloc path of host_block_take
+        free(p);
+        return;
+    }
+    void *b = static_cast<unsigned char *>(p) - kHostHeader;
+    const size_t cap = *static_cast<size_t *>(b);
+    {
+        std::lock_guard<std::mutex> lock(g_host_mu);
+        if (g_host_cache.size() < kHostCacheBlocks && g_host_cached + cap <= kHostCacheBytes) {
+            g_host_cache.push_back({cap, b});
+            g_host_cached += cap;
+            return;
+        }
+    }
+    free(b);
+}
 
 hipEvent_t take_event(cpm_ctx *ctx) {
     if (!ctx->ev_pool.empty()) {
@@ -782,6 +841,18 @@ int cpm_patterns_download(const cpm_patterns *p, uint32_t *parent, uint32_t *ite
         if (p->periodic) memcpy(period, p->period.data(), p->n * 8);
         else memset(period, 0, p->n * 8);
     }
+    return CPM_OK;
+}
+
+int cpm_patterns_view(const cpm_patterns *p, const uint32_t **parent, const uint32_t **item,
+                      const uint64_t **support, const int64_t **period, const uint32_t **db) {
+    if (!p) return fail(CPM_ERR_INVALID, "null patterns");
+    const bool any = p->n != 0;
+    if (parent) *parent = any ? p->parent.data() : nullptr;
+    if (item) *item = any ? p->item.data() : nullptr;
+    if (support) *support = any ? p->support.data() : nullptr;
+    if (period) *period = any && p->periodic ? p->period.data() : nullptr;
+    if (db) *db = any && p->db.size() == p->n ? p->db.data() : nullptr;
     return CPM_OK;
 }
 

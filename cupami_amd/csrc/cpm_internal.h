@@ -6,6 +6,8 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <memory>
+#include <new>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -129,12 +131,50 @@ struct cpm_bits {
     uint64_t *rows() const { return view ? view : store.as<uint64_t>(); }
 };
 
+namespace cpm {
+// Large host blocks are recycled through a small process-wide cache, so a
+// search's result arrays land in memory that is already mapped (fresh pages
+// cost the kernel a zeroing pass: ~9 ms per 120 MB).  Blocks under 1 MiB go
+// straight to malloc / free.
+void *host_block_take(size_t bytes);
+void host_block_give(void *p, size_t bytes);  // bytes: as passed to host_block_take
+// Allocator over that cache; elements are default-initialised, so a resize
+// writes nothing before the real data does.
+template <class T>
+struct HostCache {
+    using value_type = T;
+    HostCache() = default;
+    template <class U>
+    HostCache(const HostCache<U> &) {}
+    T *allocate(size_t n) {
+        void *p = host_block_take(n * sizeof(T));
+        if (!p) throw std::bad_alloc();
+        return static_cast<T *>(p);
+    }
+    void deallocate(T *p, size_t n) { host_block_give(p, n * sizeof(T)); }
+    template <class U>
+    void construct(U *p) noexcept {
+        ::new (static_cast<void *>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+    template <class U>
+    bool operator==(const HostCache<U> &) const { return true; }
+    template <class U>
+    bool operator!=(const HostCache<U> &) const { return false; }
+};
+template <class T>
+using hvec = std::vector<T, HostCache<T>>;
+}  // namespace cpm
+
 struct cpm_patterns {
     uint64_t n = 0;
     bool periodic = false;
-    std::vector<uint32_t> parent, item, db;
-    std::vector<uint64_t> support;
-    std::vector<int64_t> period;
+    cpm::hvec<uint32_t> parent, item, db;
+    cpm::hvec<uint64_t> support;
+    cpm::hvec<int64_t> period;
 };
 
 namespace cpm {

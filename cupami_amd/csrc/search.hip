@@ -581,6 +581,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             uint32_t *dst = p.N.prefix + (uint64_t)ct * kMaxDepth;
             for (uint32_t i = 0; i < d; ++i) dst[i] = src[i];
             dst[d] = f;
+            p.N.par[ct] = t;
             p.N.pid[ct] = (uint32_t)(pb + lane);
             p.N.db[ct] = dbi;
             uint32_t w = p.off_cext[t] + off;
@@ -653,6 +654,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
                 uint32_t *dst = p.N.prefix + (uint64_t)ct * kMaxDepth;
                 for (uint32_t i = 0; i < d; ++i) dst[i] = src[i];
                 dst[d] = fj;
+                p.N.par[ct] = t;
                 p.N.pid[ct] = (uint32_t)(pb + j);
                 p.N.db[ct] = dbi;
                 uint32_t w = xb + carry + off;
@@ -957,12 +959,13 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
 // ---- host side ---------------------------------------------------------------------
 
 struct LevelSet {
-    DBuf prefix, pid, db, ext_off, ext, sup, per;
+    DBuf prefix, pid, par, db, ext_off, ext, sup, per;
     DBuf kept, cnt[3], off[3], off2, ext2, live, pos, parts;
     LevelPtrs ptrs(LevelSizes *sz) const {
         LevelPtrs p;
         p.prefix = prefix.as<uint32_t>();
         p.pid = pid.as<uint32_t>();
+        p.par = par.as<uint32_t>();
         p.db = db.as<uint32_t>();
         p.ext_off = ext_off.as<uint32_t>();
         p.ext = ext.as<uint32_t>();
@@ -975,6 +978,7 @@ struct LevelSet {
         const uint64_t t1 = c.tasks + 1, e1 = c.ext + 1;
         CPM_CHECK(prefix.ensure(t1 * kMaxDepth * 4));
         CPM_CHECK(pid.ensure(t1 * 4));
+        CPM_CHECK(par.ensure(t1 * 4));
         CPM_CHECK(db.ensure(t1 * 4));
         CPM_CHECK(ext_off.ensure(t1 * 4));
         CPM_CHECK(off2.ensure(t1 * 4));
@@ -1128,6 +1132,7 @@ struct Search {
         a.parts_tiles = parts_tiles;
         a.te = te;
         a.stamps = w.stamps.as<unsigned long long>();
+        a.siblings = 1;
         return a;
     }
 
@@ -1168,7 +1173,7 @@ struct Search {
         auto add = [&](const DBuf &b) { k.ptrs[i++] = b.p; };
         for (int j = 0; j < 2; ++j) {
             const LevelSet &S = w.set[(c + j) & 1];
-            add(S.prefix); add(S.pid); add(S.db); add(S.ext_off); add(S.ext); add(S.sup); add(S.per); add(S.kept); add(S.off2); add(S.ext2); add(S.live);
+            add(S.prefix); add(S.pid); add(S.par); add(S.db); add(S.ext_off); add(S.ext); add(S.sup); add(S.per); add(S.kept); add(S.off2); add(S.ext2); add(S.live);
             add(S.pos); add(S.parts);
             for (int q = 0; q < 3; ++q) { add(S.cnt[q]); add(S.off[q]); }
         }
@@ -1616,13 +1621,8 @@ struct Search {
         cpm_patterns *p = new cpm_patterns();
         p->n = n;
         p->periodic = periodic;
-        p->parent.resize(n);
-        p->item.resize(n);
-        p->support.resize(n);
-        p->db.resize(n);
-        std::vector<uint32_t> s32(n);
-        if (periodic) p->period.resize(n);
         hipStream_t st = ctx->stream;
+        const unsigned char *h = nullptr;
         if (n) {
             const size_t row = periodic ? 20 : 12;
             int sc = w.down.ensure(n * row);
@@ -1630,30 +1630,47 @@ struct Search {
                 delete p;
                 return sc;
             }
-            unsigned char *h = w.down.as<unsigned char>();
+            unsigned char *hd = w.down.as<unsigned char>();
             // parent, item and support columns in one strided copy
-            hipError_t e = hipMemcpy2DAsync(h, n * 4, w.pat.p, pat_pitch(), n * 4, 3, hipMemcpyDeviceToHost, st);
+            hipError_t e = hipMemcpy2DAsync(hd, n * 4, w.pat.p, pat_pitch(), n * 4, 3, hipMemcpyDeviceToHost, st);
             if (e == hipSuccess && periodic)
-                e = hipMemcpyAsync(h + n * 12, w.pat_per.p, n * 8, hipMemcpyDeviceToHost, st);
+                e = hipMemcpyAsync(hd + n * 12, w.pat_per.p, n * 8, hipMemcpyDeviceToHost, st);
             if (e == hipSuccess) e = hipStreamSynchronize(st);
             if (e != hipSuccess) {
                 delete p;
                 return fail(CPM_ERR_RUNTIME, "pattern download failed: %s", hipGetErrorString(e));
             }
-            memcpy(p->parent.data(), h, n * 4);
-            memcpy(p->item.data(), h + n * 4, n * 4);
-            memcpy(s32.data(), h + n * 8, n * 4);
-            if (periodic) memcpy(p->period.data(), h + n * 12, n * 8);
+            h = hd;
         }
-        // database of each pattern: singletons by range, others by parent
-        uint32_t di = 0;
-        for (uint64_t i = 0; i < n; ++i) {
-            p->support[i] = s32[i];
-            if (p->parent[i] == kNone) {
-                while (di + 1 < dbs.size() && i >= (uint64_t)dbs[di].fbase + dbs[di].nf) ++di;
-                p->db[i] = di;
-            } else {
-                p->db[i] = p->db[p->parent[i]];
+        try {
+            // one pass over the pinned copy into the result (its arrays come
+            // from recycled host blocks and are not cleared first)
+            p->parent.resize(n);
+            p->item.resize(n);
+            p->support.resize(n);
+            p->db.resize(n);
+            if (periodic) p->period.resize(n);
+        } catch (const std::bad_alloc &) {
+            delete p;
+            return fail(CPM_ERR_NOMEM, "host memory for %llu patterns", (unsigned long long)n);
+        }
+        if (n) {
+            const uint32_t *hp = reinterpret_cast<const uint32_t *>(h);
+            memcpy(p->parent.data(), hp, n * 4);
+            memcpy(p->item.data(), hp + n, n * 4);
+            if (periodic) memcpy(p->period.data(), h + n * 12, n * 8);
+            const uint32_t *hs = hp + 2 * n;
+            // database of each pattern: singletons by range, others by parent
+            uint32_t di = 0;
+            for (uint64_t i = 0; i < n; ++i) {
+                p->support[i] = hs[i];
+                const uint32_t par = hp[i];
+                if (par == kNone) {
+                    while (di + 1 < dbs.size() && i >= (uint64_t)dbs[di].fbase + dbs[di].nf) ++di;
+                    p->db[i] = di;
+                } else {
+                    p->db[i] = p->db[par];
+                }
             }
         }
         // statistics

@@ -54,9 +54,10 @@ __device__ __forceinline__ uint32_t col_of(cptr<uint32_t> fcol, uint32_t f) {
 // through the constant address space, so that the per-row lookups of a work
 // item (task offsets, prefix and entry items, columns) are scalar loads.
 struct LevelView {
-    cptr<uint32_t> ext_off, dbi, prefix, ext, fcol;
+    cptr<uint32_t> ext_off, dbi, prefix, ext, fcol, par;
     cptr<DbDev> dbs;
     uint32_t G, te, n_tasks, d, n_ext;
+    int siblings;
     uint64_t total;
 };
 
@@ -76,6 +77,8 @@ __device__ __forceinline__ bool level_view(const SupportArgs &a, LevelView &v) {
     v.dbs = to_const(a.dbs);
     v.G = (uint32_t)a.L.sz->group;
     v.te = a.te;
+    v.siblings = a.siblings && a.L.par;
+    v.par = v.siblings ? to_const(a.L.par) : nullptr;
     return true;
 }
 
@@ -127,6 +130,12 @@ __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
     for (uint64_t it = blockIdx.x; it < v.total; it += gridDim.x) {
         const Range r = range_of(v, it);
         uint32_t t = find_task(v.ext_off, v.n_tasks, r.r0);
+        // The AND of the first d - 1 prefix rows of the last task built here
+        // (its parent's itemset), reused while the next tasks are its
+        // siblings: children of one task are stored next to each other.
+        uint4 PP[QPL];
+        bool pp_ok = false;      // PP holds a parent's rows for this tile
+        uint32_t pp_par = 0;     // that parent
         for (uint32_t e0 = r.r0; e0 < r.r1;) {
             while (v.ext_off[t + 1] <= e0) ++t;  // empty tasks
             const uint32_t e1 = min(r.r1, v.ext_off[t + 1]);
@@ -138,37 +147,77 @@ __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
                 e0 = e1;
                 continue;
             }
-            bytes += 8ull * tile_words_of(db->words, r.tile, 2 * kQuadsPerTile) * (v.d + e1 - e0);
             const gptr<uint4> store = to_global(reinterpret_cast<const uint4 *>(db->store));
             const uint64_t stride_q = db->stride / 2;
             const uint64_t q0 = (uint64_t)r.tile * kQuadsPerTile + tid;
-            uint4 P[QPL];
+            // Lane offsets inside a row, clamped into it: every load stays in
+            // bounds and is issued without a branch (a guarded load makes the
+            // compiler wait for it on the spot).  Lanes past the row's end are
+            // zero in P, which zeroes their popcounts.
+            uint64_t off[QPL];
             bool valid[QPL];
 #pragma unroll
             for (int q = 0; q < QPL; ++q) {
-                valid[q] = q0 + (uint64_t)q * 256 < stride_q;
-                P[q] = valid[q] ? make_uint4(~0u, ~0u, ~0u, ~0u) : make_uint4(0, 0, 0, 0);
+                const uint64_t o = q0 + (uint64_t)q * 256;
+                valid[q] = o < stride_q;
+                off[q] = valid[q] ? o : stride_q - 1;
             }
+            auto row_of = [&](uint32_t f) { return store + (uint64_t)col_of(v.fcol, f) * stride_q; };
             const cptr<uint32_t> pre = v.prefix + (uint64_t)t * kMaxDepth;
-            for (uint32_t j = 0; j < v.d; ++j) {
-                const gptr<uint4> row = store + (uint64_t)col_of(v.fcol, pre[j]) * stride_q + q0;
+            // the parent PP was built for (siblings share the database and the
+            // first d - 1 prefix items); a prefix of at most one item has
+            // nothing to share
+            const uint32_t par = v.siblings && v.d >= 2 ? v.par[t] : ~0u;
+            const bool same = pp_ok && par != ~0u && par == pp_par;
+            uint32_t rows_read = e1 - e0;
+            // the last prefix row first, so its load overlaps the parent's
+            uint4 last[QPL];
+#pragma unroll
+            for (int q = 0; q < QPL; ++q) last[q] = make_uint4(~0u, ~0u, ~0u, ~0u);
+            if (v.d) {
+                const gptr<uint4> row = row_of(pre[v.d - 1]);
+#pragma unroll
+                for (int q = 0; q < QPL; ++q) last[q] = row[off[q]];
+                rows_read += 1;
+            }
+            if (!same) {
 #pragma unroll
                 for (int q = 0; q < QPL; ++q)
-                    if (valid[q]) P[q] = and4(P[q], row[q * 256]);
+                    PP[q] = valid[q] ? make_uint4(~0u, ~0u, ~0u, ~0u) : make_uint4(0, 0, 0, 0);
+                const uint32_t np = v.d ? v.d - 1 : 0;  // parent rows, two in flight
+                uint32_t j = 0;
+                for (; j + 2 <= np; j += 2) {
+                    const gptr<uint4> r0 = row_of(pre[j]), r1 = row_of(pre[j + 1]);
+                    uint4 x[QPL], y[QPL];
+#pragma unroll
+                    for (int q = 0; q < QPL; ++q) {
+                        x[q] = r0[off[q]];
+                        y[q] = r1[off[q]];
+                    }
+#pragma unroll
+                    for (int q = 0; q < QPL; ++q) PP[q] = and4(PP[q], and4(x[q], y[q]));
+                }
+                if (j < np) {
+                    const gptr<uint4> r0 = row_of(pre[j]);
+#pragma unroll
+                    for (int q = 0; q < QPL; ++q) PP[q] = and4(PP[q], r0[off[q]]);
+                }
+                rows_read += np;
             }
+            pp_ok = true;
+            pp_par = par;
+            uint4 P[QPL];
+#pragma unroll
+            for (int q = 0; q < QPL; ++q) P[q] = and4(PP[q], last[q]);
+            bytes += 8ull * tile_words_of(db->words, r.tile, 2 * kQuadsPerTile) * rows_read;
             uint32_t e = e0;
             for (; e + 1 < e1; e += 2) {
-                const gptr<uint4> ra = store + (uint64_t)col_of(v.fcol, v.ext[e]) * stride_q + q0;
-                const gptr<uint4> rb = store + (uint64_t)col_of(v.fcol, v.ext[e + 1]) * stride_q + q0;
+                const gptr<uint4> ra = row_of(v.ext[e]), rb = row_of(v.ext[e + 1]);
                 uint4 A[QPL], B[QPL];
 #pragma unroll
                 for (int q = 0; q < QPL; ++q) {
-                    A[q] = make_uint4(0, 0, 0, 0);
-                    B[q] = make_uint4(0, 0, 0, 0);
-                    if (valid[q]) {
-                        A[q] = ra[q * 256];
-                        B[q] = rb[q * 256];
-                    }
+                    A[q] = ra[off[q]];
+                    B[q] = rb[off[q]];
                 }
                 uint32_t ca = 0, cb = 0;
 #pragma unroll
@@ -184,11 +233,10 @@ __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
                 }
             }
             if (e < e1) {
-                const gptr<uint4> ra = store + (uint64_t)col_of(v.fcol, v.ext[e]) * stride_q + q0;
+                const gptr<uint4> ra = row_of(v.ext[e]);
                 uint32_t ca = 0;
 #pragma unroll
-                for (int q = 0; q < QPL; ++q)
-                    if (valid[q]) ca += pop4(and4(P[q], ra[q * 256]));
+                for (int q = 0; q < QPL; ++q) ca += pop4(and4(P[q], ra[off[q]]));
                 ca = wave_sum(ca);
                 if (lane == 0) part[wave][e - r.r0] = ca;
             }
@@ -389,12 +437,25 @@ __global__ __launch_bounds__(256) void k_intersect(const uint64_t *__restrict__ 
 
 }  // namespace
 
+// Persistent grid of a support kernel, in blocks per CU (a multiple of 8 in
+// all, so tile == XCD), at most `grid`.  Measured on MI355X: the 1024-word
+// kernel holds 8 blocks per CU and runs best at 7 (a full grid waits for
+// slots the previous launch's last waves still hold); the 2048-word kernel
+// holds 4 and runs best oversubscribed at 8 (late blocks take the leftover
+// work); the periodic kernel runs best at 6.
+uint32_t persistent_grid(const cpm_ctx *ctx, uint32_t per_cu, uint32_t grid) {
+    const uint32_t g = (uint32_t)ctx->cu_count * per_cu / 8 * 8;
+    return std::max<uint32_t>(8, std::min(grid, g));
+}
+
 int launch_support_count(cpm_ctx *ctx, const SupportArgs &a, uint32_t tile_words, uint32_t grid) {
     if (a.te == 0) return fail(CPM_ERR_INVALID, "zero tiles per work range");
     if (tile_words == 1024)
-        hipLaunchKernelGGL(k_support_count<2>, dim3(grid), dim3(256), 0, ctx->stream, a);
+        hipLaunchKernelGGL(k_support_count<2>, dim3(persistent_grid(ctx, 7, grid)), dim3(256), 0,
+                           ctx->stream, a);
     else if (tile_words == 2048)
-        hipLaunchKernelGGL(k_support_count<4>, dim3(grid), dim3(256), 0, ctx->stream, a);
+        hipLaunchKernelGGL(k_support_count<4>, dim3(persistent_grid(ctx, 8, grid)), dim3(256), 0,
+                           ctx->stream, a);
     else
         return fail(CPM_ERR_INVALID, "tile of %u words", tile_words);
     CPM_HIP(hipGetLastError());
@@ -403,7 +464,8 @@ int launch_support_count(cpm_ctx *ctx, const SupportArgs &a, uint32_t tile_words
 
 int launch_support_periodic(cpm_ctx *ctx, const SupportArgs &a, uint32_t grid) {
     if (a.te == 0) return fail(CPM_ERR_INVALID, "zero tiles per work range");
-    hipLaunchKernelGGL(k_support_periodic, dim3(grid), dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_support_periodic, dim3(persistent_grid(ctx, 6, grid)), dim3(256), 0,
+                       ctx->stream, a);
     CPM_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_periodic_finalize, dim3(grid), dim3(256), 0, ctx->stream, a);
     CPM_HIP(hipGetLastError());

@@ -43,6 +43,7 @@ struct LevelSizes {
 struct LevelPtrs {
     uint32_t *prefix = nullptr;
     uint32_t *pid = nullptr;
+    uint32_t *par = nullptr;  // parent task: its index in the previous level (levels >= 3)
     uint32_t *db = nullptr;
     uint32_t *ext_off = nullptr;
     uint32_t *ext = nullptr;
@@ -84,6 +85,9 @@ struct SupportArgs {
     // per-block stamps (start, end, bytes, candidates), one set per grid
     // block; null to skip
     unsigned long long *stamps = nullptr;
+    // L.par holds each task's parent: tasks with one parent share the AND
+    // of their first d - 1 prefix rows
+    int siblings = 0;
 };
 
 // Entries per work range for a level of n_ext entries: at most G, and small
@@ -118,5 +122,6 @@ inline uint32_t tiles_per_row(uint64_t stride, uint32_t tile_words = kTileWords)
 
 // persistent grid for the support kernels on this device (a multiple of 8)
 inline uint32_t support_grid(const cpm_ctx *ctx) { return (uint32_t)ctx->cu_count * 8; }
+
 
 }  // namespace cpm

@@ -206,6 +206,13 @@ int cpm_patterns_download(const cpm_patterns *p, uint32_t *parent, uint32_t *ite
 int cpm_patterns_destroy(cpm_patterns *p);
 /* Database index of each pattern (0 for single-database results). */
 int cpm_patterns_databases(const cpm_patterns *p, uint32_t *db);
+/* The forest's own arrays (count elements each), valid until
+ * cpm_patterns_destroy: period is NULL unless the result is periodic, db is
+ * NULL when the result does not record databases, all are NULL for an empty
+ * result.  No counterpart in the reference (it builds Python dicts); for
+ * callers that read large results without copying them. */
+int cpm_patterns_view(const cpm_patterns *p, const uint32_t **parent, const uint32_t **item,
+                      const uint64_t **support, const int64_t **period, const uint32_t **db);
 
 /* ---- measurement -------------------------------------------------------- */
 
