@@ -225,11 +225,13 @@ __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
                     ca += pop4(and4(P[q], A[q]));
                     cb += pop4(and4(P[q], B[q]));
                 }
-                ca = wave_sum(ca);
-                cb = wave_sum(cb);
+                // one reduction for both: a wave's count is at most
+                // 64 lanes x 128 x QPL bits < 2^16 for QPL <= 4
+                static_assert(64 * 128 * QPL < 65536, "packed wave counts");
+                const uint32_t cab = wave_sum(ca | (cb << 16));
                 if (lane == 0) {
-                    part[wave][e - r.r0] = ca;
-                    part[wave][e + 1 - r.r0] = cb;
+                    part[wave][e - r.r0] = cab & 0xffffu;
+                    part[wave][e + 1 - r.r0] = cab >> 16;
                 }
             }
             if (e < e1) {
