@@ -268,8 +268,9 @@ struct GramJob {
 // Work table for a batch of databases; *out_elems = sum of n_items^2.
 int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<GramJob> &jobs,
               uint64_t *out_elems);
-// Zero `out` and run the jobs (device tables); n_ops feeds the statistics.
+// Run the jobs (device tables) into `out`, zeroing it first when `clear` (the
+// kernel adds its K slices); n_ops feeds the statistics.
 int launch_gram(cpm_ctx *ctx, const GramJob *d_jobs, uint64_t n_jobs, const GramDb *d_dbs,
-                   int32_t *d_out, uint64_t out_elems, uint64_t n_ops);
+                int32_t *d_out, uint64_t out_elems, uint64_t n_ops, bool clear);
 
 }  // namespace cpm

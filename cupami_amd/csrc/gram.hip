@@ -225,8 +225,8 @@ int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<Gr
 }
 
 int launch_gram(cpm_ctx *ctx, const GramJob *d_jobs, uint64_t n_jobs, const GramDb *d_dbs,
-                   int32_t *d_out, uint64_t out_elems, uint64_t n_ops) {
-    CPM_HIP(hipMemsetAsync(d_out, 0, out_elems * sizeof(int32_t), ctx->stream));
+                int32_t *d_out, uint64_t out_elems, uint64_t n_ops, bool clear) {
+    if (clear) CPM_HIP(hipMemsetAsync(d_out, 0, out_elems * sizeof(int32_t), ctx->stream));
     if (n_jobs == 0) return CPM_OK;
     Timed timer(ctx, 5);
     hipLaunchKernelGGL(k_gram, dim3((unsigned)n_jobs), dim3(kThreads), 0, ctx->stream, d_jobs, d_dbs,
@@ -274,8 +274,8 @@ extern "C" int cpm_pair_gram(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t 
                                hipMemcpyHostToDevice, ctx->stream));
     CPM_HIP(hipMemcpyAsync(ddb.p, dbs.data(), sizeof(GramDb), hipMemcpyHostToDevice, ctx->stream));
     const uint64_t ops = (uint64_t)bits->n_transactions * n * (uint64_t)(n + 1);
-    CPM_CHECK(launch_gram(ctx, djobs.as<GramJob>(), jobs.size(), ddb.as<GramDb>(),
-                             dout.as<int32_t>(), elems, ops));
+    CPM_CHECK(launch_gram(ctx, djobs.as<GramJob>(), jobs.size(), ddb.as<GramDb>(), dout.as<int32_t>(), elems,
+                          ops, true));
     CPM_CHECK(copy_to_host(ctx, gram, dout.p, (size_t)n * n * 4));
     return CPM_OK;
 }

@@ -950,12 +950,6 @@ __global__ __launch_bounds__(256) void k_scatter(const PutRec *__restrict__ recs
     }
 }
 
-__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = v;
-}
-
 // ---- host side ---------------------------------------------------------------------
 
 struct LevelSet {
@@ -1474,6 +1468,8 @@ struct Search {
             return CPM_OK;
         }
         LevelSet &S1 = w.set[1];
+        // Level 2's tables, the Gram's job tables and the cleared Gram output
+        // and pair tables go to the device in one queued flush.
         CPM_CHECK(w.l2.ensure(n_db * sizeof(L2Db)));
         qput(w.l2.p, l2.data(), n_db * sizeof(L2Db));
         {
@@ -1488,18 +1484,16 @@ struct Search {
             qput(sz(1), &ls, sizeof(ls));
             const uint32_t e2u = (uint32_t)e2;
             qput(S1.ext_off.as<uint32_t>() + t2, &e2u, 4);
-            CPM_CHECK(flush());
-            hipLaunchKernelGGL(k_level2, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0, st,
-                               w.l2.as<L2Db>(), n_db, (uint32_t)t2, S1.ptrs(sz(1)), w.dbs.as<DbDev>());
-            CPM_HIP(hipGetLastError());
         }
         bool gram_ok = use_gram && !periodic;
         for (uint32_t i = 0; i < n_db && gram_ok; ++i)
             if (nf[i] > 16384 || dbs[i].n >= (1ull << 31)) gram_ok = false;
+        std::vector<GramJob> jobs;
+        uint64_t gram_elems = 0, gram_ops = 0;
         if (gram_ok) {
             std::vector<GramDb> gdb(n_db);
             std::vector<uint64_t> goff(n_db);
-            uint64_t off = 0, ops = 0;
+            uint64_t off = 0;
             for (uint32_t i = 0; i < n_db; ++i) {
                 gdb[i].store = dbs[i].store;
                 gdb[i].stride = dbs[i].stride;
@@ -1510,29 +1504,18 @@ struct Search {
                 gdb[i].out_off = off;
                 goff[i] = off;
                 off += (uint64_t)gdb[i].n_items * gdb[i].n_items;
-                ops += dbs[i].n * (uint64_t)gdb[i].n_items * (gdb[i].n_items + 1);
+                gram_ops += dbs[i].n * (uint64_t)gdb[i].n_items * (gdb[i].n_items + 1);
             }
-            std::vector<GramJob> jobs;
-            uint64_t elems = 0;
-            CPM_CHECK(gram_plan(ctx, gdb, jobs, &elems));
-            CPM_CHECK(w.gram.ensure(std::max<uint64_t>(elems, 1) * 4));
+            CPM_CHECK(gram_plan(ctx, gdb, jobs, &gram_elems));
+            CPM_CHECK(w.gram.ensure(std::max<uint64_t>(gram_elems, 1) * 4));
             CPM_CHECK(w.goff.ensure(n_db * 8));
             CPM_CHECK(w.gjobs.ensure(std::max<size_t>(jobs.size(), 1) * sizeof(GramJob)));
             CPM_CHECK(w.gdbs.ensure(n_db * sizeof(GramDb)));
             qput(w.goff.p, goff.data(), n_db * 8);
             qput(w.gjobs.p, jobs.data(), jobs.size() * sizeof(GramJob));
             qput(w.gdbs.p, gdb.data(), n_db * sizeof(GramDb));
-            CPM_CHECK(flush());
-            CPM_CHECK(launch_gram(ctx, w.gjobs.as<GramJob>(), jobs.size(), w.gdbs.as<GramDb>(),
-                                     w.gram.as<int32_t>(), elems, ops));
-            hipLaunchKernelGGL(k_gram_to_level2, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0,
-                               st, S1.ptrs(sz(1)), w.dbs.as<DbDev>(), w.gram.as<int32_t>(),
-                               w.goff.as<uint64_t>());
-            CPM_HIP(hipGetLastError());
-        } else {
-            CPM_CHECK(count(1, false));
+            qzero(w.gram.p, gram_elems * 4);  // the Gram adds its K slices into it
         }
-        // pair tables
         tmask_w = 0;
         if (pair_words) {
             // per-task kept masks for plan/fill, when they fit a modest budget
@@ -1543,14 +1526,30 @@ struct Search {
                 CPM_CHECK(w.tmask.ensure((w.caps.tasks + 1) * wmax * 4));
             }
             CPM_CHECK(w.pairbits.ensure(pair_words * 4));
-            hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<uint64_t>((pair_words + 255) / 256, 16384)),
-                               dim3(256), 0, st, w.pairbits.as<uint32_t>(), pair_words, 0u);
+            qzero(w.pairbits.p, pair_words * 4);
+        } else {
+            w.pairbits.release();
+        }
+        CPM_CHECK(flush());
+
+        hipLaunchKernelGGL(k_level2, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0, st,
+                           w.l2.as<L2Db>(), n_db, (uint32_t)t2, S1.ptrs(sz(1)), w.dbs.as<DbDev>());
+        CPM_HIP(hipGetLastError());
+        if (gram_ok) {
+            CPM_CHECK(launch_gram(ctx, w.gjobs.as<GramJob>(), jobs.size(), w.gdbs.as<GramDb>(),
+                                  w.gram.as<int32_t>(), gram_elems, gram_ops, false));
+            hipLaunchKernelGGL(k_gram_to_level2, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0,
+                               st, S1.ptrs(sz(1)), w.dbs.as<DbDev>(), w.gram.as<int32_t>(),
+                               w.goff.as<uint64_t>());
+            CPM_HIP(hipGetLastError());
+        } else {
+            CPM_CHECK(count(1, false));
+        }
+        if (pair_words) {
             PlanArgs p = plan_args(1, 1);
             hipLaunchKernelGGL(k_pair_table, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0, st, p,
                                w.pairbits.as<uint32_t>());
             CPM_HIP(hipGetLastError());
-        } else {
-            w.pairbits.release();
         }
 
         // ---- levels >= 2: replayed steps ----
