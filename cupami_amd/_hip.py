@@ -495,11 +495,8 @@ class Patterns:
     def itemsets(self):
         """Column tuples, one per pattern (parents precede children)."""
         out = [None] * self.n
-        parent = self.parent.tolist()
-        item = self.item.tolist()
-        for i in range(self.n):
-            p = parent[i]
-            out[i] = (item[i],) if p == NO_PARENT else out[p] + (item[i],)
+        for i, p, c in zip(range(self.n), self.parent.tolist(), self.item.tolist()):
+            out[i] = (c,) if p == NO_PARENT else out[p] + (c,)
         return out
 
     def split(self, n_db):
@@ -507,7 +504,7 @@ class Patterns:
         sets = self.itemsets()
         out = [dict() for _ in range(n_db)]
         for s, v, d in zip(sets, self.support.tolist(), self.db.tolist()):
-            out[d][s] = int(v)
+            out[d][s] = v
         return out
 
     def to_dict(self, labels=None):
@@ -519,9 +516,8 @@ class Patterns:
             sets = [tuple(lab[c] for c in s) for s in sets]
         sup = self.support.tolist()
         if self.periodic:
-            per = self.period.tolist()
-            return {s: (int(a), int(b)) for s, a, b in zip(sets, sup, per)}
-        return {s: int(a) for s, a in zip(sets, sup)}
+            return dict(zip(sets, zip(sup, self.period.tolist())))
+        return dict(zip(sets, sup))
 
 
 _contexts = {}
