@@ -288,12 +288,17 @@ def run_batch(args):
     miner = BatchMiner(ctx)
     config, table = miner.tune(specs, minsups)
     log(f"bench: autotune {({f'{g}x{w}': round(v * 1e3, 2) for (g, w), v in table.items()})} -> {config}")
+    # each step releases the previous result first, as a consumer that hands
+    # results on would (its host blocks are then reused)
+    pats = None
     for _ in range(args.warmup):
+        pats = None
         pats = miner.mine(specs, minsups)
     ctx.reset_stats()
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        pats = None
         pats = miner.mine(specs, minsups)
     ctx.synchronize()
     batched_s = (time.perf_counter() - t0) / args.steps

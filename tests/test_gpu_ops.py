@@ -264,3 +264,39 @@ def test_batch_equals_separate_searches(ctx):
     got = ctx.eclat_mine_batch(stores, mins2, use_gram=False).split(len(stores))
     for i in range(len(stores)):
         assert got[i] == oracle.eclat(hosts[i], mins2[i]), i
+
+
+def test_large_results_views_outlive_their_patterns(ctx):
+    """Results of 262144+ patterns live in recycled host blocks and reach
+    Python as views: a view keeps its result alive after the Patterns object
+    is gone, later searches (which reuse freed blocks) do not disturb it, and
+    their own results stay exact."""
+    import gc
+    rng = np.random.default_rng(808)
+    n, items = 5000, 20
+    dense = rng.random((n, items)) < 0.8
+    rows, cols = np.nonzero(dense)
+    rows = rows.astype(np.uint64)
+    cols = cols.astype(np.uint32)
+    bits = ctx.build_bits(rows, cols, n, items)
+    host = oracle.bits_build(rows, cols, n, items)
+    first = ctx.eclat_mine(bits, 600)
+    assert len(first) > 262144
+    expect_first = oracle.eclat(host, 600)
+    assert first.to_dict() == expect_first
+    support = first.support
+    kept = support.copy()
+    del first
+    gc.collect()
+    for minsup in (700, 650):
+        again = ctx.eclat_mine(bits, minsup)
+        assert again.to_dict() == oracle.eclat(host, minsup)
+        del again
+        gc.collect()
+    assert np.array_equal(support, kept)
+    del support
+    gc.collect()
+    last = ctx.eclat_mine_batch([bits, bits], [600, 800])
+    parts = last.split(2)
+    assert parts[0] == expect_first
+    assert parts[1] == oracle.eclat(host, 800)
