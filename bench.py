@@ -55,14 +55,19 @@ def measured_traffic():
     return float(fields["k_support_count_hbm_bytes_per_launch"])
 
 
-def csrc_digest():
-    """SHA-256 over the kernel sources (names and contents, sorted)."""
+def csrc_digest(root=None):
+    """SHA-256 over the kernel sources (names and code, sorted), with comments
+    removed and whitespace runs folded, so that editing a comment keeps a
+    measurement valid while any change to the code does not."""
     import hashlib
+    import re
+    token = re.compile(r'"(?:\\.|[^"\\])*"|\'(?:\\.|[^\'\\])*\'|//[^\n]*|/\*.*?\*/', re.S)
     h = hashlib.sha256()
-    for path in sorted((ROOT / "cupami_amd" / "csrc").iterdir()):
+    for path in sorted(((root or ROOT) / "cupami_amd" / "csrc").iterdir()):
         if path.suffix in (".hip", ".h"):
+            text = token.sub(lambda m: m.group(0) if m.group(0)[0] in "\"'" else " ", path.read_text())
             h.update(path.name.encode())
-            h.update(path.read_bytes())
+            h.update(" ".join(text.split()).encode())
     return h.hexdigest()
 
 

@@ -3,9 +3,10 @@
 // A level's entries are cut into work items (a range of G consecutive
 // entries, a tile of words).  A 256-lane workgroup walks the tasks its range
 // touches; for each it builds its tile of the task's prefix in registers (the
-// AND of the d prefix item rows), then streams the task's extension rows of
-// the range over it: AND, popcount, wave reduction, one atomic per entry per
-// workgroup.  The grid is persistent and a multiple of 8 and the tiles per
+// AND of the d prefix item rows -- of only the last one when the previous task
+// was a sibling, whose parent part is kept), then streams the task's
+// extension rows of the range over it: AND, popcount, wave reduction, one
+// atomic per entry per workgroup.  The grid is persistent and a multiple of 8 and the tiles per
 // range are too (when rows have 8 or more tiles), so a block -- which stays
 // on XCD blockIdx % 8 -- only ever touches tiles congruent to that XCD: each
 // XCD's L2 holds its own slice of the hot rows.  Sizes are read from device
