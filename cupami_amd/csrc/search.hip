@@ -32,6 +32,8 @@
 
 #include <string.h>
 #include <algorithm>
+#include <system_error>
+#include <thread>
 
 namespace cpm {
 
@@ -40,6 +42,7 @@ namespace {
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kMaskWords = 2048;   // LDS kept mask: up to 65536 frequent items per database
 constexpr uint64_t kTaskMaskBudget = 256ull << 20;  // bytes of per-task kept masks
+constexpr uint64_t kParallelCopyOut = 1 << 20;      // patterns from which the host copy-out uses threads
 
 enum Overflow : uint32_t {
     kOvTasks = 1, kOvEntries = 2, kOvPatterns = 4, kOvDepth = 8, kOvParts = 32,
@@ -75,7 +78,7 @@ struct PlanArgs {
     uint32_t stamp_blocks;
     uint32_t *tmask;            // kept-item mask of each task (tmask_w words apart), null:
     uint32_t tmask_w;           //   plan/write writes the child entries itself
-    uint32_t *pat_parent, *pat_item, *pat_sup;
+    uint32_t *pat_parent, *pat_item, *pat_sup, *pat_db;
     int64_t *pat_per;
     Ctl *ctl;
     Caps caps;                  // capacities of the next level set
@@ -538,6 +541,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             p.pat_parent[pb + lane] = p.L.pid[t];
             p.pat_item[pb + lane] = p.fcol ? p.fcol[f] : f;
             p.pat_sup[pb + lane] = p.L.sup[e];
+            p.pat_db[pb + lane] = dbi;
             if (p.pat_per) p.pat_per[pb + lane] = p.L.per[e];
             if (stage) {
                 uint64_t h = kSetSeed;
@@ -613,6 +617,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             p.pat_parent[pb + j] = parent;
             p.pat_item[pb + j] = p.fcol ? p.fcol[f] : f;
             p.pat_sup[pb + j] = p.L.sup[e];
+            p.pat_db[pb + j] = dbi;
             if (p.pat_per) p.pat_per[pb + j] = p.L.per[e];
             if (stage) {
                 uint64_t h = kSetSeed;
@@ -1006,7 +1011,7 @@ struct GraphKey {
 struct SearchWs {
     LevelSet set[2];
     DBuf ctl, dbs, fcol, pairbits, tmask, gram, goff, gjobs, gdbs, l2, part, partl, scatter, ts, stamps;
-    DBuf pat;             // pattern forest: parent, item and support columns, pat_pitch() bytes apart
+    DBuf pat;             // pattern forest: parent, item, support and database columns, pat_pitch() bytes apart
     DBuf filter;          // subset-test Bloom filter (filter_words() words)
     bool filter_dirty = true;  // holds bits of an attempt that stopped early
     DBuf pat_per;
@@ -1102,6 +1107,7 @@ struct Search {
         p.pat_parent = pat_col(0);
         p.pat_item = pat_col(1);
         p.pat_sup = pat_col(2);
+        p.pat_db = pat_col(3);
         p.pat_per = periodic ? w.pat_per.as<int64_t>() : nullptr;
         p.ctl = ctl();
         p.caps = w.caps;
@@ -1228,7 +1234,7 @@ struct Search {
 
     int reserve() {
         for (int i = 0; i < 2; ++i) CPM_CHECK(w.set[i].reserve(w.caps, periodic, parts_tiles));
-        CPM_CHECK(w.pat.ensure(3 * pat_pitch()));
+        CPM_CHECK(w.pat.ensure(4 * pat_pitch()));
         {
             void *before = w.filter.p;
             CPM_CHECK(w.filter.ensure(filter_words() * 8));
@@ -1623,17 +1629,17 @@ struct Search {
         hipStream_t st = ctx->stream;
         const unsigned char *h = nullptr;
         if (n) {
-            const size_t row = periodic ? 20 : 12;
+            const size_t row = periodic ? 24 : 16;
             int sc = w.down.ensure(n * row);
             if (sc != CPM_OK) {
                 delete p;
                 return sc;
             }
             unsigned char *hd = w.down.as<unsigned char>();
-            // parent, item and support columns in one strided copy
-            hipError_t e = hipMemcpy2DAsync(hd, n * 4, w.pat.p, pat_pitch(), n * 4, 3, hipMemcpyDeviceToHost, st);
+            // parent, item, support and database columns in one strided copy
+            hipError_t e = hipMemcpy2DAsync(hd, n * 4, w.pat.p, pat_pitch(), n * 4, 4, hipMemcpyDeviceToHost, st);
             if (e == hipSuccess && periodic)
-                e = hipMemcpyAsync(hd + n * 12, w.pat_per.p, n * 8, hipMemcpyDeviceToHost, st);
+                e = hipMemcpyAsync(hd + n * 16, w.pat_per.p, n * 8, hipMemcpyDeviceToHost, st);
             if (e == hipSuccess) e = hipStreamSynchronize(st);
             if (e != hipSuccess) {
                 delete p;
@@ -1642,8 +1648,8 @@ struct Search {
             h = hd;
         }
         try {
-            // one pass over the pinned copy into the result (its arrays come
-            // from recycled host blocks and are not cleared first)
+            // the result's arrays come from recycled host blocks and are not
+            // cleared first: every element is written below
             p->parent.resize(n);
             p->item.resize(n);
             p->support.resize(n);
@@ -1655,22 +1661,31 @@ struct Search {
         }
         if (n) {
             const uint32_t *hp = reinterpret_cast<const uint32_t *>(h);
-            memcpy(p->parent.data(), hp, n * 4);
-            memcpy(p->item.data(), hp + n, n * 4);
-            if (periodic) memcpy(p->period.data(), h + n * 12, n * 8);
-            const uint32_t *hs = hp + 2 * n;
-            // database of each pattern: singletons by range, others by parent
-            uint32_t di = 0;
-            for (uint64_t i = 0; i < n; ++i) {
-                p->support[i] = hs[i];
-                const uint32_t par = hp[i];
-                if (par == kNone) {
-                    while (di + 1 < dbs.size() && i >= (uint64_t)dbs[di].fbase + dbs[di].nf) ++di;
-                    p->db[i] = di;
-                } else {
-                    p->db[i] = p->db[par];
+            // independent column copies: on four threads for large results
+            auto widen = [&](uint64_t lo, uint64_t hi) {
+                for (uint64_t i = lo; i < hi; ++i) p->support[i] = hp[2 * n + i];
+            };
+            auto parts = [&](int k) {
+                switch (k) {
+                case 0: memcpy(p->parent.data(), hp, n * 4); break;
+                case 1: memcpy(p->item.data(), hp + n, n * 4); break;
+                case 2: memcpy(p->db.data(), hp + 3 * n, n * 4); break;
+                default:
+                    widen(0, n);
+                    if (periodic) memcpy(p->period.data(), h + n * 16, n * 8);
+                }
+            };
+            std::thread helpers[3];
+            int started = 0;
+            if (n >= kParallelCopyOut) {
+                try {
+                    for (; started < 3; ++started) helpers[started] = std::thread(parts, started);
+                } catch (const std::system_error &) {
+                    // no thread: this one does the rest
                 }
             }
+            for (int k = started; k < 4; ++k) parts(k);
+            for (int k = 0; k < started; ++k) helpers[k].join();
         }
         // statistics
         int wall_khz = 0;
