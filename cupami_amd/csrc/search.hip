@@ -465,7 +465,9 @@ __device__ __forceinline__ uint32_t upper_task(const uint32_t *__restrict__ off,
 //
 // The itemsets of the level being planned go into a blocked Bloom filter:
 // a 64-bit hash of the itemset picks one 64-bit word and four bits in it,
-// set with one atomicOr (nothing waits for it).  The filter has 32 bits per
+// set with one atomicOr (nothing waits for it).  The hash of a set is the
+// sum of a mix of each item (then mixed once more), so a subset's hash is the
+// whole set's minus one term: a candidate's d - 1 subset hashes cost O(d).  The filter has 32 bits per
 // itemset (a power of two), which keeps false positives near 0.1%.  A child
 // candidate survives when all its subsets test present.  A false positive can
 // only keep a candidate that should have been pruned; its support, counted
@@ -478,10 +480,10 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
     return z ^ (z >> 31);
 }
-constexpr uint64_t kSetSeed = 0x243f6a8885a308d3ull;
-__device__ __forceinline__ uint64_t set_step(uint64_t h, uint32_t item) {
-    return mix64(h ^ (item + 0x9e3779b97f4a7c15ull));
-}
+// one item's term of a set hash
+__device__ __forceinline__ uint64_t item_term(uint32_t item) { return mix64(item + 0x9e3779b97f4a7c15ull); }
+// filter hash of a set from the sum of its items' terms
+__device__ __forceinline__ uint64_t set_hash(uint64_t sum) { return mix64(sum ^ 0x243f6a8885a308d3ull); }
 // words of the filter for n_pat itemsets, less one: next_pow2(32 n_pat) bits
 __device__ __forceinline__ uint64_t filter_mask(uint64_t n_pat) {
     const uint64_t bits = n_pat <= 2 ? 64 : 1ull << (64 - __clzll(32 * n_pat - 1));
@@ -495,11 +497,6 @@ __device__ __forceinline__ unsigned long long filter_bits(uint64_t h) {
 __device__ __forceinline__ void filter_insert(unsigned long long *f, uint64_t mask, uint64_t h) {
     atomicOr(f + (h & mask), filter_bits(h));
 }
-__device__ __forceinline__ bool filter_test(const unsigned long long *f, uint64_t mask, uint64_t h) {
-    const unsigned long long b = filter_bits(h);
-    return (f[h & mask] & b) == b;
-}
-
 __device__ __forceinline__ bool prune_active(const PlanArgs &p) {
     return p.prune && !p.ctl->overflow && p.emit_children && p.L.sz->d >= 2 && p.N.sz->n_tasks > 0;
 }
@@ -544,9 +541,9 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             p.pat_db[pb + lane] = dbi;
             if (p.pat_per) p.pat_per[pb + lane] = p.L.per[e];
             if (stage) {
-                uint64_t h = kSetSeed;
-                for (uint32_t i = 0; i < d; ++i) h = set_step(h, src[i]);
-                filter_insert(p.filter, fmask, set_step(h, f));
+                uint64_t sum = item_term(f);
+                for (uint32_t i = 0; i < d; ++i) sum += item_term(src[i]);
+                filter_insert(p.filter, fmask, set_hash(sum));
             }
         }
         if (!(children && nk >= 2)) continue;
@@ -620,9 +617,9 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             p.pat_db[pb + j] = dbi;
             if (p.pat_per) p.pat_per[pb + j] = p.L.per[e];
             if (stage) {
-                uint64_t h = kSetSeed;
-                for (uint32_t i = 0; i < d; ++i) h = set_step(h, src[i]);
-                filter_insert(p.filter, fmask, set_step(h, f));
+                uint64_t sum = item_term(f);
+                for (uint32_t i = 0; i < d; ++i) sum += item_term(src[i]);
+                filter_insert(p.filter, fmask, set_hash(sum));
             }
         }
         if (!(children && nk >= 2)) continue;
@@ -767,14 +764,25 @@ __global__ __launch_bounds__(256) void k_prune_flags(PlanArgs p) {
         const uint32_t ct = upper_task(p.off2, n_child, w);
         const uint32_t *pre = p.N.prefix + (uint64_t)ct * kMaxDepth;
         const uint32_t x = p.ext2[w];
-        bool ok = true;
         // the subsets without the last prefix item or without x are kept by
-        // construction; test the ones without each earlier prefix item
-        for (uint32_t drop = 0; ok && drop + 1 < d1; ++drop) {
-            uint64_t h = kSetSeed;
-            for (uint32_t i = 0; i < d1; ++i)
-                if (i != drop) h = set_step(h, pre[i]);
-            ok = filter_test(p.filter, fmask, set_step(h, x));
+        // construction; test the ones without each earlier prefix item, four
+        // probes in flight at a time
+        uint64_t all = item_term(x);
+        for (uint32_t i = 0; i < d1; ++i) all += item_term(pre[i]);
+        bool ok = true;
+        for (uint32_t d0 = 0; ok && d0 + 1 < d1; d0 += 4) {
+            unsigned long long word[4], bits[4];
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                word[k] = bits[k] = 0;
+                if (d0 + k + 1 < d1) {  // the same for every lane (one prefix length per level)
+                    const uint64_t h = set_hash(all - item_term(pre[d0 + k]));
+                    bits[k] = filter_bits(h);
+                    word[k] = p.filter[h & fmask];
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) ok = ok && (word[k] & bits[k]) == bits[k];
         }
         p.live[w] = ok ? 1u : 0u;
         c += ok ? 1u : 0u;
