@@ -705,26 +705,27 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
 // largest tasks would otherwise decide the launch's time).  Child ct of task
 // t with item f takes the kept items above f whose pair with f is kept: the
 // set bits of tmask[t] AND f's pair-table row, in ascending order.
-__global__ __launch_bounds__(256) void k_child_fill(PlanArgs p) {
-    if (p.ctl->overflow || !p.tmask || !p.pairbits || !p.emit_children) return;
-    const uint32_t n_child = (uint32_t)p.N.sz->n_tasks;
-    if (n_child == 0) return;
-    const uint32_t n_tasks = (uint32_t)p.L.sz->n_tasks;
-    const uint32_t d = (uint32_t)p.L.sz->d;
-    const bool stage = prune_active(p);
+// Child entries: a group of S lanes per child (S = the widest kept-mask row,
+// rounded up to 8, 16, 32 or 64 words), so that short rows do not leave most
+// of a wave idle.  Lanes cover mask words; a scan within the group places the
+// items.
+template <uint32_t S>
+__device__ __forceinline__ void child_fill_groups(const PlanArgs &p, uint32_t n_child, uint32_t n_tasks, uint32_t d,
+                                                  bool stage) {
     uint32_t *const xdst = stage ? p.ext2 : p.N.ext;
     const uint32_t *const odst = stage ? p.off2 : p.N.ext_off;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t ct = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); ct < n_child; ct += waves) {
+    constexpr uint32_t G = 64 / S;  // children per wave
+    const uint32_t lane = threadIdx.x & 63, sub = lane % S, g = lane / S;
+    const uint32_t groups = gridDim.x * (blockDim.x >> 6) * G;
+    for (uint32_t ct = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * G + g; ct < n_child; ct += groups) {
         const uint32_t t = upper_task(p.off_child, n_tasks, ct);
         const DbDev &D = p.dbs[p.L.db[t]];
         const uint32_t fl = p.N.prefix[(uint64_t)ct * kMaxDepth + d] - D.fbase;
         const uint32_t *mask = p.tmask + (uint64_t)t * p.tmask_w;
         const uint32_t *prow = p.pairbits + D.pair_off + (uint64_t)fl * D.pair_w32;
         uint32_t w = odst[ct];
-        for (uint32_t q0 = fl >> 5; q0 < D.pair_w32; q0 += 64) {
-            const uint32_t q = q0 + lane;
+        for (uint32_t q0 = fl >> 5; q0 < D.pair_w32; q0 += S) {
+            const uint32_t q = q0 + sub;
             uint32_t m = 0;
             if (q < D.pair_w32) {
                 m = mask[q] & prow[q];
@@ -733,9 +734,9 @@ __global__ __launch_bounds__(256) void k_child_fill(PlanArgs p) {
             const uint32_t c = __popc(m);
             uint32_t x = c;
 #pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(x, o, 64);
-                if (lane >= (uint32_t)o) x += y;
+            for (uint32_t o = 1; o < S; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o, S);
+                if (sub >= o) x += y;
             }
             uint32_t pos = w + x - c;
             while (m) {
@@ -744,9 +745,22 @@ __global__ __launch_bounds__(256) void k_child_fill(PlanArgs p) {
                 ++pos;
                 m &= m - 1;
             }
-            w += __shfl(x, 63, 64);
+            w += __shfl(x, S - 1, S);
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_child_fill(PlanArgs p) {
+    if (p.ctl->overflow || !p.tmask || !p.pairbits || !p.emit_children) return;
+    const uint32_t n_child = (uint32_t)p.N.sz->n_tasks;
+    if (n_child == 0) return;
+    const uint32_t n_tasks = (uint32_t)p.L.sz->n_tasks;
+    const uint32_t d = (uint32_t)p.L.sz->d;
+    const bool stage = prune_active(p);
+    if (p.tmask_w > 32) child_fill_groups<64>(p, n_child, n_tasks, d, stage);
+    else if (p.tmask_w > 16) child_fill_groups<32>(p, n_child, n_tasks, d, stage);
+    else if (p.tmask_w > 8) child_fill_groups<16>(p, n_child, n_tasks, d, stage);
+    else child_fill_groups<8>(p, n_child, n_tasks, d, stage);
 }
 
 // Is (prefix without item `drop`) + {x} -- d1 items, ascending -- in lv?
