@@ -45,6 +45,11 @@ SIGNATURES = {
                                                    ctypes.POINTER(_vp)]),
     "cpm_bits_build_batch": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                             _u64p, _u64p, _u32p, ctypes.POINTER(_vp)]),
+    "cpm_bits_build_csr": (ctypes.c_int, [_vp, _u64p, ctypes.c_uint64, _u32p, ctypes.c_uint64,
+                                          ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "cpm_bits_build_csr_device": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64,
+                                                 ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "cpm_bits_labels": (ctypes.c_int, [_vp, _vp, _u32p]),
     "cpm_bits_destroy": (ctypes.c_int, [_vp]),
     "cpm_bits_shape": (ctypes.c_int, [_vp, _u64p, _u32p, _u64p, _u64p]),
     "cpm_bits_download": (ctypes.c_int, [_vp, _vp, _u64p]),
@@ -189,6 +194,29 @@ class Context:
                                            int(n_transactions), int(n_items),
                                            ctypes.byref(handle)))
         return Bits(self, handle)
+
+    def build_bits_csr(self, offsets, ids, universe, keep_empty=False):
+        """Store straight from a basket file's arrays (``offsets`` u64[n+1],
+        ``ids`` u32[m]).  Returns the store and the item id of each column
+        (ids in decimal-string order)."""
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        if offsets.size < 1:
+            raise ValueError("Invalid BIN file: invalid transaction offsets")
+        handle = _vp()
+        check(load().cpm_bits_build_csr(self.handle, _p(offsets, _u64p), offsets.size - 1, _p(ids, _u32p),
+                                        ids.size, int(universe), int(bool(keep_empty)), ctypes.byref(handle)))
+        bits = Bits(self, handle)
+        return bits, bits.labels()
+
+    def build_bits_csr_device(self, d_offsets, n_baskets, d_ids, n_ids, universe, keep_empty=False):
+        """``build_bits_csr`` with the arrays given as device addresses (ints)."""
+        handle = _vp()
+        check(load().cpm_bits_build_csr_device(self.handle, _vp(d_offsets), int(n_baskets), _vp(d_ids),
+                                               int(n_ids), int(universe), int(bool(keep_empty)),
+                                               ctypes.byref(handle)))
+        bits = Bits(self, handle)
+        return bits, bits.labels()
 
     def build_bits_device_batch(self, specs):
         """Several stores in one call: ``specs`` is a sequence of
@@ -428,6 +456,13 @@ class Bits:
         check(load().cpm_bits_shape(self.handle, ctypes.byref(n), ctypes.byref(i),
                                     ctypes.byref(w), ctypes.byref(s)))
         return n.value, i.value, w.value, s.value
+
+    def labels(self):
+        """Item id of each column (stores built from basket arrays)."""
+        _, items, _, _ = self.shape()
+        out = np.zeros(items, dtype=np.uint32)
+        check(load().cpm_bits_labels(self.ctx.handle, self.handle, _p(out, _u32p)))
+        return out
 
     def download(self):
         _, items, words, _ = self.shape()

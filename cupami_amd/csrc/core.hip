@@ -275,6 +275,8 @@ int cpm_ctx_destroy(cpm_ctx *ctx) {
     ctx->bounce.release();
     if (ctx->search_ws_free) ctx->search_ws_free(ctx->search_ws);
     ctx->search_ws = nullptr;
+    if (ctx->ingest_ws_free) ctx->ingest_ws_free(ctx->ingest_ws);
+    ctx->ingest_ws = nullptr;
     ctx->spare_store.release();
     ctx->spare_arena.release();
     for (auto &t : ctx->ev_pending) {
@@ -295,8 +297,9 @@ int cpm_ctx_synchronize(cpm_ctx *ctx) {
 
 void *cpm_ctx_stream(cpm_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
-static int bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits **out,
-                      bool clear = true) {
+}  // extern "C"
+
+int cpm::bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits **out, bool clear) {
     cpm_bits *b = new cpm_bits();
     b->ctx = ctx;
     b->n_transactions = n_trans;
@@ -321,8 +324,6 @@ static int bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits
     *out = b;
     return CPM_OK;
 }
-
-}  // extern "C"
 
 namespace cpm {
 
@@ -481,11 +482,6 @@ struct BuildIn {
     uint64_t nnz, n_trans;
     uint32_t n_items;
 };
-
-// words per workgroup of the LDS tile build (0: too many items for it)
-uint32_t tile_words_for(uint32_t n_items) {
-    return n_items ? std::min<uint32_t>(8, 8192 / std::max<uint32_t>(n_items, 1)) : 0;
-}
 
 // Build n stores from device COO.  The sortedness and range checks of the
 // whole batch are read back in one copy, so a batch costs one host

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <stdint.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -64,6 +65,12 @@ inline uint64_t row_stride(uint64_t words) { return (words + 63) / 64 * 64; }
 constexpr uint32_t kTileWords = 2048;
 constexpr uint32_t kBlock = 256;
 
+// Words per workgroup of the LDS tile store builds: the tile (n_items x B
+// words) stays within 64 KB.  0: too many items for it (global atomics).
+inline uint32_t tile_words_for(uint32_t n_items) {
+    return n_items ? std::min<uint32_t>(8, 8192 / std::max<uint32_t>(n_items, 1)) : 0;
+}
+
 }  // namespace cpm
 
 struct cpm_ctx {
@@ -86,6 +93,9 @@ struct cpm_ctx {
     // search buffers (miner.hip), reused across searches
     void *search_ws = nullptr;
     void (*search_ws_free)(void *) = nullptr;
+    // store build from basket arrays (ingest.hip)
+    void *ingest_ws = nullptr;
+    void (*ingest_ws_free)(void *) = nullptr;
     // allocation of the last destroyed store, reused by the next store build
     cpm::DBuf spare_store;
     cpm::DBuf spare_arena;   // batch-build arena kept for the next batch
@@ -129,6 +139,9 @@ struct cpm_bits {
     std::shared_ptr<cpm::StoreArena> arena;  // ... or rows inside a batch arena
     uint64_t *view = nullptr;
     uint64_t *rows() const { return view ? view : store.as<uint64_t>(); }
+    // item id of each column, for stores built from basket arrays
+    cpm::DBuf labels;
+    bool has_labels = false;
 };
 
 namespace cpm {
@@ -181,6 +194,8 @@ namespace cpm {
 
 // Make the context's device current for the calling thread.
 int bind(cpm_ctx *ctx);
+// A store of n_items rows for n_trans transactions, zeroed when `clear`.
+int bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits **out, bool clear = true);
 
 // Event-timed launch bracket: accumulates milliseconds into stats[slot].
 // Event timing of one launch: two events on the stream, read later by

@@ -1,0 +1,392 @@
+// Store build straight from a basket file's arrays (CUPAMI01 / CUPAMIT1):
+// offsets u64[n+1] and item ids u32[m], as the file holds them.  Replaces the
+// host work of the reference's BIN path (cuApriori.py:273-276, 301-305: the
+// array checks; cuApriori.py:381-396: unique ids, labels sorted as strings,
+// row and column of every entry) and the COO it hands to build_bitsets.
+//
+//   1. k_ingest_scan_input  checks the offsets and ids, marks each kept
+//                           basket, and sets bit x of a presence bitmap for
+//                           every id x that occurs (LDS first when the
+//                           bitmap is small)
+//   2. scans                presence popcounts -> pre[w] (ids below word w),
+//                           kept flags -> act[i] (store row of basket i)
+//   3. host                 reads the two flags and two totals (one copy):
+//                           the only synchronization
+//   4. k_ingest_labels      per present id: its rank in decimal-string order
+//                           (the column) from range counts over the bitmap;
+//                           labels[column] = id
+//   5. k_ingest_rows        row starts of the kept baskets (skipped when every
+//                           basket is kept)
+//   6. k_ingest_tile        one workgroup per B words of every column: its
+//                           baskets' ids are read coalesced, each entry's row
+//                           found by a binary search over the rows' starts in
+//                           LDS, its bit ORed into an LDS tile written once
+//      (k_ingest_scatter: global atomics when the columns exceed the LDS tile)
+
+#include "cpm_internal.h"
+#include "sort.h"
+
+#include <algorithm>
+
+namespace cpm {
+namespace {
+
+constexpr uint32_t kLdsPresenceWords = 2048;  // ids below 131072 are marked in LDS first
+constexpr uint32_t kRowsPerTileMax = 64 * 8;  // rows of one tile workgroup (B <= 8 words)
+
+struct IngestWs {
+    DBuf state;   // u64 totals[2] (present ids, store rows), u32 flags[2]
+    DBuf pres;    // presence bitmap, (universe >> 6) + 1 words
+    DBuf cnt;     // popcount of each presence word
+    DBuf pre;     // ids present below each word (+ the total)
+    DBuf keep;    // u32 per basket: 1 if it becomes a store row
+    DBuf act;     // u64 per basket (+1): its store row
+    DBuf rows;    // u64 row starts of the store rows (+1)
+    DBuf col;     // column of the u-th present id in numeric order
+    SortWs sort;
+    HBuf host;
+};
+
+void free_ingest_ws(void *p) { delete static_cast<IngestWs *>(p); }
+
+IngestWs &ingest_ws(cpm_ctx *ctx) {
+    if (!ctx->ingest_ws) {
+        ctx->ingest_ws = new IngestWs();
+        ctx->ingest_ws_free = free_ingest_ws;
+    }
+    return *static_cast<IngestWs *>(ctx->ingest_ws);
+}
+
+// Offsets non-decreasing from 0 to m (flags[0] otherwise), ids <= universe
+// (flags[1] otherwise); keep[i] = basket i becomes a store row; presence bit
+// x set for every id x.  A bit is tested before the atomic that sets it, so
+// frequent ids cost one load after their first occurrence.
+__global__ __launch_bounds__(256) void k_ingest_scan_input(const uint64_t *__restrict__ off, uint64_t n,
+                                                          const uint32_t *__restrict__ ids, uint64_t m,
+                                                          uint32_t universe, uint64_t pres_words,
+                                                          int keep_empty, uint32_t *__restrict__ keep,
+                                                          unsigned long long *pres, unsigned int *flags) {
+    __shared__ unsigned long long lp[kLdsPresenceWords];
+    const bool lds = pres_words <= kLdsPresenceWords;
+    const uint32_t tid = threadIdx.x;
+    if (lds)
+        for (uint32_t i = tid; i < pres_words; i += blockDim.x) lp[i] = 0;
+    __syncthreads();
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + tid;
+    bool bad_off = g == 0 && (off[0] != 0 || off[n] != m);
+    bool bad_id = false;
+    for (uint64_t i = g; i < n; i += step) {
+        const uint64_t a = off[i], b = off[i + 1];
+        bad_off |= a > b;
+        keep[i] = keep_empty || b > a;
+    }
+    for (uint64_t k = g; k < m; k += step) {
+        const uint32_t x = ids[k];
+        if (x > universe) {
+            bad_id = true;
+            continue;
+        }
+        const unsigned long long bit = 1ull << (x & 63);
+        unsigned long long *w = lds ? &lp[x >> 6] : &pres[x >> 6];
+        if (!(*w & bit)) atomicOr(w, bit);
+    }
+    if (bad_off) atomicOr(flags, 1u);
+    if (bad_id) atomicOr(flags + 1, 1u);
+    if (lds) {
+        __syncthreads();
+        for (uint32_t i = tid; i < pres_words; i += blockDim.x)
+            if (lp[i]) atomicOr(&pres[i], lp[i]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_popcount_words(const uint64_t *__restrict__ w, uint64_t n,
+                                                       uint32_t *__restrict__ cnt) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        cnt[i] = (uint32_t)__popcll(w[i]);
+}
+
+// Ids present below v (v may exceed the bitmap: then all of them).
+__device__ __forceinline__ uint64_t present_below(const uint64_t *__restrict__ pres, const uint32_t *__restrict__ pre,
+                                                  uint64_t nbits, uint64_t v) {
+    if (v >= nbits) return pre[(nbits + 63) >> 6];
+    const uint64_t w = v >> 6;
+    return pre[w] + (uint64_t)__popcll(pres[w] & ((1ull << (v & 63)) - 1));
+}
+
+__device__ __forceinline__ bool is_present(const uint64_t *__restrict__ pres, uint64_t nbits, uint64_t v) {
+    return v < nbits && ((pres[v >> 6] >> (v & 63)) & 1);
+}
+
+// Position of str(x) among the decimal strings of the present ids, sorted as
+// strings (the reference orders labels with sorted() over str, so "10" < "2").
+// A present y of m digits sorts before x (L digits) when
+//   m <  L: y < the first m digits p of x, or y == p (a proper prefix);
+//   m == L: y < x;
+//   m >  L: y < x * 10^(m-L) (its first L digits are below x).
+// Each case is a range count over the bitmap.
+__device__ uint32_t string_rank(uint64_t x, const uint64_t *__restrict__ pres, const uint32_t *__restrict__ pre,
+                                uint64_t nbits) {
+    uint64_t p10[11];
+    p10[0] = 1;
+#pragma unroll
+    for (int i = 1; i < 11; ++i) p10[i] = p10[i - 1] * 10;
+    int L = 1;
+    while (L < 10 && x >= p10[L]) ++L;
+    uint64_t r = 0;
+    for (int m = 1; m <= 10; ++m) {
+        const uint64_t lo = m == 1 ? 0 : p10[m - 1];
+        if (lo >= nbits) break;
+        uint64_t end;
+        if (m < L) {
+            end = x / p10[L - m];
+            r += is_present(pres, nbits, end);
+        } else if (m == L) {
+            end = x;
+        } else {
+            end = std::min(p10[m], x * p10[m - L]);
+        }
+        if (end > lo) r += present_below(pres, pre, nbits, end) - present_below(pres, pre, nbits, lo);
+    }
+    return (uint32_t)r;
+}
+
+// One lane per presence word: the u-th present id (numeric order) gets column
+// col[u] = its string rank, and labels[column] = the id.
+__global__ __launch_bounds__(256) void k_ingest_labels(const uint64_t *__restrict__ pres, const uint32_t *__restrict__ pre,
+                                                      uint64_t pres_words, uint64_t nbits,
+                                                      uint32_t *__restrict__ col, uint32_t *__restrict__ labels) {
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < pres_words;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t bits = pres[w];
+        uint32_t u = pre[w];
+        while (bits) {
+            const uint64_t x = w * 64 + (uint64_t)__builtin_ctzll(bits);
+            const uint32_t c = string_rank(x, pres, pre, nbits);
+            col[u++] = c;
+            labels[c] = (uint32_t)x;
+            bits &= bits - 1;
+        }
+    }
+}
+
+// Row starts of the kept baskets: rows[act[i]] = off[i]; rows[n_rows] = m.
+__global__ __launch_bounds__(256) void k_ingest_rows(const uint64_t *__restrict__ off, uint64_t n,
+                                                    const uint32_t *__restrict__ keep, const uint64_t *__restrict__ act,
+                                                    uint64_t n_rows, uint64_t m, uint64_t *__restrict__ rows) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) rows[n_rows] = m;
+    for (uint64_t i = g; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        if (keep[i]) rows[act[i]] = off[i];
+}
+
+__device__ __forceinline__ uint32_t column_of(uint32_t x, const uint64_t *__restrict__ pres,
+                                              const uint32_t *__restrict__ pre, const uint32_t *__restrict__ col) {
+    const uint32_t w = x >> 6;
+    return col[pre[w] + (uint32_t)__popcll(pres[w] & ((1ull << (x & 63)) - 1))];
+}
+
+// Workgroup b owns words [b*B, b*B+B) of every column, i.e. store rows
+// [64*b*B, 64*(b+1)*B).  Those rows' entries are one contiguous run of ids;
+// lanes read it coalesced, four entries in flight each, find each entry's
+// row by a binary search over the row starts held in LDS, and OR its bit into
+// the LDS tile (n_items x B words), written out once: the store needs no
+// clearing and HBM sees no atomics.
+__global__ __launch_bounds__(256) void k_ingest_tile(const uint64_t *__restrict__ rows, uint64_t n_rows,
+                                                    const uint32_t *__restrict__ ids,
+                                                    const uint64_t *__restrict__ pres, const uint32_t *__restrict__ pre,
+                                                    const uint32_t *__restrict__ col, uint32_t n_items, uint32_t B,
+                                                    uint64_t stride, uint64_t *__restrict__ store) {
+    extern __shared__ unsigned long long tile[];
+    __shared__ uint64_t rs[kRowsPerTileMax + 1];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t w0 = (uint64_t)blockIdx.x * B;
+    const uint32_t cells = n_items * B;
+    for (uint32_t i = tid; i < cells; i += blockDim.x) tile[i] = 0;
+    const uint64_t r_lo = w0 * 64;
+    const uint32_t nr = r_lo < n_rows ? (uint32_t)std::min<uint64_t>((uint64_t)B * 64, n_rows - r_lo) : 0;
+    if (nr)
+        for (uint32_t i = tid; i <= nr; i += blockDim.x) rs[i] = rows[r_lo + i];
+    __syncthreads();
+    if (nr) {
+        const uint64_t lo = rs[0], hi = rs[nr];
+        constexpr int kBatch = 4;
+        for (uint64_t i0 = lo; i0 < hi; i0 += (uint64_t)kBatch * blockDim.x) {
+            uint32_t x[kBatch];
+#pragma unroll
+            for (int q = 0; q < kBatch; ++q) {
+                const uint64_t i = i0 + (uint64_t)q * blockDim.x + tid;
+                x[q] = i < hi ? ids[i] : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < kBatch; ++q) {
+                const uint64_t i = i0 + (uint64_t)q * blockDim.x + tid;
+                if (i >= hi) continue;
+                // the last row r in [0, nr) with rs[r] <= i
+                uint32_t a = 0, b = nr;
+                while (b - a > 1) {
+                    const uint32_t mid = (a + b) >> 1;
+                    if (rs[mid] <= i) a = mid; else b = mid;
+                }
+                const uint32_t c = column_of(x[q], pres, pre, col);
+                atomicOr(&tile[c * B + (a >> 6)], 1ull << (a & 63));
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < cells; i += blockDim.x) {
+        const uint32_t c = i / B, k = i % B;
+        if (w0 + k < stride) store[(uint64_t)c * stride + w0 + k] = tile[i];
+    }
+}
+
+// One lane per store row, its entries ORed into the (cleared) store.
+__global__ __launch_bounds__(256) void k_ingest_scatter(const uint64_t *__restrict__ rows, uint64_t n_rows,
+                                                       const uint32_t *__restrict__ ids,
+                                                       const uint64_t *__restrict__ pres, const uint32_t *__restrict__ pre,
+                                                       const uint32_t *__restrict__ col, uint64_t stride,
+                                                       unsigned long long *store) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_rows;
+         r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t a = rows[r], b = rows[r + 1];
+        for (uint64_t i = a; i < b; ++i) {
+            const uint32_t c = column_of(ids[i], pres, pre, col);
+            atomicOr(&store[(uint64_t)c * stride + (r >> 6)], 1ull << (r & 63));
+        }
+    }
+}
+
+unsigned grid_for(cpm_ctx *ctx, uint64_t n, uint64_t per_cu = 8) {
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->cu_count * per_cu));
+}
+
+}  // namespace
+
+// The whole build from device arrays; the store's labels are kept with it.
+int build_store_csr(cpm_ctx *ctx, const uint64_t *d_off, uint64_t n, const uint32_t *d_ids, uint64_t m,
+                    uint32_t universe, int keep_empty, cpm_bits **out) {
+    if (!d_off || (m && !d_ids)) return fail(CPM_ERR_INVALID, "offsets/ids are NULL");
+    if (universe < 1) return fail(CPM_ERR_INVALID, "Invalid BIN file: item universe must fit in uint32");
+    hipStream_t st = ctx->stream;
+    IngestWs &ws = ingest_ws(ctx);
+    const uint64_t nbits = (uint64_t)universe + 1;
+    const uint64_t pw = (nbits + 63) / 64;
+    CPM_CHECK(ws.state.ensure(32));
+    CPM_CHECK(ws.pres.ensure(pw * 8));
+    CPM_CHECK(ws.cnt.ensure(pw * 4));
+    CPM_CHECK(ws.pre.ensure((pw + 1) * 4));
+    CPM_CHECK(ws.keep.ensure(std::max<uint64_t>(n, 1) * 4));
+    CPM_CHECK(ws.act.ensure((n + 1) * 8));
+    CPM_CHECK(ws.host.ensure(32));
+    uint64_t *totals = ws.state.as<uint64_t>();
+    unsigned int *flags = reinterpret_cast<unsigned int *>(totals + 2);
+    uint64_t *pres = ws.pres.as<uint64_t>();
+    uint32_t *pre = ws.pre.as<uint32_t>();
+    uint32_t *keep = ws.keep.as<uint32_t>();
+    uint64_t *act = ws.act.as<uint64_t>();
+
+    CPM_HIP(hipMemsetAsync(totals, 0, 32, st));
+    CPM_HIP(hipMemsetAsync(pres, 0, pw * 8, st));
+    hipLaunchKernelGGL(k_ingest_scan_input, dim3(grid_for(ctx, std::max(n, m))), dim3(256), 0, st, d_off, n, d_ids,
+                       m, universe, pw, keep_empty, keep, reinterpret_cast<unsigned long long *>(pres), flags);
+    hipLaunchKernelGGL(k_popcount_words, dim3(grid_for(ctx, pw)), dim3(256), 0, st, pres, pw, ws.cnt.as<uint32_t>());
+    CPM_HIP(hipGetLastError());
+    CPM_CHECK(exclusive_scan_u32(ctx, ws.sort, ws.cnt.as<uint32_t>(), pre, pw, totals));
+    CPM_CHECK(exclusive_scan_u32_to_u64(ctx, ws.sort, keep, act, n, totals + 1));
+    uint64_t *h = ws.host.as<uint64_t>();
+    CPM_HIP(hipMemcpyAsync(h, totals, 24, hipMemcpyDeviceToHost, st));
+    CPM_HIP(hipStreamSynchronize(st));
+    const unsigned int *hflags = reinterpret_cast<const unsigned int *>(h + 2);
+    // the reference checks the offsets before the ids (cuApriori.py:273-276)
+    if (hflags[0]) return fail(CPM_ERR_INVALID, "Invalid BIN file: invalid transaction offsets");
+    if (hflags[1]) return fail(CPM_ERR_INVALID, "Invalid BIN file: item ID outside configured universe");
+    const uint64_t n_items = h[0], n_rows = h[1];
+    if (n_items > 0xFFFFFFFFull) return fail(CPM_ERR_INVALID, "too many distinct items");
+    if (n_rows >= (1ull << 40)) return fail(CPM_ERR_INVALID, "too many transactions");
+
+    const uint32_t B = tile_words_for((uint32_t)n_items);
+    cpm_bits *b = nullptr;
+    CPM_CHECK(bits_alloc(ctx, n_rows, (uint32_t)n_items, &b, B == 0));
+    int s = b->labels.ensure(std::max<uint64_t>(n_items, 1) * 4);
+    if (s == CPM_OK) s = ws.col.ensure(std::max<uint64_t>(n_items, 1) * 4);
+    const uint64_t *rows = d_off;  // every basket is a row
+    if (s == CPM_OK && !keep_empty) {
+        s = ws.rows.ensure((n_rows + 1) * 8);
+        if (s == CPM_OK) {
+            hipLaunchKernelGGL(k_ingest_rows, dim3(grid_for(ctx, n)), dim3(256), 0, st, d_off, n, keep, act, n_rows,
+                               m, ws.rows.as<uint64_t>());
+            rows = ws.rows.as<uint64_t>();
+        }
+    }
+    if (s == CPM_OK && n_items) {
+        uint32_t *col = ws.col.as<uint32_t>();
+        hipLaunchKernelGGL(k_ingest_labels, dim3(grid_for(ctx, pw)), dim3(256), 0, st, pres, pre, pw, nbits, col,
+                           b->labels.as<uint32_t>());
+        if (B) {
+            const uint64_t blocks = (b->stride + B - 1) / B;
+            if (blocks >= (1ull << 31)) {
+                cpm_bits_destroy(b);
+                return fail(CPM_ERR_INVALID, "too many transactions for one store");
+            }
+            hipLaunchKernelGGL(k_ingest_tile, dim3((unsigned)blocks), dim3(256), (size_t)n_items * B * 8, st, rows,
+                               n_rows, d_ids, pres, pre, col, (uint32_t)n_items, B, b->stride, b->rows());
+        } else {
+            hipLaunchKernelGGL(k_ingest_scatter, dim3(grid_for(ctx, n_rows, 32)), dim3(256), 0, st, rows, n_rows,
+                               d_ids, pres, pre, col, b->stride, reinterpret_cast<unsigned long long *>(b->rows()));
+        }
+    }
+    if (s == CPM_OK) {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) s = fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(e));
+    }
+    if (s != CPM_OK) {
+        (void)hipStreamSynchronize(st);
+        cpm_bits_destroy(b);
+        return s;
+    }
+    b->has_labels = true;
+    *out = b;
+    return CPM_OK;
+}
+
+}  // namespace cpm
+
+using namespace cpm;
+
+extern "C" {
+
+int cpm_bits_build_csr_device(cpm_ctx *ctx, const uint64_t *d_offsets, uint64_t n_baskets, const uint32_t *d_ids,
+                              uint64_t n_ids, uint32_t universe, int keep_empty, cpm_bits **out) {
+    clear_error();
+    CPM_CHECK(bind(ctx));
+    if (!out) return fail(CPM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    return build_store_csr(ctx, d_offsets, n_baskets, d_ids, n_ids, universe, keep_empty, out);
+}
+
+int cpm_bits_build_csr(cpm_ctx *ctx, const uint64_t *offsets, uint64_t n_baskets, const uint32_t *ids,
+                       uint64_t n_ids, uint32_t universe, int keep_empty, cpm_bits **out) {
+    clear_error();
+    CPM_CHECK(bind(ctx));
+    if (!out) return fail(CPM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!offsets || (n_ids && !ids)) return fail(CPM_ERR_INVALID, "offsets/ids are NULL");
+    DBuf &d_off = ctx->scratch[0], &d_ids = ctx->scratch[1];
+    CPM_CHECK(d_off.ensure((n_baskets + 1) * sizeof(uint64_t)));
+    CPM_CHECK(d_ids.ensure(std::max<uint64_t>(n_ids, 1) * sizeof(uint32_t)));
+    CPM_HIP(hipMemcpyAsync(d_off.p, offsets, (n_baskets + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+    if (n_ids)
+        CPM_HIP(hipMemcpyAsync(d_ids.p, ids, n_ids * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    return build_store_csr(ctx, d_off.as<uint64_t>(), n_baskets, d_ids.as<uint32_t>(), n_ids, universe, keep_empty,
+                           out);
+}
+
+int cpm_bits_labels(cpm_ctx *ctx, const cpm_bits *bits, uint32_t *ids) {
+    clear_error();
+    CPM_CHECK(bind(ctx));
+    if (!bits || (!ids && bits->n_items)) return fail(CPM_ERR_INVALID, "null argument");
+    if (!bits->has_labels) return fail(CPM_ERR_INVALID, "the store was not built from basket arrays");
+    return copy_to_host(ctx, ids, bits->labels.p, (size_t)bits->n_items * sizeof(uint32_t));
+}
+
+}  // extern "C"

@@ -59,6 +59,7 @@ class cuApriori:
         self.use_gds = use_gds
         self.use_gpu_io = use_gpu_io
         self._binaryArrays = None
+        self._binaryUniverse = 0
         self._transactionCount = 0
         self._inputBackend = {"backend": "cpu", "reason": "Text or Parquet input"}
         if use_gds is True and (not isinstance(iFile, (str, Path))
@@ -135,8 +136,10 @@ class cuApriori:
         # Bulk numeric path: typed arrays go straight to the COO builder.
         self._transactionCount = int(np.count_nonzero(offsets[1:] > offsets[:-1]))
         self._binaryArrays = (offsets.astype(np.uint64), ids.astype(np.uint32))
+        self._binaryUniverse = int(universe)
         self._inputBackend = {"backend": "numpy-hip",
-                              "reason": "Bulk numeric BIN reads; no Python transactions"}
+                              "reason": "Bulk numeric BIN reads; the store is built on the device "
+                                        "from the file's arrays"}
         return None
 
     def getInputBackend(self):
@@ -230,6 +233,26 @@ class cuApriori:
                            dtype=np.uint32, count=int(sum(lengths)))
         return items, rows, cols
 
+    def _getDeviceStore(self, ctx):
+        """Labels and the device store (``None`` when there are no items).
+
+        A BIN file's arrays go to the device as stored and the store is
+        built there (``cpm_bits_build_csr``): the distinct ids, their string
+        order, and each entry's row and column never exist on the host.
+        Other inputs build it from the host pairs of ``_getDevicePairs``."""
+        if self._binaryArrays is not None:
+            offsets, ids = self._binaryArrays
+            bits, labelIds = ctx.build_bits_csr(offsets, ids, self._binaryUniverse)
+            items = [str(v) for v in labelIds.tolist()]
+            if not items:
+                bits.close()
+                return items, None
+            return items, bits
+        items, rows, columns = self._getDevicePairs(ctx)
+        if not items:
+            return items, None
+        return items, ctx.build_bits(rows, columns, self._transactionCount, len(items))
+
     # ---- reference hooks, kept for subclasses and operator-level use --------------
     def _generateCandidates(self, previousLevel):
         """Host join/prune over label tuples (``cuApriori.py:473-492``)."""
@@ -259,10 +282,9 @@ class cuApriori:
     def mine(self):
         self._startTime = time.perf_counter()
         ctx = self._getCudaModule()
-        items, rows, columns = self._getDevicePairs(ctx)
         self._finalPatterns = {}
-        if items:
-            bits = ctx.build_bits(rows, columns, self._transactionCount, len(items))
+        items, bits = self._getDeviceStore(ctx)
+        if bits is not None:
             try:
                 patterns = self._deviceSearch(ctx, bits, items)
             finally:

@@ -50,9 +50,8 @@ class cuEclat(cuApriori):
         self._startTime = time.perf_counter()
         ctx = self._getCudaModule()
         self._finalPatterns = {}
-        items, rows, columns = self._getDevicePairs(ctx)
-        if items:
-            bits = _bitsets.build_bitsets(ctx, rows, columns, self._transactionCount, len(items))
+        items, bits = self._getDeviceStore(ctx)
+        if bits is not None:
             try:
                 self._mineHost(ctx, items, bits)
             finally:

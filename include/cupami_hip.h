@@ -77,6 +77,31 @@ int cpm_bits_build_device_batch(cpm_ctx *ctx, uint32_t n_db, const uint64_t *con
 int cpm_bits_build_batch(cpm_ctx *ctx, uint32_t n_db, const uint64_t *const *rows, const uint32_t *const *cols,
                          const uint64_t *nnz, const uint64_t *n_transactions, const uint32_t *n_items,
                          cpm_bits **out);
+/* Replaces the host side of the reference's BIN input (cuApriori.py:256-404):
+ * the array checks of _readBinaryFile / _tryBinaryBulkRead
+ * (cuApriori.py:273-276, 301-305), and the BIN branch of _getDevicePairs
+ * (cuApriori.py:381-396) that finds the distinct ids, sorts them as strings
+ * and maps every entry to its (row, column) for build_bitsets.  Input is a
+ * basket file's arrays as stored (CUPAMI01, and the offsets and ids of
+ * CUPAMIT1): offsets[n_baskets + 1] (basket i holds ids[offsets[i] ..
+ * offsets[i+1])) and ids[n_ids], all ids <= universe.  Columns are the
+ * distinct ids in decimal-string order ("10" < "2"); rows are the baskets in
+ * file order, empty baskets left out unless keep_empty (the temporal readers
+ * keep them, _temporal.py:125-139).  Repeated ids in a basket set one bit.
+ * Fails with CPM_ERR_INVALID and the reference's message for offsets that
+ * do not rise from 0 to n_ids ("Invalid BIN file: invalid transaction
+ * offsets") and for ids above universe ("... item ID outside configured
+ * universe"); then no store is returned.  cpm_bits_labels gives the id of
+ * each column and cpm_bits_shape the row count.  Host pointers. */
+int cpm_bits_build_csr(cpm_ctx *ctx, const uint64_t *offsets, uint64_t n_baskets, const uint32_t *ids,
+                       uint64_t n_ids, uint32_t universe, int keep_empty, cpm_bits **out);
+/* Same, with offsets/ids already in device memory. */
+int cpm_bits_build_csr_device(cpm_ctx *ctx, const uint64_t *d_offsets, uint64_t n_baskets, const uint32_t *d_ids,
+                              uint64_t n_ids, uint32_t universe, int keep_empty, cpm_bits **out);
+/* The item id of each column of a store built by cpm_bits_build_csr*
+ * (n_items values, host memory): the reference's `items` list
+ * (cuApriori.py:384-385) as numbers. */
+int cpm_bits_labels(cpm_ctx *ctx, const cpm_bits *bits, uint32_t *ids);
 int cpm_bits_destroy(cpm_bits *bits);
 /* words per row (ceil(N/64)), and the padded row stride in words */
 int cpm_bits_shape(const cpm_bits *bits, uint64_t *n_transactions, uint32_t *n_items,

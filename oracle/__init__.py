@@ -184,6 +184,28 @@ def clean_basket(row):
     return tuple(items)
 
 
+def csr_to_coo(offsets, ids, keep_empty=False):
+    """A basket file's arrays -> ``(labels, rows, cols, n_rows)``, the BIN
+    branch of ``cuApriori._getDevicePairs`` (``cuApriori.py:381-396``): the
+    distinct ids, their labels sorted as strings, each entry's basket renumbered
+    over the non-empty baskets (``activeRows``, ``:391``) and its column from a
+    search of the id among the distinct ones (``:394-395``).  ``keep_empty``
+    keeps every basket as a row (the temporal readers, ``_temporal.py:125-139``).
+    """
+    offsets = np.asarray(offsets, dtype=np.uint64)
+    ids = np.asarray(ids, dtype=np.uint32)
+    unique = np.unique(ids)
+    labels = sorted(str(int(v)) for v in unique)
+    index = {label: i for i, label in enumerate(labels)}
+    remap = np.array([index[str(int(v))] for v in unique], dtype=np.uint32)
+    lengths = np.diff(offsets.astype(np.int64))
+    kept = np.ones(lengths.size, bool) if keep_empty else lengths > 0
+    row_of_basket = np.cumsum(kept) - 1
+    rows = np.repeat(row_of_basket.astype(np.uint64), lengths)
+    cols = remap[np.searchsorted(unique, ids)] if ids.size else np.zeros(0, np.uint32)
+    return labels, rows, cols.astype(np.uint32), int(kept.sum())
+
+
 def support_threshold(minSup, n):
     """``cuApriori._getSupportCount`` (``cuApriori.py:452-456``)."""
     if isinstance(minSup, float) and 0 < minSup <= 1:

@@ -49,3 +49,23 @@ def temporal_coo(rows, end=None):
 
 def random_rows(rng, n, n_items, max_len):
     return [rng.sample(range(n_items), rng.randrange(max_len + 1)) for _ in range(n)]
+
+
+def csr(rows):
+    """Baskets (lists of int ids, repeats and empty baskets kept) -> the
+    ``offsets`` u64[n+1] and ``ids`` u32[m] arrays of a CUPAMI01 file."""
+    offsets = np.zeros(len(rows) + 1, np.uint64)
+    offsets[1:] = np.cumsum([len(r) for r in rows], dtype=np.uint64) if rows else []
+    ids = np.asarray([i for r in rows for i in r], np.uint32)
+    return offsets, ids
+
+
+def write_bin(path, rows, universe=None):
+    """A CUPAMI01 file (docs/transaction-formats.md: magic, N, M, U, offsets,
+    ids; little endian) holding the baskets as given."""
+    import struct
+    offsets, ids = csr(rows)
+    if universe is None:
+        universe = max(1, int(ids.max()) if ids.size else 1)
+    Path(path).write_bytes(struct.pack("<8sQQQ", b"CUPAMI01", len(rows), ids.size, universe)
+                           + offsets.astype("<u8").tobytes() + ids.astype("<u4").tobytes())

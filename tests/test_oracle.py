@@ -9,7 +9,7 @@ import pytest
 
 import generator
 import oracle
-from helpers import coo, kats, temporal_coo, random_rows
+from helpers import coo, csr, kats, temporal_coo, random_rows
 
 
 K = kats()
@@ -168,3 +168,47 @@ def test_exact_above_2_53():
     bits = oracle.bits_build(rr, cc, n, len(labels))
     got = oracle.labelled(oracle.pfeclat(bits, ts, n, end, 1, 2**53 + 1), labels)
     assert got == oracle.brute_periodic(rows, 1, 2**53 + 1)
+
+
+def _raw_baskets(rng, universe, n):
+    # ids across every decimal length up to the universe, repeats and empty
+    # baskets included, as a BIN file may hold them
+    pool = sorted({0, universe, *(rng.randrange(universe + 1) for _ in range(25)),
+                   *(min(universe, 10 ** k) for k in range(10))})
+    return [[rng.choice(pool) for _ in range(rng.randrange(7))] for _ in range(n)]
+
+
+@pytest.mark.parametrize("universe", [9, 100, 12345, 2**32 - 1])
+def test_csr_oracle_matches_the_basket_path(universe):
+    """The BIN branch of _getDevicePairs (cuApriori.py:381-396) and the
+    basket branch (_cleanTransaction + :397-404) must give the same labels,
+    row count and store for the same baskets."""
+    rng = random.Random(universe)
+    for trial in range(30):
+        rows = _raw_baskets(rng, universe, rng.randrange(80))
+        labels, rr, cc, n = oracle.csr_to_coo(*csr(rows))
+        l2, r2, c2, n2 = coo(rows)
+        assert labels == l2 and n == n2
+        assert np.array_equal(oracle.bits_build(rr, cc, n, len(labels)),
+                              oracle.bits_build(r2, c2, n2, len(l2)))
+
+
+def test_csr_oracle_keep_empty_matches_the_temporal_path():
+    rng = random.Random(5)
+    for trial in range(20):
+        rows = _raw_baskets(rng, 1000, rng.randrange(1, 80))
+        labels, rr, cc, n = oracle.csr_to_coo(*csr(rows), keep_empty=True)
+        l2, r2, c2, n2, _, _ = temporal_coo([[t, *r] for t, r in enumerate(rows)])
+        assert labels == l2 and n == n2 == len(rows)
+        assert np.array_equal(oracle.bits_build(rr, cc, n, len(labels)),
+                              oracle.bits_build(r2, c2, n2, len(l2)))
+
+
+@pytest.mark.parametrize("rows", [[], [[0]], [[1, 1, 2], [], [0, 2], [4294967295, 2]]])
+def test_csr_oracle_bulk_loading_cases(rows):
+    """The bulk BIN cases of test_transaction_formats.py:185-210: mining the
+    store built from the file's arrays gives the per-basket subset counts."""
+    labels, rr, cc, n = oracle.csr_to_coo(*csr(rows))
+    bits = oracle.bits_build(rr, cc, n, len(labels))
+    got = oracle.labelled(oracle.eclat(bits, 1), labels) if labels else {}
+    assert got == oracle.brute_force_rows(rows, 1)
