@@ -78,7 +78,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="eclat_1m_1k",
-                    choices=["eclat_1m_1k", "pfeclat_2m_500", "batch_256", "serve"])
+                    choices=["eclat_1m_1k", "pfeclat_2m_500", "batch_256", "serve", "ingest_1m_1k"])
     ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
     ap.add_argument("--qps", type=float, default=500.0, help="serve: mean arrival rate")
     ap.add_argument("--hbm-budget-gb", type=float, default=8.0, help="serve: admission budget")
@@ -99,6 +99,12 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(run_serve(args)), flush=True)
+        return
+    if args.workload == "ingest_1m_1k":
+        line = run_ingest(args)
+        print(json.dumps(line), flush=True)
+        if line["parity"] is False:
+            sys.exit(1)
         return
 
     import numpy as np
@@ -412,6 +418,110 @@ def run_batch(args):
         "parity": parity,
     }
     return line
+
+
+def run_ingest(args):
+    """The step before the search: the store built from a CUPAMI01 file's
+    arrays (offsets u64[N+1], ids u32[M]) already in HBM, labels included.
+    Baseline: the reference's host work for the same arrays (the BIN branch of
+    _getDevicePairs, restated in the oracle) plus the oracle's store build."""
+    import numpy as np
+    import torch
+    from cupami_amd import _hip, datagen
+
+    t = time.perf_counter()
+    db = datagen.eclat_1m_1k()
+    n, m, universe = db.n_transactions, db.items.size, db.n_items
+    offsets = np.searchsorted(db.rows, np.arange(n + 1, dtype=np.uint64)).astype(np.uint64)
+    log(f"bench: generated {n} baskets, {m} ids in {time.perf_counter() - t:.1f}s")
+    dev = torch.device("cuda:0")
+    d_off = torch.from_numpy(offsets.view(np.int64)).to(dev)
+    d_ids = torch.from_numpy(db.items.view(np.int32)).to(dev)
+    torch.cuda.synchronize()
+    ctx = _hip.context(0)
+
+    def step():
+        bits, labels = ctx.build_bits_csr_device(d_off.data_ptr(), n, d_ids.data_ptr(), m, universe)
+        return bits, labels
+
+    for _ in range(args.warmup):
+        bits, labels = step()
+        bits.close()
+    ctx.reset_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bits, labels = step()
+        if _ + 1 < args.steps:
+            bits.close()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = ctx.ingest_stats()
+    ms_per_step = elapsed / args.steps * 1e3
+    _, n_items, _, stride = bits.shape()
+    file_bytes = 8 * (n + 1) + 4 * m
+    store_bytes = 8 * n_items * stride
+    builds = max(st["builds"], 1)
+    tile_ms = st["tile_ms"] / builds
+    tile_bytes = st["tile_bytes"] / builds
+    achieved = tile_bytes / (tile_ms / 1e3) / 1e9 if tile_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_ingest_tile",
+                "avg_launch_ms": round(tile_ms, 4), "bytes_per_launch": tile_bytes}
+    log(f"bench: {args.steps} builds, {ms_per_step:.3f} ms/step (device span {st['build_ms'] / builds:.3f} ms), "
+        f"tile {tile_ms:.4f} ms at {achieved:.0f} GB/s")
+
+    parity = None
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        t = time.perf_counter()
+        labs, rr, cc, n_rows = oracle.csr_to_coo(offsets, db.items)
+        host = oracle.bits_build(rr, cc, n_rows, len(labs))
+        cpu_s = time.perf_counter() - t
+        parity = ([str(v) for v in labels.tolist()] == labs
+                  and np.array_equal(bits.download(), host))
+        cpu = {"value": round(m / cpu_s, 1), "unit": "entries/s", "cores": 1, "kind": "port",
+               "sample": f"whole workload, one run ({cpu_s:.2f}s): the BIN branch of _getDevicePairs "
+                         "(numpy, oracle.csr_to_coo) then oracle/cpm_oracle.c's store build"}
+        log(f"bench: cpu baseline {cpu_s:.2f}s, parity={parity}")
+    bits.close()
+    if args.copy_calibration:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("bench: copied 268435456 bytes device to device (counter calibration)")
+
+    # the same build with the arrays in host memory (PCIe copies included),
+    # and the pairs path it replaces: host COO from the arrays, then the pairs build
+    t = time.perf_counter()
+    for _ in range(3):
+        b, _l = ctx.build_bits_csr(offsets, db.items, universe)
+        b.close()
+    host_arrays_ms = (time.perf_counter() - t) / 3 * 1e3
+    from cupami_amd.frequentPattern.cuApriori import binary_pairs
+    t = time.perf_counter()
+    labs, rr, cc = binary_pairs(offsets, db.items)
+    b = ctx.build_bits(rr, cc, int(np.count_nonzero(np.diff(offsets))), len(labs))
+    ctx.synchronize()
+    pairs_ms = (time.perf_counter() - t) * 1e3
+    b.close()
+    return {
+        "metric": "CUPAMI01 arrays -> bitset store (entries/s)", "value": round(m / (elapsed / args.steps), 1),
+        "unit": "entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic (numpy PCG64 seed 202)",
+        "config": {"workload": "ingest_1m_1k", "baskets": n, "ids": m, "universe": universe,
+                   "items": n_items, "file_bytes": file_bytes, "store_bytes": store_bytes,
+                   "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "build_device_ms": round(st["build_ms"] / builds, 4),
+        "host_arrays_ms": round(host_arrays_ms, 3),
+        "pairs_path_ms": round(pairs_ms, 1),
+        "parity": parity,
+    }
 
 
 def run_serve(args):

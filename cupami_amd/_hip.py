@@ -50,6 +50,7 @@ SIGNATURES = {
     "cpm_bits_build_csr_device": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64,
                                                  ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(_vp)]),
     "cpm_bits_labels": (ctypes.c_int, [_vp, _vp, _u32p]),
+    "cpm_ingest_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "cpm_bits_destroy": (ctypes.c_int, [_vp]),
     "cpm_bits_shape": (ctypes.c_int, [_vp, _u64p, _u32p, _u64p, _u64p]),
     "cpm_bits_download": (ctypes.c_int, [_vp, _vp, _u64p]),
@@ -171,6 +172,11 @@ class Context:
         keys = ("launches", "kernel_ms", "bytes", "candidates", "gram_launches", "gram_ms",
                 "gram_ops", "levels")
         return dict(zip(keys, list(out)))
+
+    def ingest_stats(self):
+        out = (ctypes.c_double * 4)()
+        check(load().cpm_ingest_stats(self.handle, out))
+        return dict(zip(("builds", "tile_ms", "tile_bytes", "build_ms"), list(out)))
 
     def reset_stats(self):
         check(load().cpm_stats_reset(self.handle))

@@ -863,7 +863,15 @@ int cpm_stats(cpm_ctx *ctx, double *out8) {
     if (!ctx || !out8) return fail(CPM_ERR_INVALID, "null argument");
     CPM_CHECK(bind(ctx));
     CPM_CHECK(collect_timing(ctx));
-    memcpy(out8, ctx->stats, sizeof(ctx->stats));
+    memcpy(out8, ctx->stats, 8 * sizeof(double));
+    return CPM_OK;
+}
+
+int cpm_ingest_stats(cpm_ctx *ctx, double *out4) {
+    if (!ctx || !out4) return fail(CPM_ERR_INVALID, "null argument");
+    CPM_CHECK(bind(ctx));
+    CPM_CHECK(collect_timing(ctx));
+    memcpy(out4, ctx->stats + 8, 4 * sizeof(double));
     return CPM_OK;
 }
 

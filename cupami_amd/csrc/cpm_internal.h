@@ -78,7 +78,9 @@ struct cpm_ctx {
     hipStream_t stream = nullptr;
     int cu_count = 256;
     bool timing = true;
-    double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // [0..8) the device search (cpm_stats), [8..12) the store builds from
+    // basket arrays (cpm_ingest_stats)
+    double stats[12] = {};
     // event pairs of timed launches not read yet (read by cpm_stats), and
     // spare events
     struct TimedPair {
@@ -212,6 +214,12 @@ struct Timed {
             start = nullptr;
         }
     }
+    // a bracket left open by an early return gives its event back
+    ~Timed() {
+        if (start) ctx->ev_pool.push_back(start);
+    }
+    Timed(const Timed &) = delete;
+    Timed &operator=(const Timed &) = delete;
     int finish() {
         if (!start) return CPM_OK;
         hipEvent_t stop = take_event(ctx);

@@ -38,6 +38,22 @@ if __name__ == "__main__" and not __package__:
 from .. import _hip
 
 
+def binary_pairs(offsets, ids):
+    """Host (labels, rows, cols) of a BIN file's arrays (``cuApriori.py:381-396``):
+    distinct ids as string-sorted labels, each entry's row among the non-empty
+    baskets and its column.  The miners build the store on the device instead
+    (``cpm_bits_build_csr``); this serves ``_getDevicePairs`` callers."""
+    unique = np.unique(ids)
+    items = sorted(str(int(v)) for v in unique)
+    index = {label: i for i, label in enumerate(items)}
+    remap = np.array([index[str(int(v))] for v in unique], dtype=np.uint32)
+    lengths = np.diff(offsets).astype(np.int64)
+    active = np.cumsum(lengths > 0) - 1
+    rows = np.repeat(active.astype(np.uint64), lengths)
+    cols = remap[np.searchsorted(unique, ids)] if ids.size else np.zeros(0, np.uint32)
+    return items, rows, cols.astype(np.uint32)
+
+
 class cuApriori:
     """Frequent itemsets with exact supports (Agrawal and Srikant, VLDB 1994).
 
@@ -214,16 +230,7 @@ class cuApriori:
     def _getDevicePairs(self, ctx=None):
         """Labels sorted as strings, and the (row, column) pairs of the store."""
         if self._binaryArrays is not None:
-            offsets, ids = self._binaryArrays
-            unique = np.unique(ids)
-            items = sorted(str(int(v)) for v in unique)
-            index = {label: i for i, label in enumerate(items)}
-            remap = np.array([index[str(int(v))] for v in unique], dtype=np.uint32)
-            lengths = np.diff(offsets).astype(np.int64)
-            active = np.cumsum(lengths > 0) - 1
-            rows = np.repeat(active.astype(np.uint64), lengths)
-            cols = remap[np.searchsorted(unique, ids)] if ids.size else np.zeros(0, np.uint32)
-            return items, rows, cols.astype(np.uint32)
+            return binary_pairs(*self._binaryArrays)
         items = sorted({item for row in self._Database for item in row})
         index = {item: i for i, item in enumerate(items)}
         lengths = [len(row) for row in self._Database]

@@ -249,6 +249,13 @@ int cpm_patterns_view(const cpm_patterns *p, const uint32_t **parent, const uint
  * Event times are read here: cpm_stats and cpm_stats_reset wait for the
  * context's stream when timed launches are outstanding. */
 int cpm_stats(cpm_ctx *ctx, double *out8);
+/* No counterpart in the reference (its BIN path has no device build to
+ * time).  Totals since the last cpm_stats_reset of the store builds from
+ * basket arrays: [0] builds, [1] ms of the tile build kernel, [2] its
+ * algorithmic bytes (8 per row start + 4 per id read, 8 per store word
+ * written), [3] ms from the first to the last launch of each build (the one
+ * host readback of the build included).  Event-timed. */
+int cpm_ingest_stats(cpm_ctx *ctx, double *out4);
 int cpm_stats_reset(cpm_ctx *ctx);
 /* Enable (1) / disable (0) per-launch event timing of the gram (default on;
  * it adds no synchronization). */

@@ -49,7 +49,9 @@ def random_csr(seed, n, universe, distinct, max_len, empty_share=0.1):
     (1, 5000, 100, 60, 12, False),            # presence bitmap in LDS, LDS tile B = 8
     (2, 20000, 200_000, 3000, 20, False),     # presence bitmap in global memory, B = 2
     (3, 3000, 2**32 - 1, 40, 6, False),       # 2^32-bit bitmap, ids of every decimal length
-    (4, 3000, 50_000, 9000, 30, False),       # > 8192 columns: global atomics
+    (4, 3000, 50_000, 9000, 30, False),       # > 7168 columns: global atomics
+    (8, 3000, 100_000, 6000, 30, False),      # 4097..7168 columns: one-word tile, map in global memory
+    (9, 6000, 4095, 4000, 12, False),         # largest map held in LDS
     (5, 4000, 1000, 700, 10, True),           # every basket a row
     (6, 129, 99, 99, 3, False),               # ragged: 129 baskets, a part-filled word
 ])
