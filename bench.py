@@ -78,7 +78,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="eclat_1m_1k",
-                    choices=["eclat_1m_1k", "pfeclat_2m_500", "batch_256", "serve", "ingest_1m_1k"])
+                    choices=["eclat_1m_1k", "pfeclat_2m_500", "batch_256", "serve", "ingest_1m_1k",
+                             "output_1m_1k"])
     ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
     ap.add_argument("--qps", type=float, default=500.0, help="serve: mean arrival rate")
     ap.add_argument("--hbm-budget-gb", type=float, default=8.0, help="serve: admission budget")
@@ -99,6 +100,12 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(run_serve(args)), flush=True)
+        return
+    if args.workload == "output_1m_1k":
+        line = run_output(args)
+        print(json.dumps(line), flush=True)
+        if line["parity"] is False:
+            sys.exit(1)
         return
     if args.workload == "ingest_1m_1k":
         line = run_ingest(args)
@@ -520,6 +527,79 @@ def run_ingest(args):
         "build_device_ms": round(st["build_ms"] / builds, 4),
         "host_arrays_ms": round(host_arrays_ms, 3),
         "pairs_path_ms": round(pairs_ms, 1),
+        "parity": parity,
+    }
+
+
+def run_output(args):
+    """The step after the search: the 1M x 1k Eclat result (43,645 patterns)
+    turned into getPatterns()'s dict and save()'s file by the library
+    (cpm_patterns_keys / cpm_patterns_save), against the reference's Python
+    rendering of the same result (cuApriori.py:537-552: a tuple per pattern,
+    "\t".join per key, sorted() for the file)."""
+    import tempfile
+    import numpy as np
+    import torch
+    from cupami_amd import _hip, datagen
+
+    db = datagen.eclat_1m_1k()
+    labels, cols = datagen.label_columns(db.items)
+    ctx = _hip.context(0)
+    bits = ctx.build_bits(db.rows, cols, db.n_transactions, len(labels))
+    pats = ctx.eclat_mine(bits, MINSUP, use_gram=True)
+    bits.close()
+    torch.cuda.synchronize()
+    n = len(pats)
+    tmp = Path(tempfile.mkdtemp())
+
+    def native():
+        d = dict(zip(pats.keys(labels), pats.support.tolist()))
+        pats.save(tmp / "native.txt", labels)
+        return d
+
+    def python():
+        tuples = pats.to_dict(labels)
+        d = {"\t".join(p): s for p, s in tuples.items()}
+        with (tmp / "python.txt").open("w", encoding="utf-8") as handle:
+            for pattern, support in sorted(tuples.items()):
+                handle.write(f"{chr(9).join(pattern)} : {support}\n")
+        return d
+
+    for _ in range(args.warmup):
+        native()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        d_native = native()
+    native_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    t0 = time.perf_counter()
+    t_keys = 0.0
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        pats.keys(labels)
+        t_keys += time.perf_counter() - t
+    keys_ms = t_keys / args.steps * 1e3
+    t0 = time.perf_counter()
+    d_python = python()
+    python_ms = (time.perf_counter() - t0) * 1e3
+    parity = (d_native == d_python
+              and (tmp / "native.txt").read_bytes() == (tmp / "python.txt").read_bytes())
+    file_bytes = (tmp / "native.txt").stat().st_size
+    log(f"bench: output of {n} patterns: native {native_ms:.2f} ms (keys {keys_ms:.2f} ms), "
+        f"python {python_ms:.1f} ms, parity={parity}")
+    return {
+        "metric": "patterns rendered to getPatterns() + save() (patterns/s)",
+        "value": round(n / (native_ms / 1e3), 1), "unit": "patterns/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(native_ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (numpy PCG64 seed 202)",
+        "config": {"workload": "output_1m_1k", "patterns": n, "save_bytes": file_bytes,
+                   "parallelism": "host (library renders; the search ran on one GPU)"},
+        "roofline": None,
+        "cpu_baseline": {"value": round(n / (python_ms / 1e3), 1), "unit": "patterns/s", "cores": 1,
+                         "kind": "port",
+                         "sample": "one run of the reference's Python rendering of the same result "
+                                   "(cuApriori.py:537-552) in this process"},
+        "keys_ms": round(keys_ms, 3), "python_ms": round(python_ms, 1),
         "parity": parity,
     }
 

@@ -51,6 +51,9 @@ SIGNATURES = {
                                                  ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(_vp)]),
     "cpm_bits_labels": (ctypes.c_int, [_vp, _vp, _u32p]),
     "cpm_ingest_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
+    "cpm_patterns_keys": (ctypes.c_int, [_vp, ctypes.c_char_p, _u64p, ctypes.c_uint32, ctypes.c_char, _vp,
+                                         ctypes.c_uint64, _u64p]),
+    "cpm_patterns_save": (ctypes.c_int, [_vp, ctypes.c_char_p, _u64p, ctypes.c_uint32, ctypes.c_char_p]),
     "cpm_bits_destroy": (ctypes.c_int, [_vp]),
     "cpm_bits_shape": (ctypes.c_int, [_vp, _u64p, _u32p, _u64p, _u64p]),
     "cpm_bits_download": (ctypes.c_int, [_vp, _vp, _u64p]),
@@ -490,6 +493,15 @@ class _PatternsHandle:
             self.handle = None
 
 
+def _label_buffer(labels):
+    """UTF-8 labels concatenated, and their byte offsets (n + 1)."""
+    encoded = [label.encode("utf-8") for label in labels]
+    off = np.zeros(len(encoded) + 1, dtype=np.uint64)
+    if encoded:
+        off[1:] = np.cumsum([len(e) for e in encoded], dtype=np.uint64)
+    return b"".join(encoded), off
+
+
 def _view(owner, address, n, ctype, dtype):
     """numpy view of n elements at `address`, keeping `owner` alive."""
     buf = (ctype * n).from_address(address)
@@ -505,6 +517,7 @@ class Patterns:
     def __init__(self, handle, periodic=False, databases=False):
         lib = load()
         owner = _PatternsHandle(handle)
+        self._owner = owner
         n = ctypes.c_uint64()
         check(lib.cpm_patterns_count(handle, ctypes.byref(n)))
         self.n = n.value
@@ -547,6 +560,32 @@ class Patterns:
         for s, v, d in zip(sets, self.support.tolist(), self.db.tolist()):
             out[d][s] = v
         return out
+
+    def keys(self, labels):
+        """The key of every pattern (its labels joined by a tab, in column
+        order), in result order, rendered by the library."""
+        if self.n == 0:
+            return []
+        labels = [str(label) for label in labels]
+        sep = next((c for c in ("\n", "\x00", "\x01", "\x1f") if not any(c in lab for lab in labels)), None)
+        if sep is None:  # every candidate separator occurs in some label
+            return ["\t".join(labels[c] for c in s) for s in self.itemsets()]
+        text, off = _label_buffer(labels)
+        size = ctypes.c_uint64()
+        lib = load()
+        check(lib.cpm_patterns_keys(self._owner.handle, text, _p(off, _u64p), len(labels), sep.encode(), None, 0,
+                                    ctypes.byref(size)))
+        buf = ctypes.create_string_buffer(max(size.value, 1))
+        check(lib.cpm_patterns_keys(self._owner.handle, text, _p(off, _u64p), len(labels), sep.encode(), buf,
+                                    size.value, ctypes.byref(size)))
+        return buf.raw[:size.value].decode("utf-8").split(sep)
+
+    def save(self, path, labels):
+        """The reference's save file for this result: one line per pattern,
+        sorted by label tuple, written by the library."""
+        text, off = _label_buffer([str(label) for label in labels])
+        check(load().cpm_patterns_save(self._owner.handle, text, _p(off, _u64p), len(off) - 1,
+                                       os.fsencode(str(path))))
 
     def to_dict(self, labels=None):
         """``{itemset: support}`` (or ``(support, period)``), keyed by column tuples

@@ -296,20 +296,62 @@ class cuApriori:
                 patterns = self._deviceSearch(ctx, bits, items)
             finally:
                 bits.close()
-            self._finalPatterns = patterns.to_dict(items)
+            self._keepDeviceResult(patterns, items)
         ctx.synchronize()
         self._endTime = time.perf_counter()
 
+    # ---- results ---------------------------------------------------------------
+    # A device search's result stays in the library's arrays: the outputs
+    # render it there in bulk (cpm_patterns_keys / cpm_patterns_save), and the
+    # reference's _finalPatterns dict (label tuple -> support) is built only
+    # when something reads it.
+    @property
+    def _finalPatterns(self):
+        pending = self.__dict__.get("_pendingResult")
+        if pending is not None:
+            self.__dict__["_pendingResult"] = None
+            self.__dict__["_finalPatternStore"] = self._pendingToDict(*pending)
+        return self.__dict__.setdefault("_finalPatternStore", {})
+
+    @_finalPatterns.setter
+    def _finalPatterns(self, value):
+        self.__dict__["_pendingResult"] = None
+        self.__dict__["_finalPatternStore"] = value
+
+    def _keepDeviceResult(self, patterns, labels):
+        self._finalPatterns = {}
+        self.__dict__["_pendingResult"] = (patterns, labels)
+
+    def _pendingToDict(self, patterns, labels):
+        return patterns.to_dict(labels)
+
+    def _pending(self):
+        return self.__dict__.get("_pendingResult")
+
     # ---- outputs (cuApriori.py:537-590) -----------------------------------------------
     def getPatterns(self):
+        pending = self._pending()
+        if pending is not None:
+            patterns, labels = pending
+            return dict(zip(patterns.keys(labels), patterns.support.tolist()))
         return {"\t".join(p): s for p, s in self._finalPatterns.items()}
 
     def getPatternsAsDataFrame(self):
         import pandas as pd
+        pending = self._pending()
+        if pending is not None:
+            patterns, labels = pending
+            return pd.DataFrame({"Patterns": patterns.keys(labels),
+                                 "Support": patterns.support.astype(np.int64)})
         rows = [("\t".join(p), s) for p, s in self._finalPatterns.items()]
         return pd.DataFrame(rows, columns=["Patterns", "Support"])
 
     def save(self, outputFile):
+        pending = self._pending()
+        if pending is not None:
+            patterns, labels = pending
+            patterns.save(outputFile, labels)
+            return
         with Path(outputFile).open("w", encoding="utf-8") as handle:
             for pattern, support in sorted(self._finalPatterns.items()):
                 handle.write(f"{chr(9).join(pattern)} : {support}\n")
@@ -343,7 +385,9 @@ class cuApriori:
         return int(psutil.Process().memory_full_info().uss)
 
     def printResults(self):
-        print("Total number of Frequent Patterns:", len(self._finalPatterns))
+        pending = self._pending()
+        print("Total number of Frequent Patterns:",
+              len(pending[0]) if pending is not None else len(self._finalPatterns))
         print("Total Memory in USS:", self.getMemoryUSS())
         print("Total Memory in RSS:", self.getMemoryRSS())
         print("Loading time (seconds):", self.getLoadingTime())

@@ -177,16 +177,38 @@ class TemporalInput(cuApriori):
             return True
         return False
 
+    def _pendingToDict(self, patterns, labels):
+        # support in _finalPatterns, period in _periods, as _accept fills them
+        sets = [tuple(labels[c] for c in s) for s in patterns.itemsets()]
+        self._periods = dict(zip(sets, patterns.period.tolist()))
+        return dict(zip(sets, patterns.support.tolist()))
+
     def getPatterns(self):
+        pending = self._pending()
+        if pending is not None:
+            patterns, labels = pending
+            return dict(zip(patterns.keys(labels),
+                            zip(patterns.support.tolist(), patterns.period.tolist())))
         return {"\t".join(p): (s, self._periods[p]) for p, s in self._finalPatterns.items()}
 
     def getPatternsAsDataFrame(self):
         import pandas as pd
+        pending = self._pending()
+        if pending is not None:
+            patterns, labels = pending
+            return pd.DataFrame({"Patterns": patterns.keys(labels),
+                                 "Support": patterns.support.astype(np.int64),
+                                 "Periodicity": patterns.period.astype(np.int64)})
         return pd.DataFrame([("\t".join(p), s, self._periods[p])
                              for p, s in self._finalPatterns.items()],
                             columns=["Patterns", "Support", "Periodicity"])
 
     def save(self, outputFile):
+        pending = self._pending()
+        if pending is not None:
+            patterns, labels = pending
+            patterns.save(outputFile, labels)
+            return
         with Path(outputFile).open("w", encoding="utf-8") as handle:
             for pattern, support in sorted(self._finalPatterns.items()):
                 handle.write("{} : {} : {}\n".format("\t".join(pattern), support,

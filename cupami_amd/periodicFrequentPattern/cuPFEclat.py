@@ -38,11 +38,7 @@ class cuPFEclat(TemporalInput):
             finally:
                 self._temporalBits.close()
                 self._temporalBits = None
-            sets = patterns.itemsets()
-            for ids, s, p in zip(sets, patterns.support.tolist(), patterns.period.tolist()):
-                key = tuple(labels[c] for c in ids)
-                self._finalPatterns[key] = int(s)
-                self._periods[key] = int(p)
+            self._keepDeviceResult(patterns, labels)
         ctx.synchronize()
         self._endTime = time.perf_counter()
 

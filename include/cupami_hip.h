@@ -238,6 +238,22 @@ int cpm_patterns_databases(const cpm_patterns *p, uint32_t *db);
  * callers that read large results without copying them. */
 int cpm_patterns_view(const cpm_patterns *p, const uint32_t **parent, const uint32_t **item,
                       const uint64_t **support, const int64_t **period, const uint32_t **db);
+/* Replaces the key building of getPatterns / getPatternsAsDataFrame
+ * (cuApriori.py:537-547, _temporal.py:170-176): the key of every pattern --
+ * the labels of its columns in ascending column order joined by "\t" -- in
+ * the result's order, separated by `sep`, into out[cap].  Labels are the
+ * n_labels column labels as UTF-8, concatenated in `labels`, label i being
+ * labels[label_off[i] .. label_off[i+1]).  *bytes receives the size of the
+ * whole text; with out == NULL only the size is computed.  Host memory. */
+int cpm_patterns_keys(const cpm_patterns *p, const char *labels, const uint64_t *label_off, uint32_t n_labels,
+                      char sep, char *out, uint64_t cap, uint64_t *bytes);
+/* Replaces save (cuApriori.py:548-552; periodic results _temporal.py:178-181):
+ * writes "<key> : <support>\n" ("<key> : <support> : <period>\n" for a
+ * periodic result) for every pattern to the file at `path`, in ascending
+ * order of the label tuples (Python's sorted() over the reference's
+ * _finalPatterns items).  Labels as for cpm_patterns_keys. */
+int cpm_patterns_save(const cpm_patterns *p, const char *labels, const uint64_t *label_off, uint32_t n_labels,
+                      const char *path);
 
 /* ---- measurement -------------------------------------------------------- */
 
