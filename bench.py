@@ -41,18 +41,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic():
-    """HBM bytes per support-kernel launch from the committed counter runs
-    (profiles/eclat_1m_1k_hbm_traffic.md), used only while the kernel source
-    is the one that was measured."""
-    prof = ROOT / "profiles" / "eclat_1m_1k_hbm_traffic.md"
+def measured_traffic(profile="eclat_1m_1k_hbm_traffic.md", key="k_support_count_hbm_bytes_per_launch"):
+    """HBM bytes per launch of a kernel from the committed counter runs
+    (profiles/<profile>), used only while the kernel sources are the ones
+    that were measured."""
+    prof = ROOT / "profiles" / profile
     if not prof.exists():
         return None
     fields = dict(line.split(": ", 1) for line in prof.read_text().splitlines()
-                  if line.startswith(("k_support_count_hbm_bytes_per_launch:", "csrc_sha256:")))
-    if fields.get("csrc_sha256") != csrc_digest():
+                  if line.startswith((key + ":", "csrc_sha256:")))
+    if fields.get("csrc_sha256") != csrc_digest() or key not in fields:
         return None
-    return float(fields["k_support_count_hbm_bytes_per_launch"])
+    return float(fields[key])
 
 
 def csrc_digest(root=None):
@@ -473,7 +473,9 @@ def run_ingest(args):
     tile_bytes = st["tile_bytes"] / builds
     achieved = tile_bytes / (tile_ms / 1e3) / 1e9 if tile_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_ingest_tile",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic("ingest_1m_1k_hbm_traffic.md", "k_ingest_tile_hbm_bytes_per_launch"),
+                "kernel": "k_ingest_tile",
                 "avg_launch_ms": round(tile_ms, 4), "bytes_per_launch": tile_bytes}
     log(f"bench: {args.steps} builds, {ms_per_step:.3f} ms/step (device span {st['build_ms'] / builds:.3f} ms), "
         f"tile {tile_ms:.4f} ms at {achieved:.0f} GB/s")
