@@ -38,17 +38,18 @@ if __name__ == "__main__" and not __package__:
 from .. import _hip
 
 
-def binary_pairs(offsets, ids):
+def binary_pairs(offsets, ids, keep_empty=False):
     """Host (labels, rows, cols) of a BIN file's arrays (``cuApriori.py:381-396``):
     distinct ids as string-sorted labels, each entry's row among the non-empty
-    baskets and its column.  The miners build the store on the device instead
-    (``cpm_bits_build_csr``); this serves ``_getDevicePairs`` callers."""
+    baskets (every basket with ``keep_empty``) and its column.  The miners
+    build the store on the device instead (``cpm_bits_build_csr``); this
+    serves ``_getDevicePairs`` callers."""
     unique = np.unique(ids)
     items = sorted(str(int(v)) for v in unique)
     index = {label: i for i, label in enumerate(items)}
     remap = np.array([index[str(int(v))] for v in unique], dtype=np.uint32)
     lengths = np.diff(offsets).astype(np.int64)
-    active = np.cumsum(lengths > 0) - 1
+    active = np.arange(lengths.size) if keep_empty else np.cumsum(lengths > 0) - 1
     rows = np.repeat(active.astype(np.uint64), lengths)
     cols = remap[np.searchsorted(unique, ids)] if ids.size else np.zeros(0, np.uint32)
     return items, rows, cols.astype(np.uint32)

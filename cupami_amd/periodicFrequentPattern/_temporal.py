@@ -23,13 +23,14 @@ from urllib.request import urlopen
 
 import numpy as np
 
-from ..frequentPattern.cuApriori import cuApriori
+from ..frequentPattern.cuApriori import binary_pairs, cuApriori
 
 _INT64_MAX = np.iinfo(np.int64).max
 
 
-def read_temporal_binary(path):
-    """Rows ``[ts, *items]`` of a ``CUPAMIT1`` file (``_temporal_io.py:48-72``)."""
+def read_temporal_arrays(path):
+    """The arrays of a ``CUPAMIT1`` file (``_temporal_io.py:48-72``), checked:
+    ``(timestamps i64[n], offsets u64[n+1], items u32[m], universe)``."""
     header = struct.Struct("<8sQQQ")
     with Path(path).open("rb") as source:
         raw = source.read(header.size)
@@ -53,6 +54,12 @@ def read_temporal_binary(path):
         raise ValueError("Invalid temporal BIN offsets")
     if np.any(items > universe):
         raise ValueError("Temporal BIN item ID exceeds header range")
+    return ts, offsets, items, int(universe)
+
+
+def read_temporal_binary(path):
+    """Rows ``[ts, *items]`` of a ``CUPAMIT1`` file (``_temporal_io.py:48-72``)."""
+    ts, offsets, items, _ = read_temporal_arrays(path)
     return [[int(t), *items[int(offsets[i]):int(offsets[i + 1])].tolist()]
             for i, t in enumerate(ts.tolist())]
 
@@ -110,9 +117,9 @@ class TemporalInput(cuApriori):
             if suffix in (".bin", ".parquet"):
                 if is_url:
                     raise ValueError("Temporal BIN and Parquet input requires a local file")
-                rows = (read_temporal_binary(source) if suffix == ".bin"
-                        else read_temporal_parquet(source))
-                return self._cleanTemporalRows(rows)
+                if suffix == ".bin":
+                    return self._readTemporalArrays(source)
+                return self._cleanTemporalRows(read_temporal_parquet(source))
             if is_url:
                 with urlopen(str(source), timeout=30) as response:
                     text = response.read().decode("utf-8-sig")
@@ -144,17 +151,58 @@ class TemporalInput(cuApriori):
         self._inputBackend = {"backend": "cpu", "reason": "Temporal reader; mining uses HIP"}
         return [record[1] for record in records]
 
+    def _readTemporalArrays(self, source):
+        """A ``CUPAMIT1`` file's arrays, kept for the device build: every row
+        stays (timestamp-only rows too) and rows are ordered by timestamp,
+        stably, as ``_cleanTemporalRows`` orders them (``_temporal.py:125-139``)."""
+        ts, offsets, ids, universe = read_temporal_arrays(source)
+        if ts.size > 1 and np.any(ts[1:] < ts[:-1]):
+            order = np.argsort(ts, kind="stable")
+            lengths = np.diff(offsets.astype(np.int64))[order]
+            starts = offsets[:-1].astype(np.int64)[order]
+            sorted_off = np.zeros(ts.size + 1, dtype=np.uint64)
+            sorted_off[1:] = np.cumsum(lengths)
+            base = np.repeat(starts - sorted_off[:-1].astype(np.int64), lengths)
+            ids = ids[base + np.arange(ids.size, dtype=np.int64)]
+            offsets, ts = sorted_off, ts[order]
+        self._timestamps = ts.astype(np.int64)
+        last = int(ts[-1]) if ts.size else 0
+        self._endTimestamp = last if self.endTime is None else self._timestamp(self.endTime)
+        if self._endTimestamp < last:
+            raise ValueError("endTime must be at least the last timestamp")
+        self._transactionCount = int(ts.size)
+        self._binaryArrays = (offsets.astype(np.uint64), ids.astype(np.uint32))
+        self._binaryUniverse = universe
+        self._inputBackend = {"backend": "numpy-hip",
+                              "reason": "Temporal BIN arrays; the store is built on the device"}
+        return None
+
+    def _getDevicePairs(self, ctx=None):
+        if self._binaryArrays is not None:   # every row kept, timestamp-only rows too
+            return binary_pairs(*self._binaryArrays, keep_empty=True)
+        return super()._getDevicePairs(ctx)
+
     def _maxPerFloor(self):
         """Integer bound equivalent to ``period <= maxPer`` for integer periods."""
         value = math.floor(self._maxPer)
         return max(-1, min(int(value), int(_INT64_MAX)))
 
     def _prepare(self, ctx):
-        labels, rows, columns = self._getDevicePairs(ctx)
+        if self._binaryArrays is not None:
+            # the store straight from the file's arrays, every row kept
+            offsets, ids = self._binaryArrays
+            bits, label_ids = ctx.build_bits_csr(offsets, ids, self._binaryUniverse, keep_empty=True)
+            labels, rows, columns = [str(v) for v in label_ids.tolist()], None, None
+            if not labels:
+                bits.close()
+                bits = None
+        else:
+            labels, rows, columns = self._getDevicePairs(ctx)
+            bits = (ctx.build_bits(rows, columns, self._transactionCount, len(labels))
+                    if labels else None)
         self._labels = labels
         self._itemIndexes = {label: i for i, label in enumerate(labels)}
-        self._temporalBits = (ctx.build_bits(rows, columns, self._transactionCount, len(labels))
-                              if labels else None)
+        self._temporalBits = bits
         self._hostTimestamps = np.asarray(self._timestamps, dtype=np.int64)
         self._finalPatterns = {}
         self._periods = {}

@@ -69,3 +69,15 @@ def write_bin(path, rows, universe=None):
         universe = max(1, int(ids.max()) if ids.size else 1)
     Path(path).write_bytes(struct.pack("<8sQQQ", b"CUPAMI01", len(rows), ids.size, universe)
                            + offsets.astype("<u8").tobytes() + ids.astype("<u4").tobytes())
+
+
+def write_temporal_bin(path, rows, universe=None):
+    """A CUPAMIT1 file (timestamps i64[n], offsets u64[n+1], ids u32[m] after
+    the CUPAMI01-style header) holding rows ``[ts, *items]`` as given."""
+    import struct
+    ts = np.asarray([r[0] for r in rows], "<i8")
+    offsets, ids = csr([list(r[1:]) for r in rows])
+    if universe is None:
+        universe = max(1, int(ids.max()) if ids.size else 1)
+    Path(path).write_bytes(struct.pack("<8sQQQ", b"CUPAMIT1", len(rows), ids.size, universe) + ts.tobytes()
+                           + offsets.astype("<u8").tobytes() + ids.astype("<u4").tobytes())

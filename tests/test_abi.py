@@ -137,3 +137,32 @@ def test_datagen_is_seeded_and_shaped():
     assert len(pairs) == a.rows.size
     t = datagen.temporal(1000, 20, 4.0, 1.0, seed=4)
     assert np.all(np.diff(t.timestamps) >= 1) and np.all(np.diff(t.timestamps) <= 3)
+
+
+def test_temporal_bin_arrays_match_the_row_reader(tmp_path):
+    """A CUPAMIT1 file read as arrays (for the device build) gives the same
+    timestamps, row order and store as the row reader (_temporal.py:125-139):
+    timestamp-only rows kept, rows stably sorted by timestamp."""
+    import random
+    import oracle
+    from cupami_amd import cuPFEclat
+    from helpers import write_temporal_bin
+    rng = random.Random(2)
+    for trial in range(12):
+        rows = [[rng.choice([1, 2, 3, 5, 8, 13, 21, 34]) if trial % 2 else t,
+                 *[rng.choice([0, 3, 9, 10, 11, 100]) for _ in range(rng.randrange(5))]]
+                for t in range(1, rng.randrange(2, 60))]
+        path = tmp_path / "t.bin"
+        write_temporal_bin(path, rows)
+        a, b = cuPFEclat(path, 1, 10), cuPFEclat(rows, 1, 10)
+        assert a._Database is None and a._transactionCount == b._transactionCount == len(rows)
+        assert list(a._timestamps) == b._timestamps and a._endTimestamp == b._endTimestamp
+        la, ra, ca = a._getDevicePairs()
+        lb, rb, cb = b._getDevicePairs()
+        assert la == lb
+        n = len(rows)
+        assert np.array_equal(oracle.bits_build(ra, ca, n, len(la)), oracle.bits_build(rb, cb, n, len(lb)))
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"CUPAMIT1" + struct.pack("<QQQ", 1, 1, 3) + struct.pack("<q2QI", 5, 0, 1, 7))
+    with pytest.raises(ValueError):
+        cuPFEclat(bad, 1, 10)

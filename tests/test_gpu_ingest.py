@@ -9,8 +9,8 @@ import numpy as np
 import pytest
 
 import oracle
-from cupami_amd import _hip, cuApriori, cuEclat
-from helpers import csr, write_bin
+from cupami_amd import _hip, cuApriori, cuEclat, cuPFEclat
+from helpers import csr, write_bin, write_temporal_bin
 
 pytestmark = pytest.mark.gpu
 
@@ -154,3 +154,23 @@ def test_bin_file_matches_csv_for_every_miner(tmp_path):
         a.mine()
         b.mine()
         assert a.getPatterns() == b.getPatterns() and len(a.getPatterns()) > 40
+
+
+@pytest.mark.parametrize("sorted_ts", [True, False])
+def test_temporal_bin_file_mines_like_the_rows(tmp_path, sorted_ts):
+    """CUPAMIT1 through the device build (every row kept, rows ordered by
+    timestamp) == the row-list input == brute force (test_periodic_miners.py:12-25)."""
+    rng = random.Random(17)
+    rows = []
+    for t in range(1, 400):
+        stamp = t if sorted_ts else rng.randrange(1, 200)
+        rows.append([stamp, *[rng.randrange(1, 12) for _ in range(rng.randrange(6))]])
+    path = tmp_path / "t.bin"
+    write_temporal_bin(path, rows)
+    a, b = cuPFEclat(path, 12, 40), cuPFEclat(rows, 12, 40)
+    assert a.getInputBackend()["backend"] == "numpy-hip"
+    a.mine()
+    b.mine()
+    got = a.getPatterns()
+    assert got == b.getPatterns() == oracle.brute_periodic(rows, 12, 40)
+    assert len(got) > 10
