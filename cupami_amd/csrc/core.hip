@@ -165,13 +165,14 @@ __device__ uint64_t wave_lower_bound(const uint64_t *__restrict__ rows, uint64_t
     return lo + (uint64_t)__popcll(__ballot(q < hi && rows[q] < x));
 }
 
-__global__ __launch_bounds__(256) void k_bits_tile(const BuildJob *__restrict__ jobs, uint32_t n_jobs) {
+__global__ __launch_bounds__(512) void k_bits_tile(const BuildJob *__restrict__ jobs, uint32_t n_jobs) {
     extern __shared__ unsigned long long tile[];
     __shared__ uint64_t range[2];
-    uint32_t lo_j = 0, hi_j = n_jobs;  // last job with block0 <= blockIdx.x
+    const uint32_t blk = xcd_contiguous_block();
+    uint32_t lo_j = 0, hi_j = n_jobs;  // last job with block0 <= blk
     while (hi_j - lo_j > 1) {
         const uint32_t m = (lo_j + hi_j) >> 1;
-        if (jobs[m].block0 <= blockIdx.x) lo_j = m; else hi_j = m;
+        if (jobs[m].block0 <= blk) lo_j = m; else hi_j = m;
     }
     const BuildJob &j = jobs[lo_j];
     const uint64_t *__restrict__ rows = j.rows;
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(256) void k_bits_tile(const BuildJob *__restrict__ 
     const uint32_t n_items = j.n_items, B = j.B;
     const uint64_t stride = j.stride, nnz = j.nnz;
     const uint32_t tid = threadIdx.x;
-    const uint64_t w0 = (blockIdx.x - j.block0) * B;
+    const uint64_t w0 = (blk - j.block0) * B;
     const uint32_t cells = n_items * B;
     for (uint32_t i = tid; i < cells; i += blockDim.x) tile[i] = 0;
     // [lo, hi): pairs whose row lies in this word range, the two ends found
@@ -613,7 +614,7 @@ int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bi
             if (hipMemcpyAsync(d_jobs, h_jobs, (size_t)nt * sizeof(BuildJob), hipMemcpyHostToDevice, st) != hipSuccess)
                 status = fail(CPM_ERR_RUNTIME, "job table upload failed");
             else
-                hipLaunchKernelGGL(k_bits_tile, dim3((unsigned)blocks), dim3(256), lds, st, d_jobs, nt);
+                hipLaunchKernelGGL(k_bits_tile, dim3((unsigned)blocks), dim3(512), lds, st, d_jobs, nt);
         }
         for (uint32_t i = 0; i < n && status == CPM_OK; ++i) {
             if (tiled[i] || !in[i].nnz) continue;

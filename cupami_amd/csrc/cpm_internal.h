@@ -66,9 +66,21 @@ constexpr uint32_t kTileWords = 2048;
 constexpr uint32_t kBlock = 256;
 
 // Words per workgroup of the LDS tile store builds: the tile (n_items x B
-// words) stays within 64 KB.  0: too many items for it (global atomics).
-inline uint32_t tile_words_for(uint32_t n_items) {
-    return n_items ? std::min<uint32_t>(8, 8192 / std::max<uint32_t>(n_items, 1)) : 0;
+// words) stays within 32 KB up to 4096 items (four 512-lane workgroups fit a
+// CU's LDS), one word (56 KB, under 64 KB with the kernels' own LDS) up to
+// 7168; 0 above (global atomics).
+inline uint32_t tile_words_for(uint64_t n_items) {
+    if (n_items == 0 || n_items > 7168) return 0;
+    return n_items <= 4096 ? (uint32_t)std::min<uint64_t>(8, 4096 / n_items) : 1;
+}
+
+// Workgroup order for builds whose workgroups own consecutive word ranges:
+// workgroups run on XCD blockIdx % 8, and each XCD takes one contiguous run of
+// the ranges, so the partial 128-byte lines of neighbouring workgroups meet in
+// the same L2 before they are written back.  A bijection on [0, gridDim.x).
+__device__ __forceinline__ uint32_t xcd_contiguous_block() {
+    const uint32_t G = gridDim.x, x8 = blockIdx.x % 8, j = blockIdx.x / 8, per = G / 8, rem = G % 8;
+    return x8 < rem ? x8 * (per + 1) + j : rem * (per + 1) + (x8 - rem) * per + j;
 }
 
 }  // namespace cpm

@@ -322,12 +322,7 @@ __global__ __launch_bounds__(512) void k_ingest_tile(const uint64_t *__restrict_
     extern __shared__ unsigned long long tile[];
     __shared__ uint64_t rs[kRowsPerTileMax + 1];
     const uint32_t tid = threadIdx.x;
-    // workgroups run on XCD blockIdx % 8; each XCD takes one contiguous run of
-    // word ranges, so the partial 128-byte lines of neighbouring workgroups
-    // meet in the same L2 before they are written back
-    const uint32_t G = gridDim.x, x8 = blockIdx.x % 8, j = blockIdx.x / 8, per = G / 8, rem = G % 8;
-    const uint32_t blk = x8 < rem ? x8 * (per + 1) + j : rem * (per + 1) + (x8 - rem) * per + j;
-    const uint64_t w0 = (uint64_t)blk * B;
+    const uint64_t w0 = (uint64_t)xcd_contiguous_block() * B;
     const uint32_t cells = n_items * B;
     uint16_t *lmap = reinterpret_cast<uint16_t *>(tile + cells);
     for (uint32_t i = tid; i < cells; i += blockDim.x) tile[i] = 0;
@@ -379,14 +374,6 @@ __global__ __launch_bounds__(256) void k_ingest_scatter(const uint64_t *__restri
         for (uint64_t i = a; i < b; ++i)
             atomicOr(&store[(uint64_t)cm(ids[i]) * stride + (r >> 6)], 1ull << (r & 63));
     }
-}
-
-// Words per workgroup of the ingest tile: the tile stays within 32 KB up to
-// 4096 columns (four workgroups fit a CU's LDS), one word (56 KB, with the row
-// starts under 64 KB) up to 7168; 0 above (global atomics).
-uint32_t ingest_tile_words(uint64_t n_items) {
-    if (n_items == 0 || n_items > 7168) return 0;
-    return n_items <= 4096 ? (uint32_t)std::min<uint64_t>(8, 4096 / n_items) : 1;
 }
 
 unsigned grid_for(cpm_ctx *ctx, uint64_t n, uint64_t per_cu = 8) {
@@ -442,7 +429,7 @@ int build_store_csr(cpm_ctx *ctx, const uint64_t *d_off, uint64_t n, const uint3
     if (n_items > 0xFFFFFFFFull) return fail(CPM_ERR_INVALID, "too many distinct items");
     if (n_rows >= (1ull << 40)) return fail(CPM_ERR_INVALID, "too many transactions");
 
-    const uint32_t B = ingest_tile_words(n_items);
+    const uint32_t B = tile_words_for(n_items);
     cpm_bits *b = nullptr;
     CPM_CHECK(bits_alloc(ctx, n_rows, (uint32_t)n_items, &b, B == 0));
     int s = b->labels.ensure(std::max<uint64_t>(n_items, 1) * 4);

@@ -41,7 +41,7 @@ def test_bits_build(ctx, n, row_sorted):
 
 def test_bits_build_many_items_and_reuse(ctx):
     # item counts on both sides of the LDS tile limit, store buffers reused
-    for items in (1, 1000, 8192, 9000):
+    for items in (1, 1000, 4096, 4097, 7168, 7169, 9000):
         rows, cols = random_store(items, 3000, items, 0.002)
         for _ in range(2):
             bits = ctx.build_bits(rows, cols, 3000, items)
