@@ -12,6 +12,7 @@ from __future__ import annotations
 import ctypes
 import os
 import threading
+from collections.abc import Mapping
 from pathlib import Path
 
 import numpy as np
@@ -553,6 +554,23 @@ class Patterns:
             out[i] = (c,) if p == NO_PARENT else out[p] + (c,)
         return out
 
+    def split_arrays(self, n_db):
+        """Per-database results as ``PatternArrays`` (the forest cut by
+        database with numpy: a stable sort by database keeps parents before
+        children, and parents are renumbered within each database)."""
+        counts = np.bincount(self.db, minlength=n_db).astype(np.int64)
+        starts = np.zeros(n_db + 1, dtype=np.int64)
+        starts[1:] = np.cumsum(counts)
+        order = np.argsort(self.db, kind="stable")
+        local = np.empty(self.n, dtype=np.int64)
+        local[order] = np.arange(self.n, dtype=np.int64) - np.repeat(starts[:-1], counts)
+        parent = self.parent[order]
+        has = parent != NO_PARENT
+        parent = parent.copy()
+        parent[has] = local[parent[has]].astype(np.uint32)
+        item, support = self.item[order], self.support[order]
+        return [PatternArrays(parent[a:b], item[a:b], support[a:b]) for a, b in zip(starts[:-1], starts[1:])]
+
     def split(self, n_db):
         """Per-database ``{column tuple: support}`` dictionaries."""
         sets = self.itemsets()
@@ -598,6 +616,47 @@ class Patterns:
         if self.periodic:
             return dict(zip(sets, zip(sup, self.period.tolist())))
         return dict(zip(sets, sup))
+
+
+class PatternArrays(Mapping):
+    """One database's result as arrays (parent, item, support: the forest
+    of ``Patterns``), read as the mapping ``{column tuple: support}``.  The
+    dict is built on first use; ``len`` needs only the arrays."""
+
+    def __init__(self, parent, item, support):
+        self.parent, self.item, self.support = parent, item, support
+        self._dict = None
+
+    @classmethod
+    def of(cls, patterns):
+        return cls(patterns.parent, patterns.item, patterns.support)
+
+    def _materialise(self):
+        if self._dict is None:
+            sets = [None] * len(self.item)
+            for i, p, c in zip(range(len(sets)), self.parent.tolist(), self.item.tolist()):
+                sets[i] = (c,) if p == NO_PARENT else sets[p] + (c,)
+            self._dict = dict(zip(sets, self.support.tolist()))
+        return self._dict
+
+    def __len__(self):
+        return len(self.item)
+
+    def __getitem__(self, key):
+        return self._materialise()[key]
+
+    def __iter__(self):
+        return iter(self._materialise())
+
+    def __eq__(self, other):
+        if isinstance(other, Mapping):
+            return self._materialise() == dict(other.items())
+        return NotImplemented
+
+    __hash__ = None
+
+    def __repr__(self):
+        return f"PatternArrays({len(self)} patterns)"
 
 
 _contexts = {}

@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import time
 from collections import deque
+from collections.abc import Mapping
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -60,7 +61,7 @@ class Outcome:
     rid: int
     arrival: float
     finish: float = 0.0
-    patterns: dict = field(default_factory=dict)
+    patterns: Mapping = field(default_factory=dict)   # {column tuple: support}, built on first read
     batch: int = -1
     rejected: str = ""
 
@@ -96,7 +97,7 @@ class Server:
             r = batch[0]
             bits = self.ctx.build_bits(r.rows, r.cols, r.n_transactions, r.n_items)
             try:
-                return [self.ctx.eclat_mine(bits, r.minsup).to_dict()]
+                return [_hip.PatternArrays.of(self.ctx.eclat_mine(bits, r.minsup))]
             finally:
                 bits.close()
         stores = self.ctx.build_bits_batch([(r.rows, r.cols, r.n_transactions, r.n_items) for r in batch])
@@ -105,7 +106,7 @@ class Server:
         finally:
             for s in stores:
                 s.close()
-        return forest.split(len(batch))
+        return forest.split_arrays(len(batch))
 
     def run(self, requests):
         """Serve ``requests`` (any order; arrival times relative to now).
@@ -151,7 +152,7 @@ def mine_one_at_a_time(requests, ctx=None):
             time.sleep(r.arrival - now)
         bits = ctx.build_bits(r.rows, r.cols, r.n_transactions, r.n_items)
         try:
-            pats = ctx.eclat_mine(bits, r.minsup).to_dict()
+            pats = _hip.PatternArrays.of(ctx.eclat_mine(bits, r.minsup))
         finally:
             bits.close()
         o = Outcome(r.rid, r.arrival)

@@ -11,7 +11,7 @@ class StubForest:
     def __init__(self, n):
         self.n = n
 
-    def split(self, n_db):
+    def split_arrays(self, n_db):
         return [{("db", i): 1} for i in range(n_db)]
 
 
@@ -35,8 +35,10 @@ class StubCtx:
 
 
 class StubPatterns:
-    def to_dict(self):
-        return {("alone",): 1}
+    # a one-pattern forest: column 0, support 1
+    parent = np.array([0xFFFFFFFF], dtype=np.uint32)
+    item = np.array([0], dtype=np.uint32)
+    support = np.array([1], dtype=np.uint64)
 
 
 class StubStore:
