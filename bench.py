@@ -616,41 +616,54 @@ def run_serve(args):
 
     t = time.perf_counter()
     raw = datagen.serve_requests(args.requests, args.qps)
-    reqs = []
+    reqs, pair_reqs = [], []
     for i, (arrival, db, ms) in enumerate(raw):
         labels, cols = datagen.label_columns(db.items)
-        reqs.append(Request(i, arrival, db.rows, cols, db.n_transactions, len(labels), ms))
-    pairs = sum(r.rows.size for r in reqs)
+        offsets = np.searchsorted(db.rows, np.arange(db.n_transactions + 1, dtype=np.uint64)).astype(np.uint64)
+        # the BIN layout (offsets + raw ids; the device finds the labels) and
+        # the pairs layout (rows + label columns) of the same database
+        reqs.append(Request(i, arrival, None, None, db.n_transactions, len(labels), ms,
+                            offsets=offsets, ids=db.items, universe=db.n_items))
+        pair_reqs.append(Request(i, arrival, db.rows, cols, db.n_transactions, len(labels), ms))
+    pairs = sum(r.entries for r in reqs)
     log(f"bench: generated {len(reqs)} requests ({pairs} pairs) in {time.perf_counter() - t:.1f}s")
 
     ctx = _hip.context(0)
 
-    # both schedules twice, alternating (the first runs grow the device
-    # buffers); each keeps its faster run
-    served_s = alone_s = float("inf")
-    for _ in range(2):
-        srv = Server(ctx, hbm_budget=int(args.hbm_budget_gb * (1 << 30)))
-        t = time.perf_counter()
-        out = srv.run(reqs)
-        dt = time.perf_counter() - t
-        if dt < served_s:
-            served_s, served, server = dt, out, srv
-        t = time.perf_counter()
-        out = mine_one_at_a_time(reqs, ctx)
-        dt = time.perf_counter() - t
-        if dt < alone_s:
-            alone_s, alone = dt, out
+    def schedules(requests):
+        # both schedules twice, alternating (the first runs grow the device
+        # buffers); each keeps its faster run
+        best = {"served": (float("inf"), None, None), "alone": (float("inf"), None, None)}
+        for _ in range(2):
+            srv = Server(ctx, hbm_budget=int(args.hbm_budget_gb * (1 << 30)))
+            t = time.perf_counter()
+            out = srv.run(requests)
+            dt = time.perf_counter() - t
+            if dt < best["served"][0]:
+                best["served"] = (dt, out, srv)
+            t = time.perf_counter()
+            out = mine_one_at_a_time(requests, ctx)
+            dt = time.perf_counter() - t
+            if dt < best["alone"][0]:
+                best["alone"] = (dt, out, None)
+        return best
+
+    best = schedules(reqs)
+    served_s, served, server = best["served"]
+    alone_s, alone, _ = best["alone"]
+    by_pairs = schedules(pair_reqs)
     n_pat = sum(len(o.patterns) for o in served)
     sizes = [len(b[2]) for b in server.batches]
     log(f"bench: served {len(reqs)} requests in {served_s:.2f}s ({len(sizes)} batches, max {max(sizes)}), "
-        f"one at a time {alone_s:.2f}s")
+        f"one at a time {alone_s:.2f}s; pairs input {by_pairs['served'][0]:.2f}s / {by_pairs['alone'][0]:.2f}s")
 
-    parity = all(a.patterns == b.patterns for a, b in zip(served, alone) if not a.rejected)
+    parity = all(a.patterns == b.patterns == c.patterns
+                 for a, b, c in zip(served, alone, by_pairs["served"][1]) if not a.rejected)
     cpu = None
     if not args.no_cpu_baseline:
         import oracle
         threads = oracle.threads()
-        sample = reqs[: max(1, len(reqs) // 3)]
+        sample = pair_reqs[: max(1, len(reqs) // 3)]
         hosts = [oracle.bits_build(r.rows, r.cols, r.n_transactions, r.n_items) for r in sample]
         t = time.perf_counter()
         expect = [oracle.eclat(h, r.minsup) for h, r in zip(hosts, sample)]
@@ -676,6 +689,11 @@ def run_serve(args):
                    "rejected": sum(1 for o in served if o.rejected)},
         "one_at_a_time": {**latency_summary(alone), "itemsets_per_s": round(n_pat / alone_s, 1),
                           "makespan_s": round(alone_s, 3)},
+        "input": "BIN layout (offsets + raw ids, store and labels built on the device)",
+        "pairs_input": {"served": {**latency_summary(by_pairs["served"][1]),
+                                   "makespan_s": round(by_pairs["served"][0], 3)},
+                        "one_at_a_time": {**latency_summary(by_pairs["alone"][1]),
+                                          "makespan_s": round(by_pairs["alone"][0], 3)}},
         "parity": parity,
     }
     return line

@@ -10,7 +10,7 @@ import oracle
 from cupami_amd import _hip
 from cupami_amd.batch import BatchMiner
 from cupami_amd.serve import Request, Server, mine_one_at_a_time
-from helpers import coo, random_rows
+from helpers import coo, csr, random_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -49,6 +49,33 @@ def test_server_results_equal_alone_and_oracle(ctx):
         assert not s.rejected
         assert s.patterns == expect and a.patterns == expect
     assert small > 0
+
+
+def test_server_bin_layout_requests(ctx):
+    """Requests in the BIN layout (offsets + raw ids, empty baskets and
+    repeats included; the device builds store and labels), mixed in batches
+    with pairs requests: each result == the oracle on the reference's COO of
+    the same arrays (cuApriori.py:381-396)."""
+    rng = random.Random(8)
+    reqs, expect = [], []
+    for i in range(8):
+        pool = rng.sample(range(1, 400), 6 + 2 * i)
+        rows = [[rng.choice(pool) for _ in range(rng.randrange(7))] for _ in range(150 + 60 * i)]
+        offsets, ids = csr(rows)
+        labels, rr, cc, n = oracle.csr_to_coo(offsets, ids)
+        expect.append(oracle.eclat(oracle.bits_build(rr, cc, n, len(labels)), 3 + i))
+        if i % 3 == 2:   # all arrive at once, so they queue and batch
+            reqs.append(Request(i, 0.0, rr, cc, n, len(labels), 3 + i))
+        else:
+            reqs.append(Request(i, 0.0, None, None, n, len(labels), 3 + i,
+                                offsets=offsets, ids=ids, universe=399))
+    server = Server(ctx, hbm_budget=3 * max(r.footprint() for r in reqs), max_batch=3)
+    served = server.run(reqs)
+    alone = mine_one_at_a_time(reqs, ctx)
+    assert any(len(b[2]) > 1 for b in server.batches)
+    for s, a, e in zip(served, alone, expect):
+        assert not s.rejected and len(e) > 0
+        assert s.patterns == e and a.patterns == e
 
 
 def test_batch_miner_tune_and_mine(ctx):
