@@ -113,6 +113,27 @@ def test_csr_build_device_pointers(ctx):
         bits.close()
 
 
+def test_csr_build_unaligned_ids(ctx):
+    """Ids at an address that is not 16-byte aligned (as in a mapped BIN
+    file, where they follow 32 + 8(N+1) bytes) take the input scan's scalar
+    path; the store is the same."""
+    import torch
+    offsets, ids = random_csr(12, 3000, 300, 250, 9)
+    d_off = torch.from_numpy(offsets.view(np.int64)).cuda()
+    padded = np.concatenate([np.zeros(1, np.uint32), ids])
+    d_ids = torch.from_numpy(padded.view(np.int32)).cuda()
+    torch.cuda.synchronize()
+    addr = d_ids.data_ptr() + 4
+    assert addr % 16 != 0
+    bits, label_ids = ctx.build_bits_csr_device(d_off.data_ptr(), offsets.size - 1, addr, ids.size, 300)
+    try:
+        labels, rr, cc, n = oracle.csr_to_coo(offsets, ids)
+        assert [str(v) for v in label_ids.tolist()] == labels
+        assert np.array_equal(bits.download(), oracle.bits_build(rr, cc, n, len(labels)))
+    finally:
+        bits.close()
+
+
 def test_csr_build_full_size_1m_x_1k(ctx):
     """The 1M-basket, 1000-item database of the Eclat benchmark, as a
     CUPAMI01 file would hold it: same store as the pairs path."""
