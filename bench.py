@@ -92,7 +92,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(each GPU would mine its own prefix classes; see DESIGN.md).", file=sys.stderr)
+              "(split by prefix class after level 2, each GPU would mine its own classes from a replicated "
+              "store).", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batch_256":
