@@ -32,6 +32,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "itemsets/s + bitset GB/s (Eclat 1M×1k), batched vs plain; serve p99 latency"
+PF_METRIC = "itemsets/s + bitset GB/s (PF-Eclat 2M×500)"
 HBM_PEAK_GBS = 8000.0
 MINSUP = 1000  # 0.001 x 1,000,000
 PF_MINSUP, PF_MAXPER = 10_000, 2_000  # 0.005 x 2,000,000; maxPer 2,000
@@ -209,8 +210,8 @@ def main():
 
     kernel = "k_support_periodic" if periodic else "k_support_count"
     roofline["kernel"] = kernel
-    if not periodic:
-        roofline["traffic"] = measured_traffic()
+    roofline["traffic"] = (measured_traffic("pfeclat_2m_500_hbm_traffic.md", "k_support_periodic_hbm_bytes_per_launch")
+                           if periodic else measured_traffic())
     if periodic:
         config = {"workload": "pfeclat_2m_500", "transactions": n, "items": items, "lambda": 8,
                   "zipf_s": 1.05, "minsup": minsup, "maxper": PF_MAXPER, "patterns": n_pat,
@@ -220,7 +221,7 @@ def main():
                   "lambda": 10, "zipf_s": 1.1, "minsup": minsup, "patterns": n_pat,
                   "parallelism": "single GPU"}
     line = {
-        "metric": METRIC, "value": round(value, 1), "unit": "itemsets/s", "n_gpus": 1,
+        "metric": PF_METRIC if periodic else METRIC, "value": round(value, 1), "unit": "itemsets/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": f"synthetic (numpy PCG64 seed {404 if periodic else 202})",
