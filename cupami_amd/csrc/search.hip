@@ -924,6 +924,108 @@ __global__ void k_level2(const L2Db *l2, uint32_t n_db, uint32_t n_tasks, LevelP
     }
 }
 
+// Periodic searches: level 2 holds only the pairs whose support (from the
+// Gram) reaches minsup.  A pair below it cannot be periodic-frequent
+// (keep_entry), so the periodic scan, which walks every occurrence of a pair,
+// runs on the frequent pairs alone.  Task t (item a of its database) keeps
+// the items b > a with G[a][b] >= minsup: counted, scanned into ext_off,
+// then written.
+__device__ __forceinline__ uint32_t l2_db_of(const L2Db *l2, uint32_t n_db, uint32_t t) {
+    uint32_t lo = 0, hi = n_db;  // the largest database with task0 <= t owns t
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (l2[mid].task0 <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_level2_frequent_count(const L2Db *l2, uint32_t n_db, uint32_t n_tasks,
+                                                              const DbDev *dbs, const int32_t *gram,
+                                                              const uint64_t *goff, uint32_t *cnt) {
+    __shared__ uint32_t lds[8];
+    for (uint32_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {
+        const uint32_t i = l2_db_of(l2, n_db, t);
+        const DbDev &D = dbs[i];
+        const uint32_t a = t - l2[i].task0;
+        const int32_t *g = gram + goff[i] + (uint64_t)a * D.nf;
+        uint32_t c = 0;
+        for (uint32_t b = a + 1 + threadIdx.x; b < D.nf; b += blockDim.x) c += (uint64_t)g[b] >= D.minsup;
+        uint32_t tot;
+        (void)block_scan(c, lds, &tot);
+        if (threadIdx.x == 0) cnt[t] = tot;
+    }
+}
+
+// One workgroup: ext_off = exclusive scan of cnt (ext_off[n_tasks] = total)
+// and level 2's sizes from the total.
+__global__ __launch_bounds__(1024) void k_level2_frequent_scan(const uint32_t *cnt, uint32_t n_tasks, uint32_t *ext_off,
+                                                              LevelSizes *sz, uint32_t G, uint32_t te,
+                                                              uint64_t work_target) {
+    __shared__ uint32_t part[16];
+    __shared__ uint32_t carry_s;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) carry_s = 0;
+    __syncthreads();
+    for (uint32_t i0 = 0; i0 < n_tasks; i0 += blockDim.x) {
+        const uint32_t i = i0 + tid;
+        const uint32_t v = i < n_tasks ? cnt[i] : 0;
+        uint32_t x = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, 64);
+            if (lane >= (uint32_t)d) x += y;
+        }
+        if (lane == 63) part[wave] = x;
+        __syncthreads();
+        uint32_t base = carry_s, all = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+            if (w < wave) base += part[w];
+            all += part[w];
+        }
+        if (i < n_tasks) ext_off[i] = base + x - v;
+        __syncthreads();
+        if (tid == 0) carry_s += all;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const uint32_t total = carry_s;
+        ext_off[n_tasks] = total;
+        sz->n_ext = total;
+        sz->n_cand = total;
+        sz->group = level_group(total, G, te, work_target);
+        sz->n_work = level_work(total, (uint32_t)sz->group, te);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_level2_frequent_write(const L2Db *l2, uint32_t n_db, uint32_t n_tasks,
+                                                              LevelPtrs L, const DbDev *dbs, const int32_t *gram,
+                                                              const uint64_t *goff) {
+    __shared__ uint32_t lds[8];
+    for (uint32_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {
+        const uint32_t i = l2_db_of(l2, n_db, t);
+        const DbDev &D = dbs[i];
+        const uint32_t a = t - l2[i].task0;
+        const int32_t *g = gram + goff[i] + (uint64_t)a * D.nf;
+        if (threadIdx.x == 0) {
+            L.prefix[(uint64_t)t * kMaxDepth] = D.fbase + a;
+            L.pid[t] = D.fbase + a;
+            L.db[t] = i;
+        }
+        uint32_t base = L.ext_off[t];
+        for (uint32_t b0 = a + 1; b0 < D.nf; b0 += blockDim.x) {
+            const uint32_t b = b0 + threadIdx.x;
+            const uint32_t f = b < D.nf && (uint64_t)g[b] >= D.minsup;
+            uint32_t tot;
+            const uint32_t x = block_scan(f, lds, &tot);
+            if (f) {
+                L.ext[base + x] = D.fbase + b;
+                L.sup[base + x] = 0;
+            }
+            base += tot;
+        }
+    }
+}
+
 // level-2 supports from the per-database Gram matrices
 __global__ void k_gram_to_level2(LevelPtrs L, const DbDev *dbs, const int32_t *gram, const uint64_t *goff) {
     const uint32_t n = (uint32_t)L.sz->n_tasks;
@@ -1513,7 +1615,7 @@ struct Search {
             const uint32_t e2u = (uint32_t)e2;
             qput(S1.ext_off.as<uint32_t>() + t2, &e2u, 4);
         }
-        bool gram_ok = use_gram && !periodic;
+        bool gram_ok = use_gram;
         for (uint32_t i = 0; i < n_db && gram_ok; ++i)
             if (nf[i] > 16384 || dbs[i].n >= (1ull << 31)) gram_ok = false;
         std::vector<GramJob> jobs;
@@ -1560,18 +1662,34 @@ struct Search {
         }
         CPM_CHECK(flush());
 
-        hipLaunchKernelGGL(k_level2, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0, st,
-                           w.l2.as<L2Db>(), n_db, (uint32_t)t2, S1.ptrs(sz(1)), w.dbs.as<DbDev>());
-        CPM_HIP(hipGetLastError());
-        if (gram_ok) {
+        const unsigned g2 = (unsigned)std::min<uint64_t>(t2, 8192);
+        if (gram_ok && periodic) {
+            // level 2 = the frequent pairs (Gram supports), then their periods
             CPM_CHECK(launch_gram(ctx, w.gjobs.as<GramJob>(), jobs.size(), w.gdbs.as<GramDb>(),
                                   w.gram.as<int32_t>(), gram_elems, gram_ops, false));
-            hipLaunchKernelGGL(k_gram_to_level2, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0,
-                               st, S1.ptrs(sz(1)), w.dbs.as<DbDev>(), w.gram.as<int32_t>(),
+            uint32_t *cnt = S1.cnt[0].as<uint32_t>();
+            hipLaunchKernelGGL(k_level2_frequent_count, dim3(g2), dim3(256), 0, st, w.l2.as<L2Db>(), n_db,
+                               (uint32_t)t2, w.dbs.as<DbDev>(), w.gram.as<int32_t>(), w.goff.as<uint64_t>(), cnt);
+            hipLaunchKernelGGL(k_level2_frequent_scan, dim3(1), dim3(1024), 0, st, cnt, (uint32_t)t2,
+                               S1.ext_off.as<uint32_t>(), sz(1), G, te, work_target());
+            hipLaunchKernelGGL(k_level2_frequent_write, dim3(g2), dim3(256), 0, st, w.l2.as<L2Db>(), n_db,
+                               (uint32_t)t2, S1.ptrs(sz(1)), w.dbs.as<DbDev>(), w.gram.as<int32_t>(),
                                w.goff.as<uint64_t>());
             CPM_HIP(hipGetLastError());
-        } else {
             CPM_CHECK(count(1, false));
+        } else {
+            hipLaunchKernelGGL(k_level2, dim3(g2), dim3(256), 0, st, w.l2.as<L2Db>(), n_db, (uint32_t)t2,
+                               S1.ptrs(sz(1)), w.dbs.as<DbDev>());
+            CPM_HIP(hipGetLastError());
+            if (gram_ok) {
+                CPM_CHECK(launch_gram(ctx, w.gjobs.as<GramJob>(), jobs.size(), w.gdbs.as<GramDb>(),
+                                      w.gram.as<int32_t>(), gram_elems, gram_ops, false));
+                hipLaunchKernelGGL(k_gram_to_level2, dim3(g2), dim3(256), 0, st, S1.ptrs(sz(1)),
+                                   w.dbs.as<DbDev>(), w.gram.as<int32_t>(), w.goff.as<uint64_t>());
+                CPM_HIP(hipGetLastError());
+            } else {
+                CPM_CHECK(count(1, false));
+            }
         }
         if (pair_words) {
             PlanArgs p = plan_args(1, 1);
@@ -1783,7 +1901,7 @@ static int pfeclat_run(cpm_ctx *ctx, const cpm_bits *bits, const int64_t *d_ts, 
                        int64_t maxper_floor, cpm_patterns **out) {
     Search s(ctx, search_ws(ctx));
     s.periodic = true;
-    s.use_gram = false;
+    s.use_gram = true;  // level 2: the Gram's supports select the pairs to scan
     s.G = ctx->opt_group;
     s.tile_words = kTileWords;
     s.use_graph = ctx->opt_graph;
