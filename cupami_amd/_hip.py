@@ -205,10 +205,11 @@ class Context:
                                            ctypes.byref(handle)))
         return Bits(self, handle)
 
-    def build_bits_csr(self, offsets, ids, universe, keep_empty=False):
+    def build_bits_csr(self, offsets, ids, universe, keep_empty=False, labels=True):
         """Store straight from a basket file's arrays (``offsets`` u64[n+1],
         ``ids`` u32[m]).  Returns the store and the item id of each column
-        (ids in decimal-string order)."""
+        (ids in decimal-string order; None with ``labels=False``, which
+        saves the copy and the wait for them)."""
         offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
         ids = np.ascontiguousarray(ids, dtype=np.uint32)
         if offsets.size < 1:
@@ -217,7 +218,7 @@ class Context:
         check(load().cpm_bits_build_csr(self.handle, _p(offsets, _u64p), offsets.size - 1, _p(ids, _u32p),
                                         ids.size, int(universe), int(bool(keep_empty)), ctypes.byref(handle)))
         bits = Bits(self, handle)
-        return bits, bits.labels()
+        return bits, (bits.labels() if labels else None)
 
     def build_bits_csr_device(self, d_offsets, n_baskets, d_ids, n_ids, universe, keep_empty=False):
         """``build_bits_csr`` with the arrays given as device addresses (ints)."""

@@ -74,7 +74,7 @@ class Request:
 
 def _build(ctx, r):
     if r.ids is not None:
-        return ctx.build_bits_csr(r.offsets, r.ids, r.universe)[0]
+        return ctx.build_bits_csr(r.offsets, r.ids, r.universe, labels=False)[0]
     return ctx.build_bits(r.rows, r.cols, r.n_transactions, r.n_items)
 
 
