@@ -231,11 +231,39 @@ def main():
     }
     if not periodic and not args.no_extras:
         line["batched"], line["serve"] = extras(args)
+        line["quickstart"] = quickstart()
+        if any(v["patterns"] != 1275 for k, v in line["quickstart"].items() if k != "workload"):
+            parity = False
         if line["batched"]["parity"] is False or line["serve"]["parity"] is False:
             parity = False
     print(json.dumps(line), flush=True)
     if parity is False:
         sys.exit(1)
+
+
+def quickstart():
+    """The reference's own recorded case: its quickstart database
+    (TransactionalDatabase(10000, 50, avg_length=6, seed=42), restated in
+    tests/golden/generator.py) mined at minSup 0.01 -- 1275 patterns for
+    every miner.  getMiningTime() here beside the notebook's recorded times
+    (quickstart.ipynb:86-88,127-129,168-170: one first call each, NVRTC JIT
+    included, unstated GPU)."""
+    sys.path.insert(0, str(ROOT / "tests" / "golden"))
+    import generator
+    from cupami_amd import cuApriori, cuEclat, cuFPGrowth
+    rows = generator.transactional(10000, 50, 6, 42)
+    recorded = {"cuApriori": 718.0, "cuEclat": 254.5, "cuFPGrowth": 608.9}
+    out = {"workload": "quickstart 10000x50, minSup 0.01 (the reference's generator, seed 42)"}
+    for cls in (cuApriori, cuEclat, cuFPGrowth):
+        times, n = [], None
+        for _ in range(5):
+            m = cls(rows, 0.01)
+            m.mine()
+            times.append(m.getMiningTime() * 1e3)
+            n = len(m.getPatterns())
+        out[cls.__name__] = {"patterns": n, "first_ms": round(times[0], 2), "best_ms": round(min(times), 2),
+                             "reference_notebook_ms": recorded[cls.__name__]}
+    return out
 
 
 def extras(args):
