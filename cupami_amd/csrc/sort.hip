@@ -173,6 +173,95 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint64_t *__restric
     }
 }
 
+// ---- single-workgroup paths for small inputs -----------------------------------------
+// The FP-tree build sorts and scans many small arrays (conditional trees of a
+// few hundred nodes): one launch each instead of the device-wide passes.
+
+constexpr uint32_t kSmallSort = 2048;  // pairs sorted by one workgroup
+constexpr uint32_t kSmallScan = 4096;  // elements scanned by one workgroup (4 per lane)
+
+// Sort of n <= kSmallSort pairs by (key & mask), stable: bitonic over
+// (masked key, input index), a total order in which equal keys keep their
+// input order.  In place.
+__global__ __launch_bounds__(1024) void k_sort_small(uint64_t *keys, uint32_t *vals, uint32_t n, uint64_t mask) {
+    __shared__ uint64_t sk[kSmallSort];
+    __shared__ uint32_t si[kSmallSort];
+    const uint32_t tid = threadIdx.x;
+    uint32_t P = 1;
+    while (P < n) P <<= 1;
+    for (uint32_t i = tid; i < P; i += blockDim.x) {
+        sk[i] = i < n ? (keys[i] & mask) : ~0ull;  // padding sorts last (index breaks ties)
+        si[i] = i;
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= P; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < P; i += blockDim.x) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const uint64_t a = sk[i], b = sk[l];
+                    const uint32_t ia = si[i], ib = si[l];
+                    const bool gt = a > b || (a == b && ia > ib);
+                    if (gt == ((i & k) == 0)) {
+                        sk[i] = b;
+                        sk[l] = a;
+                        si[i] = ib;
+                        si[l] = ia;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    // every lane reads its sorted pairs before any lane writes (in place)
+    constexpr uint32_t kPer = kSmallSort / 1024;
+    uint64_t ok[kPer];
+    uint32_t ov[kPer];
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+        const uint32_t i = tid + r * blockDim.x;
+        if (i < n) {
+            ok[r] = keys[si[i]];
+            ov[r] = vals[si[i]];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+        const uint32_t i = tid + r * blockDim.x;
+        if (i < n) {
+            keys[i] = ok[r];
+            vals[i] = ov[r];
+        }
+    }
+}
+
+// Exclusive scan of n <= kSmallScan values: out[n] = total (and *d_total).
+template <class OutT>
+__global__ __launch_bounds__(1024) void k_scan_small(const uint32_t *in, uint64_t n, OutT *out, uint64_t *d_total) {
+    __shared__ uint64_t lds[16];
+    const uint32_t tid = threadIdx.x;
+    uint32_t v[4];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint64_t i = (uint64_t)tid * 4 + q;
+        v[q] = i < n ? in[i] : 0;
+        sum += v[q];
+    }
+    uint64_t total;
+    uint64_t run = block_scan_u64(sum, lds, &total);  // its barriers also order reads before writes
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint64_t i = (uint64_t)tid * 4 + q;
+        if (i < n) out[i] = (OutT)run;
+        run += v[q];
+    }
+    if (tid == 0) {
+        out[n] = (OutT)total;
+        if (d_total) *d_total = total;
+    }
+}
+
 __global__ void k_copy_pairs(const uint64_t *__restrict__ k, const uint32_t *__restrict__ v,
                              uint64_t n, uint64_t *__restrict__ ok, uint32_t *__restrict__ ov) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -199,6 +288,11 @@ int SortWs::ensure(uint64_t n) {
 
 int exclusive_scan_u32(cpm_ctx *ctx, SortWs &ws, const uint32_t *in, uint32_t *out, uint64_t n,
                        uint64_t *d_total) {
+    if (n <= kSmallScan) {
+        hipLaunchKernelGGL(k_scan_small<uint32_t>, dim3(1), dim3(1024), 0, ctx->stream, in, n, out, d_total);
+        CPM_HIP(hipGetLastError());
+        return CPM_OK;
+    }
     const uint64_t chunks = std::max<uint64_t>(1, (n + kScanChunk - 1) / kScanChunk);
     CPM_CHECK(ws.part.ensure((chunks + 1) * 8));
     uint64_t *part = ws.part.as<uint64_t>();
@@ -212,6 +306,11 @@ int exclusive_scan_u32(cpm_ctx *ctx, SortWs &ws, const uint32_t *in, uint32_t *o
 
 int exclusive_scan_u32_to_u64(cpm_ctx *ctx, SortWs &ws, const uint32_t *in, uint64_t *out,
                               uint64_t n, uint64_t *d_total) {
+    if (n <= kSmallScan) {
+        hipLaunchKernelGGL(k_scan_small<uint64_t>, dim3(1), dim3(1024), 0, ctx->stream, in, n, out, d_total);
+        CPM_HIP(hipGetLastError());
+        return CPM_OK;
+    }
     const uint64_t chunks = std::max<uint64_t>(1, (n + kScanChunk - 1) / kScanChunk);
     CPM_CHECK(ws.part.ensure((chunks + 1) * 8));
     uint64_t *part = ws.part.as<uint64_t>();
@@ -226,6 +325,14 @@ int exclusive_scan_u32_to_u64(cpm_ctx *ctx, SortWs &ws, const uint32_t *in, uint
 int radix_sort_pairs(cpm_ctx *ctx, SortWs &ws, uint64_t *keys, uint32_t *vals, uint64_t n,
                      int end_bit) {
     if (n <= 1 || end_bit <= 0) return CPM_OK;
+    if (n <= kSmallSort) {
+        // the device-wide passes order by whole 8-bit digits: bits [0, 8 * ceil(end_bit / 8))
+        const int bits = std::min(64, (end_bit + 7) / 8 * 8);
+        const uint64_t mask = bits >= 64 ? ~0ull : (1ull << bits) - 1;
+        hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(1024), 0, ctx->stream, keys, vals, (uint32_t)n, mask);
+        CPM_HIP(hipGetLastError());
+        return CPM_OK;
+    }
     CPM_CHECK(ws.ensure(n));
     const uint32_t tiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
     uint64_t *ka = keys, *kb = ws.keys_tmp.as<uint64_t>();
