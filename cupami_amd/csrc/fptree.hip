@@ -196,6 +196,21 @@ __global__ void k_ancestor_supports(const uint32_t *__restrict__ nodes, uint64_t
     }
 }
 
+// Conditional supports of every item of a tree at once: for each non-root
+// node n of item x and each ancestor a of n below the root,
+// M[x * I + item(a)] += count(n).  Row x is what k_ancestor_supports gives
+// for item x alone.
+__global__ void k_ancestor_matrix(const int64_t *__restrict__ parents, const int64_t *__restrict__ items,
+                                  const unsigned long long *__restrict__ counts, uint64_t n_nodes, uint32_t I,
+                                  unsigned long long *__restrict__ M) {
+    for (uint64_t n = 1 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; n < n_nodes;
+         n += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long w = counts[n];
+        unsigned long long *row = M + (uint64_t)items[n] * I;
+        for (int64_t p = parents[n]; p > 0; p = parents[p]) atomicAdd(row + items[p], w);
+    }
+}
+
 __global__ void k_proj_lengths(const uint32_t *__restrict__ nodes, uint64_t n,
                                const int64_t *__restrict__ parents, const int64_t *__restrict__ items,
                                const uint32_t *__restrict__ frequent, uint32_t *__restrict__ len) {
@@ -457,6 +472,9 @@ static int build_links(cpm_ctx *ctx, FpWs &ws, cpm_fptree *t) {
     return CPM_OK;
 }
 
+static int conditional_from(cpm_ctx *ctx, FpWs &ws, cpm_fptree *t, uint32_t item, uint64_t minsup,
+                            const unsigned long long *csup, const uint64_t *hs, cpm_fptree **out);
+
 // Conditional tree of `item` (null when no ancestor item is frequent).
 int fptree_conditional(cpm_ctx *ctx, FpWs &ws, cpm_fptree *t, uint32_t item, uint64_t minsup,
                        cpm_fptree **out) {
@@ -478,6 +496,21 @@ int fptree_conditional(cpm_ctx *ctx, FpWs &ws, cpm_fptree *t, uint32_t item, uin
     CPM_HIP(hipGetLastError());
     std::vector<uint64_t> hs(I);
     CPM_CHECK(copy_to_host(ctx, hs.data(), csup, (size_t)I * 8));
+    return conditional_from(ctx, ws, t, item, minsup, csup, hs.data(), out);
+}
+
+// The conditional tree of `item` from its conditional supports, on the device
+// (csup) and on the host (hs), I values each.
+static int conditional_from(cpm_ctx *ctx, FpWs &ws, cpm_fptree *t, uint32_t item, uint64_t minsup,
+                            const unsigned long long *csup, const uint64_t *hs, cpm_fptree **out) {
+    *out = nullptr;
+    CPM_CHECK(build_links(ctx, ws, t));
+    hipStream_t st = ctx->stream;
+    const uint32_t a = t->link_off[item], b = t->link_off[item + 1];
+    const uint64_t nn = b - a;
+    if (nn == 0) return CPM_OK;
+    const uint32_t *nodes = t->link_nodes.as<uint32_t>() + a;
+    const uint32_t I = t->item_count;
     std::vector<uint32_t> header;
     for (uint32_t i = 0; i < I; ++i)
         if (hs[i] >= minsup) header.push_back(i);
@@ -723,13 +756,23 @@ int cpm_fpgrowth_mine(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, 
         res->support.push_back(support);
         return (uint32_t)(res->parent.size() - 1);
     };
+    // A tree's conditional supports for all its items, computed when the
+    // tree is first visited (items up to kMatrixItems): one launch and one
+    // read instead of one of each per item, and items whose conditional tree
+    // would be empty or a single node need no device work.
+    constexpr uint32_t kMatrixItems = 1448;  // 16 MB of u64 supports
+    struct Mat {
+        DBuf d;
+        std::vector<uint64_t> h;
+    };
     struct Frame {
         cpm_fptree *tree;
         uint32_t pid;  // pattern id of the suffix (kNoParent for the empty suffix)
         size_t position;
+        std::shared_ptr<Mat> mat;
     };
     std::vector<Frame> stack;
-    stack.push_back({root, kNoParent, 0});
+    stack.push_back({root, kNoParent, 0, nullptr});
     while (!stack.empty() && s == CPM_OK) {
         Frame &f = stack.back();
         cpm_fptree *t = f.tree;
@@ -763,12 +806,45 @@ int cpm_fpgrowth_mine(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, 
             stack.pop_back();
             continue;
         }
+        if (f.position == 0 && !f.mat && t->item_count <= kMatrixItems) {
+            auto m = std::make_shared<Mat>();
+            const uint64_t I = t->item_count, cells = I * I;
+            if ((s = m->d.ensure(cells * 8)) != CPM_OK) break;
+            if (hipMemsetAsync(m->d.p, 0, cells * 8, st) != hipSuccess) {
+                s = fail(CPM_ERR_RUNTIME, "hipMemsetAsync failed");
+                break;
+            }
+            hipLaunchKernelGGL(k_ancestor_matrix, grid_of(t->n_nodes), dim3(256), 0, st, t->parents.as<int64_t>(),
+                               t->items.as<int64_t>(), t->counts.as<unsigned long long>(), t->n_nodes, (uint32_t)I,
+                               m->d.as<unsigned long long>());
+            m->h.resize(cells);
+            if ((s = copy_to_host(ctx, m->h.data(), m->d.p, cells * 8)) != CPM_OK) break;
+            f.mat = m;
+        }
         const size_t pos = f.position++;
         const uint32_t item = t->header_item[pos];
         const uint32_t id = emit(f.pid, item, t->header_sup[pos]);
         cpm_fptree *cond = nullptr;
-        s = fptree_conditional(ctx, ws, t, item, minsup, &cond);
-        if (s == CPM_OK && cond) stack.push_back({cond, id, 0});
+        if (f.mat) {
+            const uint64_t I = t->item_count;
+            const uint64_t *row = f.mat->h.data() + (uint64_t)item * I;
+            uint32_t nfreq = 0, only = 0;
+            for (uint32_t i = 0; i < I; ++i)
+                if (row[i] >= minsup) {
+                    ++nfreq;
+                    only = i;
+                }
+            if (nfreq == 0) continue;  // no frequent ancestor: no conditional tree
+            if (nfreq == 1) {          // a single-node conditional tree: its one pattern
+                emit(id, only, row[only]);
+                continue;
+            }
+            s = conditional_from(ctx, ws, t, item, minsup, f.mat->d.as<unsigned long long>() + (uint64_t)item * I,
+                                 row, &cond);
+        } else {
+            s = fptree_conditional(ctx, ws, t, item, minsup, &cond);
+        }
+        if (s == CPM_OK && cond) stack.push_back({cond, id, 0, nullptr});
     }
     for (auto &f : stack) cpm_fptree_destroy(f.tree);
     res->n = res->parent.size();
