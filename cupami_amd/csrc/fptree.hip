@@ -287,6 +287,19 @@ static int read_u64(cpm_ctx *ctx, FpWs &ws, const void *d, uint64_t *out) {
     return CPM_OK;
 }
 
+// Two device regions to host memory with one wait (through the pinned buffer).
+static int read_two(cpm_ctx *ctx, FpWs &ws, const void *d0, size_t b0, void *h0, const void *d1, size_t b1, void *h1) {
+    const size_t o1 = (b0 + 63) / 64 * 64;
+    CPM_CHECK(ws.host.ensure(o1 + b1 + 64));
+    unsigned char *h = ws.host.as<unsigned char>();
+    if (b0) CPM_HIP(hipMemcpyAsync(h, d0, b0, hipMemcpyDeviceToHost, ctx->stream));
+    if (b1) CPM_HIP(hipMemcpyAsync(h + o1, d1, b1, hipMemcpyDeviceToHost, ctx->stream));
+    CPM_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(h0, h, b0);
+    memcpy(h1, h + o1, b1);
+    return CPM_OK;
+}
+
 // Build a tree from weighted baskets given as (row, col) pairs in device
 // memory.  d_supports (item_count u64) skips the support count and the
 // infrequent-pair filter (conditional bases hold frequent items only).
@@ -343,8 +356,8 @@ int fptree_build(cpm_ctx *ctx, FpWs &ws, const uint64_t *d_rows, const uint32_t 
     // header: frequent items in ascending (support, then descending id)
     std::vector<uint32_t> order(item_count);
     std::vector<uint64_t> hsup(item_count);
-    CPM_CHECK(copy_to_host(ctx, order.data(), vals, (size_t)item_count * 4));
-    CPM_CHECK(copy_to_host(ctx, hsup.data(), sup, (size_t)item_count * 8));
+    CPM_CHECK(read_two(ctx, ws, vals, (size_t)item_count * 4, order.data(), sup, (size_t)item_count * 8,
+                       hsup.data()));
 
     // baskets sorted by (row, rank) with their lengths
     CPM_CHECK(ws.lengths.ensure((n_rows + 1) * 4));
@@ -365,14 +378,10 @@ int fptree_build(cpm_ctx *ctx, FpWs &ws, const uint64_t *d_rows, const uint32_t 
     CPM_HIP(hipMemsetAsync(ws.scalar.p, 0, 16, st));
     hipLaunchKernelGGL(k_max_u32, grid_of(n_rows), dim3(256), 0, st, ws.lengths.as<uint32_t>(), n_rows,
                        ws.scalar.as<unsigned int>());
-    uint64_t depth64 = 0;
-    CPM_CHECK(read_u64(ctx, ws, ws.scalar.p, &depth64));
-    const uint32_t depth = (uint32_t)(depth64 & 0xffffffffu);
 
     cpm_fptree *t = new cpm_fptree();
     t->ctx = ctx;
     t->item_count = item_count;
-    t->depth = depth;
     int s = CPM_OK;
     auto done = [&](int status) {
         if (status != CPM_OK) delete t;
@@ -399,8 +408,11 @@ int fptree_build(cpm_ctx *ctx, FpWs &ws, const uint64_t *d_rows, const uint32_t 
                                  ws.total.as<uint64_t>()));
     hipLaunchKernelGGL(k_compact_ids, grid_of(n_rows), dim3(256), 0, st, nullptr, ws.flag.as<uint32_t>(),
                        ws.pos.as<uint32_t>(), n_rows, ws.act.as<uint32_t>());
-    uint64_t n_act = 0;
-    CPM_CHECK(read_u64(ctx, ws, ws.total.p, &n_act));
+    // the depth (largest basket) and the rows active at level 0, read together
+    uint64_t n_act = 0, depth64 = 0;
+    CPM_CHECK(read_two(ctx, ws, ws.total.p, 8, &n_act, ws.scalar.p, 8, &depth64));
+    const uint32_t depth = (uint32_t)(depth64 & 0xffffffffu);
+    t->depth = depth;
 
     const int ib = std::max(1, bits_for(item_count - 1));
     uint64_t node_count = 1;
@@ -419,19 +431,21 @@ int fptree_build(cpm_ctx *ctx, FpWs &ws, const uint64_t *d_rows, const uint32_t 
                            d_weights, t->parents.as<int64_t>(), t->items.as<int64_t>(),
                            t->counts.as<unsigned long long>(), ws.pfr.as<uint32_t>());
         CPM_HIP(hipGetLastError());
-        uint64_t groups = 0;
-        CPM_CHECK(read_u64(ctx, ws, ws.total.p, &groups));
-        single = single && groups == 1;
-        node_count += groups;
-        // rows continuing to the next level
+        // rows continuing to the next level (their count in the second total)
         hipLaunchKernelGGL(k_flag_len, grid_of(n_act), dim3(256), 0, st, vals, n_act,
                            ws.lengths.as<uint32_t>(), level + 1, ws.flag.as<uint32_t>());
         CPM_CHECK(exclusive_scan_u32(ctx, ws.sort, ws.flag.as<uint32_t>(), ws.pos.as<uint32_t>(), n_act,
-                                     ws.total.as<uint64_t>()));
+                                     ws.total.as<uint64_t>() + 1));
         hipLaunchKernelGGL(k_compact_ids, grid_of(n_act), dim3(256), 0, st, vals, ws.flag.as<uint32_t>(),
                            ws.pos.as<uint32_t>(), n_act, ws.act.as<uint32_t>());
         CPM_HIP(hipGetLastError());
-        CPM_CHECK(read_u64(ctx, ws, ws.total.p, &n_act));
+        // the level's new nodes and the next level's rows, read together
+        uint64_t totals[2] = {0, 0};
+        CPM_CHECK(read_two(ctx, ws, ws.total.p, 8, &totals[0], ws.total.as<uint64_t>() + 1, 8, &totals[1]));
+        const uint64_t groups = totals[0];
+        single = single && groups == 1;
+        node_count += groups;
+        n_act = totals[1];
     }
     t->n_nodes = node_count;
     t->single_path = single;
