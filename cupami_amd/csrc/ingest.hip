@@ -40,6 +40,7 @@ constexpr uint32_t kRowsPerTileMax = 64 * 8;  // rows of one tile workgroup (B <
 constexpr uint64_t kDirectMapMax = 1ull << 26;  // ids with a direct id -> column map (256 MB)
 constexpr uint32_t kLdsMapMax = 4096;           // ... copied into LDS by the tile build
 constexpr uint32_t kTileThreads = 512;          // lanes of a tile-build workgroup
+constexpr size_t kKeepBytes = (size_t)64 << 20;  // larger workspace buffers are freed after a build
 
 struct IngestWs {
     DBuf state;   // u64 totals[2] (present ids, store rows), u32 flags[2]
@@ -485,7 +486,12 @@ int build_store_csr(cpm_ctx *ctx, const uint64_t *d_off, uint64_t n, const uint3
     b->has_labels = true;
     *out = b;
     ctx->stats[8] += 1;
-    return t_build.finish();
+    s = t_build.finish();
+    // a universe near 2^32 needs ~1 GB of bitmap and prefix counts: give
+    // workspace buffers above kKeepBytes back (hipFree waits for this build)
+    for (DBuf *d : {&ws.pres, &ws.cnt, &ws.pre, &ws.direct})
+        if (d->cap > kKeepBytes) d->release();
+    return s;
 }
 
 }  // namespace cpm
