@@ -149,6 +149,89 @@ __global__ void k_level_nodes(const uint64_t *__restrict__ keys, const uint32_t 
     }
 }
 
+// One level of the tree build for n <= kSmallLevel active rows, in one
+// workgroup: the steps of k_level_keys, the stable sort by (parent node,
+// item), k_group_flags and its scan, k_level_nodes, k_flag_len and its scan,
+// and k_compact_ids, in the same order, so the tree is the same node for node.
+// totals[0] = nodes added, totals[1] = rows that continue (now in act).
+constexpr uint32_t kSmallLevel = 2048;
+__global__ __launch_bounds__(1024) void k_level_small(uint32_t *__restrict__ act, uint32_t n,
+                                                     uint32_t *__restrict__ pfr, const uint32_t *__restrict__ cols,
+                                                     const uint32_t *__restrict__ off,
+                                                     const uint32_t *__restrict__ lengths, uint32_t level, int ib,
+                                                     uint64_t base, const uint64_t *__restrict__ w,
+                                                     int64_t *__restrict__ parents, int64_t *__restrict__ items,
+                                                     unsigned long long *__restrict__ counts,
+                                                     uint64_t *__restrict__ totals) {
+    __shared__ uint64_t sk[kSmallLevel];
+    __shared__ uint32_t si[kSmallLevel];
+    __shared__ uint32_t rows[kSmallLevel];
+    __shared__ uint32_t lds[16];
+    const uint32_t tid = threadIdx.x;
+    uint32_t P = 1;
+    while (P < n) P <<= 1;
+    for (uint32_t i = tid; i < P; i += blockDim.x) {
+        if (i < n) {
+            const uint32_t r = act[i];
+            rows[i] = r;
+            sk[i] = ((uint64_t)pfr[r] << ib) | cols[off[r] + level];
+        } else {
+            sk[i] = ~0ull;
+        }
+        si[i] = i;
+    }
+    __syncthreads();
+    bitonic_sort_lds(sk, si, P);
+    // lane tid owns sorted positions 2 tid, 2 tid + 1
+    const uint64_t mask = (1ull << ib) - 1;
+    uint32_t f[2], r[2];
+    uint64_t key[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const uint32_t i = 2 * tid + q;
+        f[q] = 0;
+        if (i < n) {
+            key[q] = sk[i];
+            r[q] = rows[si[i]];
+            f[q] = (i == 0 || sk[i] != sk[i - 1]) ? 1u : 0u;
+        }
+    }
+    uint32_t groups;
+    uint32_t excl = block_excl_scan_u32(f[0] + f[1], lds, &groups);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const uint32_t i = 2 * tid + q;
+        if (i < n) {
+            const uint64_t node = base + excl + f[q] - 1;
+            if (f[q]) {
+                parents[node] = (int64_t)(key[q] >> ib);
+                items[node] = (int64_t)(key[q] & mask);
+            }
+            atomicAdd(counts + node, (unsigned long long)w[r[q]]);
+            pfr[r[q]] = (uint32_t)node;
+        }
+        excl += f[q];
+    }
+    // rows continuing to the next level, compacted in sorted order
+    uint32_t g[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const uint32_t i = 2 * tid + q;
+        g[q] = i < n && lengths[r[q]] > level + 1 ? 1u : 0u;
+    }
+    uint32_t next;
+    uint32_t pos = block_excl_scan_u32(g[0] + g[1], lds, &next);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        if (g[q]) act[pos] = r[q];
+        pos += g[q];
+    }
+    if (tid == 0) {
+        totals[0] = groups;
+        totals[1] = next;
+    }
+}
+
 __global__ void k_root(int64_t *parents, int64_t *items, unsigned long long *counts) {
     parents[0] = -1;
     items[0] = -1;
@@ -418,6 +501,19 @@ int fptree_build(cpm_ctx *ctx, FpWs &ws, const uint64_t *d_rows, const uint32_t 
     uint64_t node_count = 1;
     bool single = true;
     for (uint32_t level = 0; level < depth && n_act; ++level) {
+        if (n_act <= kSmallLevel) {
+            hipLaunchKernelGGL(k_level_small, dim3(1), dim3(1024), 0, st, ws.act.as<uint32_t>(), (uint32_t)n_act,
+                               ws.pfr.as<uint32_t>(), scols, ws.off.as<uint32_t>(), ws.lengths.as<uint32_t>(), level,
+                               ib, node_count, d_weights, t->parents.as<int64_t>(), t->items.as<int64_t>(),
+                               t->counts.as<unsigned long long>(), ws.total.as<uint64_t>());
+            CPM_HIP(hipGetLastError());
+            uint64_t totals[2] = {0, 0};
+            CPM_CHECK(read_two(ctx, ws, ws.total.p, 8, &totals[0], ws.total.as<uint64_t>() + 1, 8, &totals[1]));
+            single = single && totals[0] == 1;
+            node_count += totals[0];
+            n_act = totals[1];
+            continue;
+        }
         hipLaunchKernelGGL(k_level_keys, grid_of(n_act), dim3(256), 0, st, ws.act.as<uint32_t>(), n_act,
                            ws.pfr.as<uint32_t>(), scols, ws.off.as<uint32_t>(), level, ib, keys, vals);
         CPM_HIP(hipGetLastError());

@@ -194,24 +194,7 @@ __global__ __launch_bounds__(1024) void k_sort_small(uint64_t *keys, uint32_t *v
         si[i] = i;
     }
     __syncthreads();
-    for (uint32_t k = 2; k <= P; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = tid; i < P; i += blockDim.x) {
-                const uint32_t l = i ^ j;
-                if (l > i) {
-                    const uint64_t a = sk[i], b = sk[l];
-                    const uint32_t ia = si[i], ib = si[l];
-                    const bool gt = a > b || (a == b && ia > ib);
-                    if (gt == ((i & k) == 0)) {
-                        sk[i] = b;
-                        sk[l] = a;
-                        si[i] = ib;
-                        si[l] = ia;
-                    }
-                }
-            }
-            __syncthreads();
-        }
+    bitonic_sort_lds(sk, si, P);
     // every lane reads its sorted pairs before any lane writes (in place)
     constexpr uint32_t kPer = kSmallSort / 1024;
     uint64_t ok[kPer];
