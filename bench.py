@@ -80,7 +80,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="eclat_1m_1k",
                     choices=["eclat_1m_1k", "pfeclat_2m_500", "batch_256", "serve", "ingest_1m_1k",
-                             "output_1m_1k"])
+                             "output_1m_1k", "quickstart"])
     ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
     ap.add_argument("--qps", type=float, default=500.0, help="serve: mean arrival rate")
     ap.add_argument("--hbm-budget-gb", type=float, default=8.0, help="serve: admission budget")
@@ -102,6 +102,12 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(run_serve(args)), flush=True)
+        return
+    if args.workload == "quickstart":
+        line = run_quickstart(args)
+        print(json.dumps(line), flush=True)
+        if line["parity"] is False:
+            sys.exit(1)
         return
     if args.workload == "output_1m_1k":
         line = run_output(args)
@@ -264,6 +270,44 @@ def quickstart():
         out[cls.__name__] = {"patterns": n, "first_ms": round(times[0], 2), "best_ms": round(min(times), 2),
                              "reference_notebook_ms": recorded[cls.__name__]}
     return out
+
+
+def run_quickstart(args):
+    """The reference's recorded case on its own (see quickstart()), with the
+    three miners' dictionaries checked against each other and against the
+    per-basket subset count of the reference's tests
+    (test_transaction_formats.py:15-23), which is also the CPU baseline."""
+    q = quickstart()
+    sys.path.insert(0, str(ROOT / "tests" / "golden"))
+    import generator
+    from cupami_amd import cuApriori, cuEclat, cuFPGrowth
+    rows = generator.transactional(10000, 50, 6, 42)
+    got = []
+    for cls in (cuApriori, cuEclat, cuFPGrowth):
+        m = cls(rows, 0.01)
+        m.mine()
+        got.append(m.getPatterns())
+    parity = got[0] == got[1] == got[2] and len(got[0]) == 1275
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        t = time.perf_counter()
+        expect = oracle.brute_force_rows(rows, oracle.support_threshold(0.01, len(rows)))
+        cpu_s = time.perf_counter() - t
+        parity = parity and got[1] == expect
+        cpu = {"value": round(len(expect) / cpu_s, 1), "unit": "itemsets/s", "cores": 1, "kind": "port",
+               "sample": f"whole workload, one run ({cpu_s:.2f}s): the per-basket subset count of "
+                         "test_transaction_formats.py:15-23 (pure Python)"}
+    best = q["cuEclat"]["best_ms"]
+    return {
+        "metric": "getMiningTime() on the reference's quickstart case (itemsets/s, cuEclat)",
+        "value": round(1275 / (best / 1e3), 1), "unit": "itemsets/s", "n_gpus": 1, "steps": 5, "warmup": 0,
+        "ms_per_step": best, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "the reference's generator restated (seed 42)",
+        "config": {"workload": "quickstart", "transactions": 10000, "items": 50, "avg_length": 6,
+                   "minsup": 0.01, "parallelism": "single GPU"},
+        "roofline": None, "cpu_baseline": cpu, "quickstart": q, "parity": parity,
+    }
 
 
 def extras(args):
