@@ -506,6 +506,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
     __shared__ uint32_t lds[8];
     __shared__ uint32_t lmask[kMaskWords];
     __shared__ uint32_t wf[4][kSmallTask];  // per wave: the task's kept items
+    __shared__ uint32_t spre[kMaxDepth];    // a large task's prefix
     if (p.ctl->overflow) return;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t n_tasks = (uint32_t)p.L.sz->n_tasks;
@@ -530,7 +531,14 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
         const DbDev &D = p.dbs[dbi];
         const uint32_t a = p.L.ext_off[t];
         const uint64_t pb = pat0 + p.off_pat[t];
-        const uint32_t *src = p.L.prefix + (uint64_t)t * kMaxDepth;
+        // the task's prefix, item i in lane i (d <= 64): one load for the
+        // wave instead of a dependent load per item in every lane
+        const uint32_t sv = lane < d ? p.L.prefix[(uint64_t)t * kMaxDepth + lane] : 0u;
+        unsigned long long psum = 0;  // sum of the prefix items' filter terms
+        if (stage) {
+            psum = lane < d ? item_term(sv) : 0ull;
+            for (int off = 32; off > 0; off >>= 1) psum += __shfl_xor(psum, off, 64);
+        }
         uint32_t f = 0;
         if (lane < nk) {
             const uint32_t e = p.kept[a + lane];
@@ -540,11 +548,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             p.pat_sup[pb + lane] = p.L.sup[e];
             p.pat_db[pb + lane] = dbi;
             if (p.pat_per) p.pat_per[pb + lane] = p.L.per[e];
-            if (stage) {
-                uint64_t sum = item_term(f);
-                for (uint32_t i = 0; i < d; ++i) sum += item_term(src[i]);
-                filter_insert(p.filter, fmask, set_hash(sum));
-            }
+            if (stage) filter_insert(p.filter, fmask, set_hash(psum + item_term(f)));
         }
         if (!(children && nk >= 2)) continue;
         wave_sync();  // the previous task's reads of wf are done
@@ -577,11 +581,15 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
         }
         uint32_t tot;
         const uint32_t off = wave_scan(cj, &tot);
+        // child c's prefix row: this task's prefix and item f of lane c, one
+        // coalesced store per child (d + 1 <= 64 when children are emitted)
+        const uint32_t cb = p.off_child[t];
+        for (uint32_t c = 0; c + 1 < nk; ++c) {
+            const uint32_t fc = __shfl(f, c, 64);
+            if (lane <= d) p.N.prefix[(uint64_t)(cb + c) * kMaxDepth + lane] = lane < d ? sv : fc;
+        }
         if (live) {
-            const uint32_t ct = p.off_child[t] + lane;
-            uint32_t *dst = p.N.prefix + (uint64_t)ct * kMaxDepth;
-            for (uint32_t i = 0; i < d; ++i) dst[i] = src[i];
-            dst[d] = f;
+            const uint32_t ct = cb + lane;
             p.N.par[ct] = t;
             p.N.pid[ct] = (uint32_t)(pb + lane);
             p.N.db[ct] = dbi;
@@ -607,7 +615,14 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
         const uint32_t a = p.L.ext_off[t];
         const uint64_t pb = pat0 + p.off_pat[t];
         const uint32_t parent = p.L.pid[t];
-        const uint32_t *src = p.L.prefix + (uint64_t)t * kMaxDepth;
+        __syncthreads();  // the previous task's reads of spre are done
+        if (tid < d) spre[tid] = p.L.prefix[(uint64_t)t * kMaxDepth + tid];
+        __syncthreads();
+        unsigned long long psum = 0;  // sum of the prefix items' filter terms
+        if (stage) {
+            psum = lane < d ? item_term(spre[lane]) : 0ull;
+            for (int off = 32; off > 0; off >>= 1) psum += __shfl_xor(psum, off, 64);
+        }
         for (uint32_t j = tid; j < nk; j += 256) {
             const uint32_t e = p.kept[a + j];
             const uint32_t f = p.L.ext[e];
@@ -616,11 +631,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             p.pat_sup[pb + j] = p.L.sup[e];
             p.pat_db[pb + j] = dbi;
             if (p.pat_per) p.pat_per[pb + j] = p.L.per[e];
-            if (stage) {
-                uint64_t sum = item_term(f);
-                for (uint32_t i = 0; i < d; ++i) sum += item_term(src[i]);
-                filter_insert(p.filter, fmask, set_hash(sum));
-            }
+            if (stage) filter_insert(p.filter, fmask, set_hash(psum + item_term(f)));
         }
         if (!(children && nk >= 2)) continue;
         if (p.pairbits) {
@@ -654,7 +665,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             if (live) {
                 const uint32_t ct = cb + j;
                 uint32_t *dst = p.N.prefix + (uint64_t)ct * kMaxDepth;
-                for (uint32_t i = 0; i < d; ++i) dst[i] = src[i];
+                for (uint32_t i = 0; i < d; ++i) dst[i] = spre[i];
                 dst[d] = fj;
                 p.N.par[ct] = t;
                 p.N.pid[ct] = (uint32_t)(pb + j);
