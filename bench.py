@@ -2,7 +2,7 @@
 
 A step is one pass of the support-counting path over the resident input:
 the vertical store is built from the COO pairs already in HBM, then the
-device search (level-1 counts, level-2 INT8 Gram on the matrix cores,
+device search (level-1 counts, level-2 FP4 Gram on the matrix cores,
 levels >= 3 on the support kernel with on-device candidate generation) runs
 to completion and the pattern forest is copied back.  ``value`` is itemsets
 per second of that step.  The JSON line also carries the support kernel's

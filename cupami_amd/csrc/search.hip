@@ -1195,6 +1195,7 @@ struct Search {
     std::vector<DbDev> dbs;          // host copy
     std::vector<uint32_t> nf;        // frequent items per database
     uint32_t parts_tiles = 0;
+    Caps need{};                     // level 2's exact sizes when they overflowed
 
     Search(cpm_ctx *c, SearchWs &ws) : ctx(c), w(ws) {}
 
@@ -1599,13 +1600,14 @@ struct Search {
         qput(w.dbs.p, dbs.data(), n_db * sizeof(DbDev));
         if (t2 == 0) return flush();
         ctl_current = false;
-        if (t2 > w.caps.tasks || e2 > w.caps.ext || e2 >= 0xffffffffull) {
-            *overflow = (t2 > w.caps.tasks ? kOvTasks : 0) | (e2 > w.caps.ext ? kOvEntries : 0);
-            if (e2 >= 0xffffffffull) return fail(CPM_ERR_NOMEM, "too many frequent item pairs");
-            return CPM_OK;
-        }
-        if (periodic && e2 * parts_tiles > w.caps.parts) {
-            *overflow = kOvParts;
+        if (e2 >= 0xffffffffull) return fail(CPM_ERR_NOMEM, "too many frequent item pairs");
+        if (t2 > w.caps.tasks || e2 > w.caps.ext || (periodic && e2 * parts_tiles > w.caps.parts)) {
+            // level 2 does not fit: the retry starts with room for it
+            *overflow = (t2 > w.caps.tasks ? kOvTasks : 0) | (e2 > w.caps.ext ? kOvEntries : 0) |
+                        (periodic && e2 * parts_tiles > w.caps.parts ? kOvParts : 0);
+            need.tasks = t2;
+            need.ext = e2;
+            need.parts = periodic ? e2 * parts_tiles : 0;
             return CPM_OK;
         }
         LevelSet &S1 = w.set[1];
@@ -1729,27 +1731,25 @@ struct Search {
     }
 
     int run(cpm_patterns **out) {
-        // first capacities: generous for the first level set, grown on overflow
-        uint64_t items = 0, e2 = 0;
-        for (auto &d : in) {
-            items += d.bits->n_items;
-            e2 += (uint64_t)d.bits->n_items * (d.bits->n_items ? d.bits->n_items - 1 : 0) / 2;
-        }
+        // First capacities: level 1 (one entry per item of every store).
+        // Level 2's sizes are known exactly once level 1 has found the
+        // frequent items (attempt() records them in `need`), so an input
+        // with many distinct but few frequent items never sizes anything by
+        // the square of its item count.  Later levels grow by doubling.
+        uint64_t items = 0;
+        for (auto &d : in) items += d.bits->n_items;
         parts_tiles = 0;
         if (periodic)
             for (auto &d : in) parts_tiles = std::max(parts_tiles, tiles_per_row(d.bits->stride));
         Caps want;
         want.tasks = std::max<uint64_t>(std::max<uint64_t>(items, in.size()), 1 << 16);
-        want.ext = std::max<uint64_t>(std::max<uint64_t>(e2, items), 1 << 18);
+        want.ext = std::max<uint64_t>(items, 1 << 18);
         want.patterns = std::max<uint64_t>(items * 4, 1 << 18);
         want.parts = periodic ? want.ext * std::max<uint32_t>(parts_tiles, 1) : 0;
-        for (int i = 0; i < 4; ++i) {
-            uint64_t *a = &w.caps.tasks + i;
-            const uint64_t *b = &want.tasks + i;
-            if (*a < *b) *a = *b;
-        }
+        grow_caps(want);
         for (int tries = 0;; ++tries) {
             uint32_t ov = 0;
+            need = Caps{};
             const int rc = attempt(&ov);
             if (rc != CPM_OK || ov) w.filter_dirty = true;  // stopped between insert and clear
             CPM_CHECK(rc);
@@ -1761,8 +1761,18 @@ struct Search {
             if (ov & kOvEntries) w.caps.ext *= 2;
             if (ov & kOvPatterns) w.caps.patterns *= 2;
             if (ov & kOvParts) w.caps.parts = std::max<uint64_t>(w.caps.parts * 2, 1 << 20);
+            grow_caps(need);
         }
         return download(out);
+    }
+
+    // capacities raised to at least `c` (field by field)
+    void grow_caps(const Caps &c) {
+        for (int i = 0; i < 4; ++i) {
+            uint64_t *a = &w.caps.tasks + i;
+            const uint64_t *b = &c.tasks + i;
+            if (*a < *b) *a = *b;
+        }
     }
 
     int download(cpm_patterns **out) {

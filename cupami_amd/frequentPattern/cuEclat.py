@@ -5,7 +5,9 @@ constructor, tunables (``_PAIR_BATCH_SIZE``, ``_INTERSECTION_BATCH_BYTES``),
 hooks (``_candidatePairs``, ``_extensionBatch``, ``_searchClass``) and CLI.
 
 ``mine()`` runs the whole search on the device (``cpm_eclat_mine``): level-2
-pair supports come from the INT8 Gram on the matrix cores, and the prefix
+pair supports come from the Gram D^T D on the matrix cores (FP4/E2M1 0/1
+operands with FP32 accumulation, cut into K slices below 2^24 transactions
+so every partial sum is an exact integer), and the prefix
 classes -- child class of K_j = the later kept siblings, exactly the
 reference's ``_searchClass`` frames -- are generated, pair-pruned and counted
 by device kernels.  A subclass that overrides ``_candidatePairs`` or

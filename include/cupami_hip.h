@@ -158,7 +158,7 @@ int cpm_eclat_mine(cpm_ctx *ctx, const cpm_bits *bits, uint64_t minsup, int use_
                    cpm_patterns **out);
 
 /* Many databases in one search: the levels of every database share each
- * launch (batched, persistent support kernel; one INT8 Gram launch for all
+ * launch (batched, persistent support kernel; one FP4 Gram launch for all
  * level-2 matrices).  bits[i] is mined at minsup[i]; the patterns of all
  * databases come back in one result, cpm_patterns_databases tells them
  * apart.  Results equal n_db separate cpm_eclat_mine calls. */
