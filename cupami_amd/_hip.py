@@ -60,6 +60,7 @@ SIGNATURES = {
     "cpm_bits_download": (ctypes.c_int, [_vp, _vp, _u64p]),
     "cpm_count_supports": (ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint64, ctypes.c_uint32,
                                           _u64p]),
+    "cpm_count_supports_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint64, ctypes.c_uint32, _vp]),
     "cpm_intersect_extensions": (ctypes.c_int, [_vp, _u64p, _u64p, ctypes.c_uint32,
                                                 ctypes.c_uint64, ctypes.c_uint64, _u64p, _u64p,
                                                 _u32p, _u32p]),
@@ -275,6 +276,12 @@ class Context:
             check(load().cpm_count_supports(self.handle, bits.handle, _p(cand, _u32p),
                                             cand.shape[0], cand.shape[1], _p(out, _u64p)))
         return out
+
+    def count_supports_noalloc(self, bits, d_cand, n_cand, width, d_supports):
+        """``count_supports`` on device arrays (addresses as ints: candidates
+        u32[n_cand x width], supports u64[n_cand]) without device allocation."""
+        check(load().cpm_count_supports_noalloc(self.handle, bits.handle, _vp(int(d_cand)), int(n_cand),
+                                                int(width), _vp(int(d_supports))))
 
     def intersect_extensions(self, prefix, extensions, keep_min=0, want_rows=True):
         prefix = np.ascontiguousarray(prefix, dtype=np.uint64)

@@ -255,6 +255,7 @@ int cpm_ctx_create(int device, cpm_ctx **out) {
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(CPM_ERR_NODEVICE, "device %d is %s; this library is built for gfx950", device,
                     prop.gcnArchName);
+    CPM_CHECK(noalloc_setup(device));
     cpm_ctx *ctx = new cpm_ctx();
     ctx->device = device;
     ctx->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;

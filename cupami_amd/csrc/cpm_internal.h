@@ -208,6 +208,9 @@ namespace cpm {
 
 // Make the context's device current for the calling thread.
 int bind(cpm_ctx *ctx);
+// Records the device address of the no-allocation entry point's scratch
+// array for `device` (the current device); once per device (search.hip).
+int noalloc_setup(int device);
 // A store of n_items rows for n_trans transactions, zeroed when `clear`.
 int bits_alloc(cpm_ctx *ctx, uint64_t n_trans, uint32_t n_items, cpm_bits **out, bool clear = true);
 

@@ -32,6 +32,7 @@
 
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <system_error>
 #include <thread>
 
@@ -1874,6 +1875,143 @@ int check_bits(const cpm_bits *b, cpm_ctx *ctx) {
 
 }  // namespace cpm
 
+namespace cpm {
+
+// ---- complete-intersection counting without device allocation ---------------
+//
+// cpm_count_supports_noalloc builds the support kernel's task tables from
+// device-resident candidates into `scratch`, a file-scope device array of
+// fixed size, so the call allocates nothing.  Layout (byte offsets):
+//   [kNaDb]     DbDev of the store
+//   [kNaSizes]  LevelSizes of the one level
+//   [kNaFlag]   u32 error flag (a column >= the store's item count)
+//   [kNaPrefix] u32 prefix items, kMaxDepth per task
+//   [kNaOff]    u32 ext_off[tasks + 1]
+//   [kNaExt]    u32 ext[candidates]
+//   [kNaDbi]    u32 db[tasks] (all 0)
+//   [kNaSup]    u32 sup[candidates] (cleared by the table kernel)
+constexpr uint64_t kNoAllocCands = 65536;   // candidates per call
+constexpr size_t kNaDb = 0, kNaSizes = 256, kNaFlag = 512, kNaPrefix = 768;
+constexpr size_t kNaOff = kNaPrefix + kNoAllocCands * kMaxDepth * 4;
+constexpr size_t kNaExt = kNaOff + (kNoAllocCands + 64) * 4;
+constexpr size_t kNaDbi = kNaExt + kNoAllocCands * 4;
+constexpr size_t kNaSup = kNaDbi + kNoAllocCands * 4;
+constexpr size_t kNoAllocBytes = kNaSup + kNoAllocCands * 4;
+static_assert(sizeof(DbDev) <= 256 && sizeof(LevelSizes) <= 256, "scratch header slots");
+static_assert(kNaOff % 256 == 0 && kNaExt % 256 == 0 && kNaDbi % 256 == 0 && kNaSup % 256 == 0,
+              "scratch sections stay 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on each device, from hipGetSymbolAddress at
+// context creation.  Host code must use these addresses: the array's own
+// name is a host-side stand-in, not device memory.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(CPM_ERR_INVALID, "device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return CPM_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(CPM_ERR_RUNTIME, "hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                    hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return CPM_OK;
+}
+
+namespace {
+
+// One 1024-lane workgroup: runs of candidates with equal first width - 1
+// columns become one task (as run_candidates groups them on the host); the
+// tables, the level's sizes and the store's DbDev go to the scratch array,
+// the support accumulators are cleared.  A column beyond the store raises
+// the flag, and the support launch that follows then does nothing.
+__global__ __launch_bounds__(1024) void k_na_tasks(const uint32_t *__restrict__ cand, uint32_t n, uint32_t width,
+                                                    uint32_t n_items, DbDev D, uint32_t group,
+                                                    unsigned long long n_work, unsigned char *ws) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t bad;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t d = width - 1;
+    uint32_t *prefix = reinterpret_cast<uint32_t *>(ws + kNaPrefix);
+    uint32_t *off = reinterpret_cast<uint32_t *>(ws + kNaOff);
+    uint32_t *ext = reinterpret_cast<uint32_t *>(ws + kNaExt);
+    uint32_t *dbi = reinterpret_cast<uint32_t *>(ws + kNaDbi);
+    uint32_t *sup = reinterpret_cast<uint32_t *>(ws + kNaSup);
+    if (tid == 0) bad = 0;
+    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t lo = min(n, tid * per), hi = min(n, lo + per);
+    auto fresh = [&](uint32_t c) {
+        if (c == 0) return true;
+        const uint32_t *a = cand + (uint64_t)c * width, *b = a - width;
+        for (uint32_t i = 0; i < d; ++i)
+            if (a[i] != b[i]) return true;
+        return false;
+    };
+    uint32_t mine = 0, badl = 0;
+    for (uint32_t c = lo; c < hi; ++c) {
+        mine += fresh(c) ? 1u : 0u;
+        for (uint32_t i = 0; i < width; ++i) badl |= cand[(uint64_t)c * width + i] >= n_items ? 1u : 0u;
+    }
+    // exclusive scan of the per-lane run starts over the workgroup
+    uint32_t x = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (badl) atomicOr(&bad, 1u);
+    uint32_t base = 0, total = 0;
+    for (uint32_t w = 0; w < 16; ++w) {
+        if (w < wave) base += wsum[w];
+        total += wsum[w];
+    }
+    base += x - mine;
+    // task of candidate lo: the last run started at or before it
+    uint32_t t = base - 1;
+    for (uint32_t c = lo; c < hi; ++c) {
+        const uint32_t *it = cand + (uint64_t)c * width;
+        if (fresh(c)) {
+            ++t;
+            off[t] = c;
+            dbi[t] = 0;
+            for (uint32_t i = 0; i < d; ++i) prefix[(uint64_t)t * kMaxDepth + i] = it[i];
+        }
+        ext[c] = it[d];
+        sup[c] = 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        off[total] = n;
+        *reinterpret_cast<DbDev *>(ws + kNaDb) = D;
+        LevelSizes ls;
+        memset(&ls, 0, sizeof(ls));
+        ls.n_tasks = total;
+        ls.d = d;
+        ls.n_ext = n;
+        ls.n_cand = n;
+        ls.group = group;
+        ls.n_work = n_work;
+        *reinterpret_cast<LevelSizes *>(ws + kNaSizes) = ls;
+        *reinterpret_cast<uint32_t *>(ws + kNaFlag) = bad;
+    }
+}
+
+// u32 supports from the scratch accumulators into the caller's u64 output
+__global__ void k_na_widen(const uint32_t *__restrict__ sup, uint32_t n, uint64_t *__restrict__ out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = sup[i];
+}
+
+}  // namespace
+
+}  // namespace cpm
+
 using namespace cpm;
 
 extern "C" {
@@ -2043,6 +2181,64 @@ static int run_candidates(cpm_ctx *ctx, const cpm_bits *b, const uint32_t *cand,
     CPM_CHECK(copy_to_host(ctx, s32.data(), dsup.p, n_cand * 4));
     if (periodic) CPM_CHECK(copy_to_host(ctx, per_out, dper.p, n_cand * 8));
     for (uint64_t c = 0; c < n_cand; ++c) sup_out[c] = s32[c];
+    return CPM_OK;
+}
+
+int cpm_count_supports_noalloc(cpm_ctx *ctx, const cpm_bits *b, const uint32_t *d_cand, uint64_t n_cand,
+                               uint32_t width, uint64_t *d_supports) {
+    clear_error();
+    CPM_CHECK(bind(ctx));
+    if (!b) return fail(CPM_ERR_INVALID, "null store");
+    if (b->ctx != ctx) return fail(CPM_ERR_INVALID, "store belongs to another context");
+    if (n_cand == 0) return CPM_OK;
+    if (!d_cand || !d_supports) return fail(CPM_ERR_INVALID, "null candidates/supports");
+    if (n_cand > kNoAllocCands)
+        return fail(CPM_ERR_INVALID, "%llu candidates in one call: at most %llu without device allocation",
+                    (unsigned long long)n_cand, (unsigned long long)kNoAllocCands);
+    if (width == 0) return fail(CPM_ERR_INVALID, "candidate width must be >= 1");
+    if (width > kMaxDepth + 1) return fail(CPM_ERR_INVALID, "candidates wider than %u items", kMaxDepth + 1);
+    if (b->n_transactions >= 0xffffffffull)
+        return fail(CPM_ERR_INVALID, "supports are counted in 32 bits: too many transactions");
+    unsigned char *ws = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < kMaxDevices)
+            ws = static_cast<unsigned char *>(g_scratch_ptr[ctx->device]);
+    }
+    if (!ws) return fail(CPM_ERR_RUNTIME, "no scratch address for device %d", ctx->device);
+    const uint32_t tw = kTileWords;
+    DbDev D;
+    memset(&D, 0, sizeof(D));
+    D.store = b->rows();
+    D.stride = b->stride;
+    D.words = b->words;
+    D.n = b->n_transactions;
+    D.tiles = tiles_per_row(b->stride, tw);
+    D.tiles_eff = tiles_eff_of(D.tiles);
+    const uint32_t group = level_group(n_cand, 32, D.tiles_eff, 4ull * support_grid(ctx));
+    const unsigned long long n_work = level_work(n_cand, group, D.tiles_eff);
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(k_na_tasks, dim3(1), dim3(1024), 0, st, d_cand, (uint32_t)n_cand, width, b->n_items, D,
+                       group, n_work, ws);
+    CPM_HIP(hipGetLastError());
+    SupportArgs a;
+    a.L.prefix = reinterpret_cast<uint32_t *>(ws + kNaPrefix);
+    a.L.db = reinterpret_cast<uint32_t *>(ws + kNaDbi);
+    a.L.ext_off = reinterpret_cast<uint32_t *>(ws + kNaOff);
+    a.L.ext = reinterpret_cast<uint32_t *>(ws + kNaExt);
+    a.L.sup = reinterpret_cast<uint32_t *>(ws + kNaSup);
+    a.L.sz = reinterpret_cast<LevelSizes *>(ws + kNaSizes);
+    a.dbs = reinterpret_cast<const DbDev *>(ws + kNaDb);
+    a.te = D.tiles_eff;
+    a.stop = reinterpret_cast<const uint32_t *>(ws + kNaFlag);
+    CPM_CHECK(launch_support_count(ctx, a, tw, support_grid(ctx)));
+    hipLaunchKernelGGL(k_na_widen, dim3((unsigned)std::min<uint64_t>((n_cand + 255) / 256, 256)), dim3(256), 0, st,
+                       a.L.sup, (uint32_t)n_cand, d_supports);
+    CPM_HIP(hipGetLastError());
+    uint32_t bad = 0;
+    CPM_HIP(hipMemcpyAsync(&bad, ws + kNaFlag, 4, hipMemcpyDeviceToHost, st));
+    CPM_HIP(hipStreamSynchronize(st));
+    if (bad) return fail(CPM_ERR_INVALID, "a candidate column is >= the store's %u items", b->n_items);
     return CPM_OK;
 }
 

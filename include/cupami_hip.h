@@ -117,6 +117,21 @@ int cpm_bits_download(cpm_ctx *ctx, const cpm_bits *bits, uint64_t *host);
 int cpm_count_supports(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t *cand,
                        uint64_t n_cand, uint32_t width, uint64_t *supports);
 
+/* No counterpart in the reference: cpm_count_supports for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Candidates (`d_cand`, n_cand x
+ * width u32 columns, row-major) and supports (`d_supports`, n_cand u64) are
+ * DEVICE pointers; the task tables live in a fixed device array of the
+ * library (about 17 MiB per device, held until the process ends), so the
+ * call allocates nothing.  At most 65,536 candidates per call (the
+ * reference's batch, cuEclat.py:185) and width <= 65; larger calls fail with
+ * CPM_ERR_INVALID.  Synchronous, like cpm_count_supports: a column >= the
+ * store's item count is reported as CPM_ERR_INVALID (supports then
+ * unspecified).  Every call on one device shares that array: calls that
+ * overlap on one device (several contexts or threads) race. */
+int cpm_count_supports_noalloc(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t *d_cand,
+                               uint64_t n_cand, uint32_t width, uint64_t *d_supports);
+
 /* Replaces _bitsets.intersect_extensions (_bitsets.py:116-125) with the keep
  * step cuEclat applies to its result (cuEclat.py:234-242) fused in:
  * child_j = prefix & ext_j for k extension rows of `words` uint64 each
