@@ -79,7 +79,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="eclat_1m_1k",
-                    choices=["eclat_1m_1k", "pfeclat_2m_500", "batch_256", "serve", "ingest_1m_1k",
+                    choices=["eclat_1m_1k", "eclat_4m_1k", "pfeclat_2m_500", "batch_256", "serve", "ingest_1m_1k", "roofs",
                              "output_1m_1k", "quickstart"])
     ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
     ap.add_argument("--qps", type=float, default=500.0, help="serve: mean arrival rate")
@@ -115,6 +115,9 @@ def main():
         if line["parity"] is False:
             sys.exit(1)
         return
+    if args.workload == "roofs":
+        print(json.dumps(run_roofs(args)), flush=True)
+        return
     if args.workload == "ingest_1m_1k":
         line = run_ingest(args)
         print(json.dumps(line), flush=True)
@@ -128,9 +131,11 @@ def main():
 
     periodic = args.workload == "pfeclat_2m_500"
     t = time.perf_counter()
-    db = datagen.temporal_2m_500() if periodic else datagen.eclat_1m_1k()
+    big = args.workload == "eclat_4m_1k"
+    db = (datagen.temporal_2m_500() if periodic else
+          datagen.transactional(4_000_000, 1000, 10.0, 1.1, 202) if big else datagen.eclat_1m_1k())
     labels, cols = datagen.label_columns(db.items)
-    minsup = PF_MINSUP if periodic else MINSUP
+    minsup = PF_MINSUP if periodic else 4 * MINSUP if big else MINSUP
     end = int(db.timestamps[-1]) if periodic else 0
     n, items, nnz = db.n_transactions, len(labels), db.rows.size
     log(f"bench: generated {n} x {items} ({nnz} pairs) in {time.perf_counter() - t:.1f}s")
@@ -223,7 +228,7 @@ def main():
                   "zipf_s": 1.05, "minsup": minsup, "maxper": PF_MAXPER, "patterns": n_pat,
                   "parallelism": "single GPU"}
     else:
-        config = {"workload": "eclat_1m_1k", "transactions": n, "items": items,
+        config = {"workload": args.workload, "transactions": n, "items": items,
                   "lambda": 10, "zipf_s": 1.1, "minsup": minsup, "patterns": n_pat,
                   "parallelism": "single GPU"}
     line = {
@@ -235,7 +240,7 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "gram": gram,
         "levels_per_step": stats["levels"], "parity": parity,
     }
-    if not periodic and not args.no_extras:
+    if not periodic and not big and not args.no_extras:
         line["batched"], line["serve"] = extras(args)
         line["quickstart"] = quickstart()
         if any(v["patterns"] != 1275 for k, v in line["quickstart"].items() if k != "workload"):
@@ -499,6 +504,62 @@ def run_batch(args):
         "parity": parity,
     }
     return line
+
+
+# (name, buffer bytes, passes, XCD-sliced): AND + popcount of the buffer's two
+# halves; sliced buffers give each XCD its own eighth, as the support kernel's
+# tile == XCD mapping does
+ROOF_CASES = (
+    ("l2_2mib_per_xcd", 16 << 20, 20000, True),
+    ("l2_4mib_per_xcd", 32 << 20, 10000, True),
+    ("mall_16mib_per_xcd", 128 << 20, 1000, True),
+    ("mall_25mib_per_xcd", 200 << 20, 600, True),
+    ("hbm_4gib", 4 << 30, 8, True),
+    ("hbm_8gib_flat", 8 << 30, 4, False),
+)
+
+
+def run_roofs(args):
+    """This device's measured roofs: AND + popcount streaming rates with the
+    data resident in L2, in the Infinity Cache and in HBM, and the FP4 MFMA
+    loop rate (best of three timed launches after one warm launch each)."""
+    import torch
+    from cupami_amd import _hip
+    ctx = _hip.context(0)
+    dev = torch.device("cuda:0")
+    sink = torch.zeros(1, dtype=torch.int64, device=dev)
+    out = {}
+    for name, nbytes, reps, sliced in ROOF_CASES:
+        buf = torch.randint(0, 2**62, (nbytes // 8,), dtype=torch.int64, device=dev)
+        torch.cuda.synchronize()
+        ctx.roof_stream(buf.data_ptr(), nbytes, max(1, reps // 10), sliced, sink.data_ptr())
+        best = None
+        for _ in range(3):
+            ms, nb = ctx.roof_stream(buf.data_ptr(), nbytes, reps, sliced, sink.data_ptr())
+            gbs = nb / (ms / 1e3) / 1e9
+            best = gbs if best is None else max(best, gbs)
+        out[name] = {"buffer_bytes": nbytes, "passes": reps, "xcd_sliced": sliced, "gb_s": round(best, 1)}
+        log(f"bench: roof {name}: {best:.1f} GB/s")
+        del buf
+    fs = torch.zeros(ctx_grid(ctx), dtype=torch.float32, device=dev)
+    for shape, name in ((16, "mfma_fp4_16x16x128"), (32, "mfma_fp4_32x32x64")):
+        ctx.roof_mfma(2000, fs.data_ptr(), shape)
+        best = 0.0
+        for _ in range(3):
+            ms, ops = ctx.roof_mfma(20000, fs.data_ptr(), shape)
+            best = max(best, ops / (ms / 1e3) / 1e12)
+        out[name] = {"tops": round(best, 1), "published_dense_tops": 10100.0}
+        log(f"bench: roof {name}: {best:.1f} TOP/s")
+    props = torch.cuda.get_device_properties(0)
+    return {"metric": "measured roofs (AND+popcount GB/s by residency; FP4 MFMA TOP/s)",
+            "value": out["mall_16mib_per_xcd"]["gb_s"], "unit": "GB/s", "n_gpus": 1, "steps": 3, "warmup": 1,
+            "higher_is_better": True, "device": props.name, "cu_count": props.multi_processor_count,
+            "roofs": out, "hbm_published_gb_s": HBM_PEAK_GBS}
+
+
+def ctx_grid(ctx):
+    import torch
+    return torch.cuda.get_device_properties(0).multi_processor_count * 8
 
 
 def run_ingest(args):

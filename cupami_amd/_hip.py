@@ -95,6 +95,10 @@ SIGNATURES = {
     "cpm_patterns_count": (ctypes.c_int, [_vp, _u64p]),
     "cpm_patterns_download": (ctypes.c_int, [_vp, _u32p, _u32p, _u64p, _i64p]),
     "cpm_patterns_destroy": (ctypes.c_int, [_vp]),
+    "cpm_roof_stream": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, _vp,
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "cpm_roof_mfma": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint32, _vp, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double)]),
     "cpm_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "cpm_stats_reset": (ctypes.c_int, [_vp]),
     "cpm_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -188,6 +192,20 @@ class Context:
 
     def set_timing(self, on):
         check(load().cpm_set_timing(self.handle, 1 if on else 0))
+
+    # ---- measured roofs --------------------------------------------------------
+    def roof_stream(self, d_buf, nbytes, reps, xcd_sliced, d_sink):
+        """AND + popcount streaming microbenchmark; returns (ms, bytes read)."""
+        ms, nb = ctypes.c_double(), ctypes.c_double()
+        check(load().cpm_roof_stream(self.handle, _vp(int(d_buf)), int(nbytes), int(reps), int(bool(xcd_sliced)),
+                                     _vp(int(d_sink)), ctypes.byref(ms), ctypes.byref(nb)))
+        return ms.value, nb.value
+
+    def roof_mfma(self, iters, d_sink, shape=16):
+        """FP4 MFMA loop microbenchmark; returns (ms, ops)."""
+        ms, ops = ctypes.c_double(), ctypes.c_double()
+        check(load().cpm_roof_mfma(self.handle, int(shape), int(iters), _vp(int(d_sink)), ctypes.byref(ms), ctypes.byref(ops)))
+        return ms.value, ops.value
 
     # ---- store -------------------------------------------------------------
     def build_bits(self, rows, cols, n_transactions, n_items):

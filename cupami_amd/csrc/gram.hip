@@ -7,23 +7,29 @@
 // stages kKW-word tiles of its 2 x 256 rows in LDS -- eight lanes per row, so
 // one load instruction reads eight whole 128-byte lines -- and each of its
 // eight waves accumulates a 128 x 64 piece with
-// v_mfma_scale_f32_16x16x128_f8f6f4 in its FP4 (E2M1) form, 8 x 4 tiles of
-// 16 x 16.  0 and 1 are exact E2M1 values (1.0 is the nibble 0b0010) and
-// every product is 0 or 1, so the FP32 accumulators hold exact integers
-// while a slice spans fewer than 2^24 transactions; the planner cuts K so
-// that it does.  The FP4 form runs at twice the INT8 rate and takes 128 K
-// elements (two words) per instruction.
+// v_mfma_scale_f32_32x32x64_f8f6f4 in its FP4 (E2M1) form, 4 x 2 tiles of
+// 32 x 32.  (That form runs at twice the rate of the 16 x 16 x 128 one on
+// MI355X: ~9.8 against ~5.2 POP/s in a loop of independent MFMAs.)  0 and 1
+// are exact E2M1 values (1.0 is the nibble 0b0010) and every product is 0 or
+// 1, so the FP32 accumulators hold exact integers while a slice spans fewer
+// than 2^24 transactions; the planner cuts K so that it does.
 //
-// The bits become nibbles in registers: lane group g (lane / 16) takes 32
-// bits of each word pair (word 2s + g / 2, half g % 2) and spreads them with
-// shift-and-mask; A (row items) and B (column items) use the same lane -> bit
-// map, so every bit is paired with itself exactly once whatever order the
-// hardware gives the K slots.  128 accumulator registers per wave leave room
-// for two waves per SIMD, so one wave's barrier and LDS waits hide under the
-// other's MFMAs.  The next K tile is loaded into registers while the current
-// one is multiplied out of LDS.  Slices are summed with integer atomics.  One
-// launch serves any number of databases: the job table lists each
-// workgroup's (database, block row, block column, K range).
+// An MFMA step takes 32 K elements per lane: lanes 0-31 feed rows (columns)
+// 0-31 with one 32-bit piece of the tile, lanes 32-63 the same rows with
+// another.  Over four steps a lane uses one 16-byte LDS read per 32-row tile
+// (u32 8j + 4h .. +3 of its row's tile, h = lane / 32), and spreads each u32
+// to nibbles with shift-and-mask.  A (row items) and B (column items) use the
+// same lane -> bit map, so every bit is paired with itself exactly once
+// whatever order the hardware gives the K slots.  128 accumulator registers
+// per wave leave room for two waves per SIMD, so one wave's barrier and LDS
+// waits hide under the other's MFMAs.  Tiles are double-buffered in LDS: the
+// next K tile is loaded into registers and stashed into the other buffer
+// while the current one is multiplied, one barrier per tile.  On diagonal
+// blocks the six pieces with work are spread 1.5 per SIMD (two waves share
+// two of them, each taking every other K group).  Slices are summed with
+// integer atomics.  One launch serves any number of databases:
+// the job table lists each workgroup's (database, block row, block column,
+// K range).
 #include "cpm_internal.h"
 
 #include <algorithm>
@@ -33,16 +39,18 @@ namespace cpm {
 namespace {
 
 typedef int v8i __attribute__((ext_vector_type(8)));
-typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
 
-constexpr int kTX = 8, kTY = 4;             // 16 x 16 tiles per wave: rows, columns
+constexpr int kTX = 4, kTY = 2;             // 32 x 32 tiles per wave: rows, columns
 constexpr int kBT = 256;                     // block tile (items)
-constexpr int kWR = kBT / (16 * kTX);        // waves along rows
-constexpr int kWC = kBT / (16 * kTY);        // waves along columns
+constexpr int kWaveRows = 32 * kTX, kWaveCols = 32 * kTY;
+constexpr int kWR = kBT / kWaveRows;         // waves along rows
+constexpr int kWC = kBT / kWaveCols;         // waves along columns
 constexpr int kThreads = 64 * kWR * kWC;     // 512: two waves per SIMD
 constexpr int kKW = 16;                      // words per K tile: one 128-byte line per row
-constexpr int kSteps = kKW / 2;              // MFMA steps (word pairs) per K tile
-constexpr int kLdsW = 2 * kKW + 4;           // u32 per LDS row: the tile and one step of read-ahead
+constexpr int kGroups = kKW / 4;             // 16-byte LDS reads (four MFMA steps) per K tile
+constexpr int kLdsW = 2 * kKW + 4;           // u32 per LDS row (stride = 4 mod 32: 8-lane groups of
+                                             // 16-byte reads cover all banks)
 constexpr uint32_t kE2M1One = 0x22222222u;   // bit 1 of each nibble: 1.0 in E2M1
 constexpr int kFmtFp4 = 4;                   // f8f6f4 operand format code of E2M1
 constexpr int kScaleOne = 0x7f7f7f7f;        // E8M0 block scales of 2^0
@@ -56,8 +64,8 @@ __device__ __forceinline__ v8i spread_fp4(uint32_t x) {
 
 __global__ __launch_bounds__(kThreads) void k_gram(const GramJob *__restrict__ jobs, const GramDb *__restrict__ dbs,
                                                    int32_t *__restrict__ out) {
-    __shared__ uint32_t sA[kBT][kLdsW];
-    __shared__ uint32_t sB[kBT][kLdsW];
+    __shared__ uint32_t sA[2][kBT][kLdsW];  // two buffers of A and B tiles: 144 KB
+    __shared__ uint32_t sB[2][kBT][kLdsW];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const GramJob job = jobs[blockIdx.x];
     const GramDb db = dbs[job.db];
@@ -87,7 +95,7 @@ __global__ __launch_bounds__(kThreads) void k_gram(const GramJob *__restrict__ j
         const uint32_t lr = R % kBT;
         const uint32_t item = (isB ? bj : bi) * kBT + lr;
         pr[q] = item < n ? (gptr<uint4>)(store + (uint64_t)cols[item] * stride) : nullptr;
-        srow[q] = (isB ? &sB[lr][0] : &sA[lr][0]) + 4 * part;
+        srow[q] = (isB ? &sB[0][lr][0] : &sA[0][lr][0]) + 4 * part;
     }
     uint4 rr[kPasses];
     auto fetch = [&](uint64_t k0) {
@@ -98,72 +106,96 @@ __global__ __launch_bounds__(kThreads) void k_gram(const GramJob *__restrict__ j
             if (pr[q] && w < k_end) rr[q] = pr[q][w / 2];
         }
     };
-    auto stash = [&]() {
+    constexpr uint32_t kBufU32 = kBT * kLdsW;  // u32 per LDS buffer of A (or B)
+    auto stash = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < kPasses; ++q) *reinterpret_cast<uint4 *>(srow[q]) = rr[q];
+        for (int q = 0; q < kPasses; ++q) *reinterpret_cast<uint4 *>(srow[q] + buf * kBufU32) = rr[q];
     };
 
-    const uint32_t wr = wave / kWC, wc = wave % kWC;
-    const uint32_t li = lane & 15, lg = lane >> 4;
-    // on a diagonal block, waves whose rows all lie below their columns idle
-    const bool idle = (bi == bj) && (wr * 16 * kTX > wc * 16 * kTY + 16 * kTY - 1);
-    v4f acc[kTX][kTY];
+    // Wave -> (piece of the block, K groups).  Off the diagonal wave w takes
+    // piece (w / kWC, w % kWC) over every K group.  On a diagonal block two of
+    // the eight pieces lie wholly below the diagonal; the six others are
+    // spread so that each SIMD (waves w and w + 4) carries 1.5 pieces: waves
+    // 0-3 take (0,0) (0,1) (1,2) (1,3) whole, waves 4-7 take (0,2) (0,3) on
+    // even K groups and (0,2) (0,3) on odd ones.  A shared piece's two halves
+    // add up in the epilogue's atomics.
+    uint32_t wr = wave / kWC, wc = wave % kWC;
+    int j0 = 0, jstep = 1;
+    if (bi == bj) {
+        static_assert(kWR == 2 && kWC == 4, "diagonal wave map is for 2 x 4 waves");
+        if (wave < 4) {
+            wr = wave >> 1;
+            wc = (wave & 1) + 2 * (wave >> 1);
+        } else {
+            wr = 0;
+            wc = 2 + (wave & 1);
+            j0 = (wave - 4) >> 1;
+            jstep = 2;
+        }
+    }
+    const uint32_t r32 = lane & 31, h = lane >> 5;
+    v16f acc[kTX][kTY];
 #pragma unroll
     for (int x = 0; x < kTX; ++x)
 #pragma unroll
-        for (int y = 0; y < kTY; ++y) acc[x][y] = (v4f){0.f, 0.f, 0.f, 0.f};
+        for (int y = 0; y < kTY; ++y)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[x][y][i] = 0.f;
 
-    const uint32_t *rowA = &sA[wr * 16 * kTX + li][lg];
-    const uint32_t *rowB = &sB[wc * 16 * kTY + li][lg];
-    if (k_begin < k_end) fetch(k_begin);
-    for (uint64_t k0 = k_begin; k0 < k_end; k0 += kKW) {
-        stash();
-        __syncthreads();
-        if (k0 + kKW < k_end) fetch(k0 + kKW);
-        if (!idle) {
-            // The next word pair's LDS reads are issued before this pair's
-            // MFMAs; the last four u32 of an LDS row are read ahead of the
-            // last pair and go unused.
-            uint32_t ca[kTX], nb[kTY];
-            v8i fb[kTY];
-#pragma unroll
-            for (int t = 0; t < kTX; ++t) ca[t] = rowA[t * 16 * kLdsW];
-#pragma unroll
-            for (int t = 0; t < kTY; ++t) fb[t] = spread_fp4(rowB[t * 16 * kLdsW]);
+    const uint32_t *rowA0 = &sA[0][wr * kWaveRows + r32][4 * h];
+    const uint32_t *rowB0 = &sB[0][wc * kWaveCols + r32][4 * h];
+    auto compute = [&](int buf) {
+        const uint32_t *rowA = rowA0 + buf * kBufU32, *rowB = rowB0 + buf * kBufU32;
 #pragma unroll 1
-            for (int s = 0; s < kSteps; ++s) {
-                uint32_t na[kTX];
+        for (int j = j0; j < kGroups; j += jstep) {
+            uint4 qa[kTX], qb[kTY];
 #pragma unroll
-                for (int t = 0; t < kTX; ++t) na[t] = rowA[t * 16 * kLdsW + 4 * (s + 1)];
+            for (int t = 0; t < kTX; ++t) qa[t] = *reinterpret_cast<const uint4 *>(rowA + t * 32 * kLdsW + 8 * j);
 #pragma unroll
-                for (int t = 0; t < kTY; ++t) nb[t] = rowB[t * 16 * kLdsW + 4 * (s + 1)];
+            for (int t = 0; t < kTY; ++t) qb[t] = *reinterpret_cast<const uint4 *>(rowB + t * 32 * kLdsW + 8 * j);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                v8i fb[kTY];
+#pragma unroll
+                for (int y = 0; y < kTY; ++y) fb[y] = spread_fp4((&qb[y].x)[c]);
 #pragma unroll
                 for (int x = 0; x < kTX; ++x) {
-                    const v8i fa = spread_fp4(ca[x]);
+                    const v8i fa = spread_fp4((&qa[x].x)[c]);
 #pragma unroll
                     for (int y = 0; y < kTY; ++y)
-                        acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                        acc[x][y] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
                             fa, fb[y], acc[x][y], kFmtFp4, kFmtFp4, 0, kScaleOne, 0, kScaleOne);
                 }
-#pragma unroll
-                for (int t = 0; t < kTX; ++t) ca[t] = na[t];
-#pragma unroll
-                for (int t = 0; t < kTY; ++t) fb[t] = spread_fp4(nb[t]);
             }
         }
-        __syncthreads();
+    };
+    // double-buffered tiles: tile t+1 is loaded and stashed into the other
+    // buffer while tile t is multiplied; one barrier per tile
+    if (k_begin < k_end) {
+        fetch(k_begin);
+        stash(0);
     }
-    if (idle) return;
-    // D tile layout: lane holds rows 4*(lane/16) + r, column lane % 16
+    __syncthreads();
+    int cur = 0;
+    for (uint64_t k0 = k_begin; k0 < k_end; k0 += kKW) {
+        const bool more = k0 + kKW < k_end;
+        if (more) fetch(k0 + kKW);
+        compute(cur);
+        if (more) stash(cur ^ 1);
+        __syncthreads();
+        cur ^= 1;
+    }
+    // D tile layout of the 32 x 32 form: element i of lane l is row
+    // 8 (i / 4) + 4 (l / 32) + i % 4, column l % 32
 #pragma unroll
     for (int x = 0; x < kTX; ++x)
 #pragma unroll
         for (int y = 0; y < kTY; ++y) {
-            const uint32_t col = bj * kBT + wc * 16 * kTY + y * 16 + li;
+            const uint32_t col = bj * kBT + wc * kWaveCols + y * 32 + r32;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const uint32_t row = bi * kBT + wr * 16 * kTX + x * 16 + lg * 4 + r;
-                const int v = (int)acc[x][y][r];
+            for (int i = 0; i < 16; ++i) {
+                const uint32_t row = bi * kBT + wr * kWaveRows + x * 32 + 8 * (i / 4) + 4 * h + (i % 4);
+                const int v = (int)acc[x][y][i];
                 if (row < n && col < n && row <= col && v) atomicAdd(res + (uint64_t)row * ld + col, v);
             }
         }
@@ -178,14 +210,17 @@ constexpr uint64_t kMaxSliceBits = 1ull << 23;
 constexpr int diagonal_active_waves() {
     int a = 0;
     for (int wr = 0; wr < kWR; ++wr)
-        for (int wc = 0; wc < kWC; ++wc) a += !(wr * 16 * kTX > wc * 16 * kTY + 16 * kTY - 1);
+        for (int wc = 0; wc < kWC; ++wc) a += !(wr * kWaveRows > wc * kWaveCols + kWaveCols - 1);
     return a;
 }
+
 
 int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<GramJob> &jobs,
               uint64_t *out_elems) {
     jobs.clear();
     // work of a block: its K tiles x the share of its waves with work
+    // (a diagonal block's six pieces are spread 1.5 per SIMD, so it takes
+    // 6/8 of a full block's time)
     const double diag = (double)diagonal_active_waves() / (kWR * kWC);
     double total = 0;
     uint64_t out = 0;

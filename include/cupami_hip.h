@@ -292,6 +292,19 @@ int cpm_stats_reset(cpm_ctx *ctx);
  * it adds no synchronization). */
 int cpm_set_timing(cpm_ctx *ctx, int on);
 
+/* ---- measured roofs (no counterpart in the reference) --------------------
+ * Device microbenchmarks that give the bounds the path's kernels are placed
+ * against.  cpm_roof_stream: AND + popcount of the two halves of d_buf
+ * (`bytes`, a multiple of 4096, device memory), `reps` passes, with
+ * xcd_sliced != 0 each XCD reading only its eighth of each half; *ms is the
+ * launch's event time, *bytes_read the bytes loaded.  d_sink: one device
+ * u64 the kernel adds into.  cpm_roof_mfma: `iters` rounds of independent FP4
+ * MFMAs on every SIMD, shape 16 (16x16x128) or 32 (32x32x64); *ops counts 2 per multiply-add; d_sink: one
+ * float per block (8 per CU), never written in practice. */
+int cpm_roof_stream(cpm_ctx *ctx, const void *d_buf, uint64_t bytes, uint32_t reps, int xcd_sliced,
+                    unsigned long long *d_sink, double *ms, double *bytes_read);
+int cpm_roof_mfma(cpm_ctx *ctx, int shape, uint32_t iters, float *d_sink, double *ms, double *ops);
+
 #ifdef __cplusplus
 }
 #endif
