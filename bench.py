@@ -81,9 +81,13 @@ def main():
     ap.add_argument("--workload", default="eclat_1m_1k",
                     choices=["eclat_1m_1k", "eclat_4m_1k", "pfeclat_2m_500", "batch_256", "serve", "ingest_1m_1k", "roofs",
                              "output_1m_1k", "quickstart"])
-    ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
-    ap.add_argument("--qps", type=float, default=500.0, help="serve: mean arrival rate")
+    ap.add_argument("--requests", type=int, default=10_000, help="serve: number of requests")
+    ap.add_argument("--qps", type=float, default=1000.0, help="serve: mean arrival rate")
+    ap.add_argument("--pool", type=int, default=48, help="serve: databases the requests are drawn against")
+    ap.add_argument("--slots", type=int, default=3, help="serve: concurrent searches (each its own stream)")
     ap.add_argument("--hbm-budget-gb", type=float, default=8.0, help="serve: admission budget")
+    ap.add_argument("--p99-bound-ms", type=float, default=25.0, help="serve: stated p99 latency bound")
+    ap.add_argument("--cpu-sample-s", type=float, default=15.0, help="CPU baseline: seconds of oracle work")
     ap.add_argument("--no-extras", action="store_true",
                     help="eclat_1m_1k: skip the batched and serving summaries")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -317,24 +321,30 @@ def run_quickstart(args):
 
 def extras(args):
     """Compact summaries of the batched (256 databases) and serving
-    workloads for the headline line (their own workloads report in full)."""
+    workloads for the headline line (their own workloads report in full);
+    both check their results against the oracle."""
     sub = argparse.Namespace(**vars(args))
-    sub.steps, sub.warmup, sub.no_cpu_baseline = 3, 1, True
+    sub.steps, sub.warmup, sub.no_cpu_baseline, sub.cpu_sample_s = 3, 1, False, 5.0
     b = run_batch(sub)
     batched = {"workload": "batch_256", "databases": b["config"]["databases"],
                "patterns": b["config"]["patterns"], "itemsets_per_s": b["value"],
                "ms_per_step": b["ms_per_step"], "one_at_a_time_ms": b["plain"]["one_at_a_time_ms"],
                "speedup_vs_one_at_a_time": b["plain"]["speedup_batched"],
                "speedup_vs_per_class_subset": b["plain"]["per_class_subset"]["speedup_batched"],
-               "schedule": b["config"]["schedule"], "parity": b["parity"]}
+               "schedule": b["config"]["schedule"], "cpu_baseline": b["cpu_baseline"], "parity": b["parity"]}
     v = run_serve(sub)
     serve = {"workload": "serve", "requests": v["config"]["requests"], "qps": v["config"]["qps"],
+             "pool_databases": v["config"]["pool_databases"], "slots": v["config"]["slots"],
              "p50_ms": v["served"]["p50_ms"], "p99_ms": v["served"]["p99_ms"],
+             "p99_bound_ms": v["p99_bound_ms"], "p99_within_bound": v["p99_within_bound"],
              "itemsets_per_s": v["served"]["itemsets_per_s"],
              "one_at_a_time_p50_ms": v["one_at_a_time"]["p50_ms"],
              "one_at_a_time_p99_ms": v["one_at_a_time"]["p99_ms"],
              "one_at_a_time_itemsets_per_s": v["one_at_a_time"]["itemsets_per_s"],
-             "batches": v["served"]["batches"], "max_batch": v["served"]["max_batch"], "parity": v["parity"]}
+             "one_at_a_time_resident_p99_ms": v["one_at_a_time_resident"]["p99_ms"],
+             "batches": v["served"]["batches"], "max_batch": v["served"]["max_batch"],
+             "slots_busy_max": v["served"]["slots_busy_max"], "cpu_baseline": v["cpu_baseline"],
+             "parity": v["parity"]}
     return batched, serve
 
 
@@ -742,93 +752,112 @@ def run_output(args):
 
 
 def run_serve(args):
-    """Mining requests with Poisson arrivals, served with continuous batching
-    and HBM-footprint admission, against the same stream served one request
-    at a time."""
+    """Served mining: ``--requests`` requests (default 10,000) with Poisson
+    arrivals at ``--qps`` over a seeded pool of ``--pool`` databases drawn
+    from the served workload's size distribution (T log-uniform in
+    [10^4, 2*10^6], I uniform in [50, 2000], lam in [4, 12], Zipf s in
+    [0.9, 1.2]); each request takes a database and its own minSup
+    (log-uniform in [0.001, 0.05]).  Served by the continuous-batching server
+    (slots with their own streams, stores resident under the HBM budget, cold
+    at the start) against the same stream one request at a time -- the plain
+    way (store built, mined and freed per request) and on resident stores.
+    Every request's result is compared across the three (order-free digest);
+    a bounded sample also against the oracle, which is the CPU baseline."""
     import numpy as np
     from cupami_amd import _hip, datagen
-    from cupami_amd.serve import Request, Server, latency_summary, mine_one_at_a_time
+    from cupami_amd.serve import Request, Server, latency_summary, mine_one_at_a_time, mine_sequential_resident
 
     t = time.perf_counter()
-    raw = datagen.serve_requests(args.requests, args.qps)
-    reqs, pair_reqs = [], []
-    for i, (arrival, db, ms) in enumerate(raw):
+    specs = datagen.serve_pool(args.pool)
+    dbs = datagen.build_pool(specs)
+    data = []
+    for db in dbs:
         labels, cols = datagen.label_columns(db.items)
-        offsets = np.searchsorted(db.rows, np.arange(db.n_transactions + 1, dtype=np.uint64)).astype(np.uint64)
-        # the BIN layout (offsets + raw ids; the device finds the labels) and
-        # the pairs layout (rows + label columns) of the same database
-        reqs.append(Request(i, arrival, None, None, db.n_transactions, len(labels), ms,
-                            offsets=offsets, ids=db.items, universe=db.n_items))
-        pair_reqs.append(Request(i, arrival, db.rows, cols, db.n_transactions, len(labels), ms))
-    pairs = sum(r.entries for r in reqs)
-    log(f"bench: generated {len(reqs)} requests ({pairs} pairs) in {time.perf_counter() - t:.1f}s")
+        data.append((db, cols, len(labels)))
+    stream = datagen.serve_stream(args.requests, args.qps, len(dbs))
+    reqs = []
+    for i, (arrival, k, frac) in enumerate(stream):
+        db, cols, items = data[k]
+        reqs.append(Request(i, arrival, db.rows, cols, db.n_transactions, items,
+                            int(np.ceil(frac * db.n_transactions)), key=k))
+    pairs = sum(db.rows.size for db in dbs)
+    log(f"bench: generated a pool of {len(dbs)} databases ({pairs} pairs) and {len(reqs)} requests "
+        f"in {time.perf_counter() - t:.1f}s")
 
     ctx = _hip.context(0)
+    budget = int(args.hbm_budget_gb * (1 << 30))
+    # warm the server's contexts (buffers, graphs) on other requests, then
+    # drop their stores: the timed run starts with no store resident
+    server = Server(hbm_budget=budget, slots=args.slots)
+    warm = [Request(i, 0.0, r.rows, r.cols, r.n_transactions, r.n_items, r.minsup, key=r.key)
+            for i, r in enumerate(reqs[: min(64, len(reqs))])]
+    server.run(warm)
+    server.evict_all()
+    mine_one_at_a_time(warm[:8], ctx)
+    t = time.perf_counter()
+    served = server.run(reqs)
+    served_s = time.perf_counter() - t
+    stats = dict(server.stats)
+    sizes = [len(b[3]) for b in server.batches]
+    server.close()
+    log(f"bench: served {len(reqs)} requests in {served_s:.2f}s ({stats})")
+    t = time.perf_counter()
+    alone = mine_one_at_a_time(reqs, ctx)
+    alone_s = time.perf_counter() - t
+    log(f"bench: one at a time {alone_s:.2f}s")
+    t = time.perf_counter()
+    resident = mine_sequential_resident(reqs, ctx)
+    resident_s = time.perf_counter() - t
+    log(f"bench: one at a time, resident stores {resident_s:.2f}s")
 
-    def schedules(requests):
-        # both schedules twice, alternating (the first runs grow the device
-        # buffers); each keeps its faster run
-        best = {"served": (float("inf"), None, None), "alone": (float("inf"), None, None)}
-        for _ in range(2):
-            srv = Server(ctx, hbm_budget=int(args.hbm_budget_gb * (1 << 30)))
-            t = time.perf_counter()
-            out = srv.run(requests)
-            dt = time.perf_counter() - t
-            if dt < best["served"][0]:
-                best["served"] = (dt, out, srv)
-            t = time.perf_counter()
-            out = mine_one_at_a_time(requests, ctx)
-            dt = time.perf_counter() - t
-            if dt < best["alone"][0]:
-                best["alone"] = (dt, out, None)
-        return best
-
-    best = schedules(reqs)
-    served_s, served, server = best["served"]
-    alone_s, alone, _ = best["alone"]
-    by_pairs = schedules(pair_reqs)
+    dig = [(o.patterns.digest(), a.patterns.digest(), q.patterns.digest())
+           for o, a, q in zip(served, alone, resident)]
+    parity = all(not o.rejected for o in served) and all(x == y == z for x, y, z in dig)
     n_pat = sum(len(o.patterns) for o in served)
-    sizes = [len(b[2]) for b in server.batches]
-    log(f"bench: served {len(reqs)} requests in {served_s:.2f}s ({len(sizes)} batches, max {max(sizes)}), "
-        f"one at a time {alone_s:.2f}s; pairs input {by_pairs['served'][0]:.2f}s / {by_pairs['alone'][0]:.2f}s")
-
-    parity = all(a.patterns == b.patterns == c.patterns
-                 for a, b, c in zip(served, alone, by_pairs["served"][1]) if not a.rejected)
     cpu = None
     if not args.no_cpu_baseline:
         import oracle
         threads = oracle.threads()
-        sample = pair_reqs[: max(1, len(reqs) // 3)]
-        hosts = [oracle.bits_build(r.rows, r.cols, r.n_transactions, r.n_items) for r in sample]
-        t = time.perf_counter()
-        expect = [oracle.eclat(h, r.minsup) for h, r in zip(hosts, sample)]
-        cpu_s = time.perf_counter() - t
-        parity = parity and all(served[r.rid].patterns == e for r, e in zip(sample, expect))
-        n_exp = sum(len(e) for e in expect)
+        hosts, expect, cpu_s, n_exp, sample = {}, [], 0.0, 0, 0
+        for r in reqs:
+            if cpu_s > args.cpu_sample_s:
+                break
+            if r.key not in hosts:
+                hosts[r.key] = oracle.bits_build(r.rows, r.cols, r.n_transactions, r.n_items)
+            t = time.perf_counter()
+            e = oracle.eclat(hosts[r.key], r.minsup)
+            cpu_s += time.perf_counter() - t
+            n_exp += len(e)
+            sample += 1
+            parity = parity and served[r.rid].patterns == e
         cpu = {"value": round(n_exp / cpu_s, 1), "unit": "itemsets/s", "cores": threads, "kind": "port",
-               "sample": f"the first {len(sample)} requests, one run ({n_exp} itemsets in {cpu_s:.2f}s): "
-                         "oracle/cpm_oracle.c vertical Eclat, OpenMP over prefix classes"}
+               "sample": f"the first {sample} requests, one run ({n_exp} itemsets in {cpu_s:.2f}s): "
+                         "oracle/cpm_oracle.c vertical Eclat, OpenMP over prefix classes; each result "
+                         "also compared with the served one"}
     log(f"bench: parity={parity}")
     lat = latency_summary(served)
+    lat_alone = latency_summary(alone)
+    lat_res = latency_summary(resident)
+    bound_ok = lat["p99_ms"] is not None and lat["p99_ms"] <= args.p99_bound_ms
     line = {
         "metric": "serve p99 latency (ms), continuous batching", "value": lat["p99_ms"], "unit": "ms",
         "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(served_s * 1e3, 1),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
-        "data": "synthetic (numpy PCG64 seeds 505, 505000+i)",
-        "config": {"workload": "serve", "requests": len(reqs), "qps": args.qps, "pairs": pairs,
-                   "hbm_budget_gb": args.hbm_budget_gb, "patterns": n_pat, "parallelism": "single GPU"},
+        "data": "synthetic (numpy PCG64: pool seeds 505000+i, request stream seed 505)",
+        "config": {"workload": "serve", "requests": len(reqs), "qps": args.qps, "pool_databases": len(dbs),
+                   "pool_pairs": pairs, "slots": args.slots, "hbm_budget_gb": args.hbm_budget_gb,
+                   "patterns": n_pat, "parallelism": "single GPU"},
         "roofline": None,
         "cpu_baseline": cpu,
+        "p99_bound_ms": args.p99_bound_ms, "p99_within_bound": bound_ok,
         "served": {**lat, "itemsets_per_s": round(n_pat / served_s, 1), "makespan_s": round(served_s, 3),
                    "batches": len(sizes), "max_batch": max(sizes), "mean_batch": round(sum(sizes) / len(sizes), 2),
+                   "slots_busy_max": stats["busy_max"], "store_builds": stats["builds"],
+                   "evictions": stats["evictions"], "resident_hits": stats["resident_hits"],
                    "rejected": sum(1 for o in served if o.rejected)},
-        "one_at_a_time": {**latency_summary(alone), "itemsets_per_s": round(n_pat / alone_s, 1),
-                          "makespan_s": round(alone_s, 3)},
-        "input": "BIN layout (offsets + raw ids, store and labels built on the device)",
-        "pairs_input": {"served": {**latency_summary(by_pairs["served"][1]),
-                                   "makespan_s": round(by_pairs["served"][0], 3)},
-                        "one_at_a_time": {**latency_summary(by_pairs["alone"][1]),
-                                          "makespan_s": round(by_pairs["alone"][0], 3)}},
+        "one_at_a_time": {**lat_alone, "itemsets_per_s": round(n_pat / alone_s, 1), "makespan_s": round(alone_s, 3)},
+        "one_at_a_time_resident": {**lat_res, "itemsets_per_s": round(n_pat / resident_s, 1),
+                                   "makespan_s": round(resident_s, 3)},
         "parity": parity,
     }
     return line

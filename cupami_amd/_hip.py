@@ -644,6 +644,39 @@ class Patterns:
         return dict(zip(sets, sup))
 
 
+def _mix64(x):
+    """splitmix64 finaliser on a uint64 array (wrapping arithmetic)."""
+    x = x.astype(np.uint64, copy=True)
+    x ^= x >> np.uint64(30)
+    x *= np.uint64(0xBF58476D1CE4E5B9)
+    x ^= x >> np.uint64(27)
+    x *= np.uint64(0x94D049BB133111EB)
+    x ^= x >> np.uint64(31)
+    return x
+
+
+def forest_digest(parent, item, support):
+    """``PatternArrays.digest`` of a forest given as arrays."""
+    n = len(item)
+    if n == 0:
+        return (0, 0)
+    parent = np.asarray(parent, dtype=np.int64)
+    parent = np.where(parent == NO_PARENT, -1, parent)
+    hi = _mix64(np.asarray(item, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15))
+    with np.errstate(over="ignore"):
+        h = hi.copy()
+        p = parent.copy()
+        live = p >= 0
+        while live.any():
+            idx = np.nonzero(live)[0]
+            h[idx] += hi[p[idx]]
+            p[idx] = parent[p[idx]]
+            live[idx] = p[idx] >= 0
+        m = _mix64(h ^ _mix64(np.asarray(support, dtype=np.uint64)))
+        total = int(m.sum(dtype=np.uint64))
+    return (n, total)
+
+
 class PatternArrays(Mapping):
     """One database's result as arrays (parent, item, support: the forest
     of ``Patterns``), read as the mapping ``{column tuple: support}``.  The
@@ -667,6 +700,14 @@ class PatternArrays(Mapping):
 
     def __len__(self):
         return len(self.item)
+
+    def digest(self):
+        """Order-free fingerprint ``(count, u64)`` of the result: each
+        pattern's itemset hashes to the sum of its items' hashes (walked
+        through its parents), mixed with its support, and the mixes are
+        summed.  Equal results give equal digests whatever order their
+        patterns are in; for comparing large results without building dicts."""
+        return forest_digest(self.parent, self.item, self.support)
 
     def __getitem__(self, key):
         return self._materialise()[key]

@@ -24,8 +24,17 @@ int fail(int code, const char *fmt, ...) {
 
 void clear_error() { g_error.clear(); }
 
+// Serialises device/pinned allocation and release with the graph captures
+// of every context in the process: an allocation call that synchronizes the
+// device must not run on one thread while another thread records a stream.
+std::recursive_mutex &alloc_capture_mutex() {
+    static std::recursive_mutex mu;
+    return mu;
+}
+
 int DBuf::ensure(size_t bytes) {
     if (bytes <= cap && p) return CPM_OK;
+    std::lock_guard<std::recursive_mutex> lock(alloc_capture_mutex());
     release();
     size_t want = std::max<size_t>(bytes, 256);
     hipError_t e = hipMalloc(&p, want);
@@ -41,13 +50,16 @@ int DBuf::ensure(size_t bytes) {
 }
 
 void DBuf::release() {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    std::lock_guard<std::recursive_mutex> lock(alloc_capture_mutex());
+    (void)hipFree(p);
     p = nullptr;
     cap = 0;
 }
 
 int HBuf::ensure(size_t bytes) {
     if (bytes <= cap && p) return CPM_OK;
+    std::lock_guard<std::recursive_mutex> lock(alloc_capture_mutex());
     release();
     size_t want = std::max<size_t>(bytes, 4096);
     hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
@@ -63,7 +75,9 @@ int HBuf::ensure(size_t bytes) {
 }
 
 void HBuf::release() {
-    if (p) (void)hipHostFree(p);
+    if (!p) return;
+    std::lock_guard<std::recursive_mutex> lock(alloc_capture_mutex());
+    (void)hipHostFree(p);
     p = nullptr;
     cap = 0;
 }

@@ -7,6 +7,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <utility>
 #include <string>
@@ -46,6 +47,10 @@ struct DBuf {
     ~DBuf() { release(); }
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
+
+// Held around device / pinned allocation and release and around every
+// graph capture, process-wide (core.hip).
+std::recursive_mutex &alloc_capture_mutex();
 
 // Pinned host staging buffer.
 struct HBuf {

@@ -1348,6 +1348,7 @@ struct Search {
                 w.graph[c] = nullptr;
             }
             hipGraph_t g = nullptr;
+            std::lock_guard<std::recursive_mutex> lock(alloc_capture_mutex());
             CPM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             int s = chunk(c);
             hipError_t e = hipStreamEndCapture(ctx->stream, &g);
@@ -1865,7 +1866,8 @@ struct Search {
 
 int check_bits(const cpm_bits *b, cpm_ctx *ctx) {
     if (!b) return fail(CPM_ERR_INVALID, "null store");
-    if (b->ctx != ctx) return fail(CPM_ERR_INVALID, "store belongs to another context");
+    if (!b->ctx || b->ctx->device != ctx->device)
+        return fail(CPM_ERR_INVALID, "store belongs to another device");
     if (b->n_transactions >= 0xffffffffull)
         return fail(CPM_ERR_INVALID, "supports are counted in 32 bits: too many transactions");
     return CPM_OK;
@@ -2189,7 +2191,8 @@ int cpm_count_supports_noalloc(cpm_ctx *ctx, const cpm_bits *b, const uint32_t *
     clear_error();
     CPM_CHECK(bind(ctx));
     if (!b) return fail(CPM_ERR_INVALID, "null store");
-    if (b->ctx != ctx) return fail(CPM_ERR_INVALID, "store belongs to another context");
+    if (!b->ctx || b->ctx->device != ctx->device)
+        return fail(CPM_ERR_INVALID, "store belongs to another device");
     if (n_cand == 0) return CPM_OK;
     if (!d_cand || !d_supports) return fail(CPM_ERR_INVALID, "null candidates/supports");
     if (n_cand > kNoAllocCands)

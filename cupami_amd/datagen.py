@@ -57,16 +57,15 @@ def _sample_baskets(rng, n, n_items, lam, zipf_s, chunk=200_000):
         m = stop - start
         need = lengths[start:stop]
         width = int(need.max())
-        chosen = np.zeros((m, n_items), dtype=bool)
-        out = np.zeros((m, width), dtype=np.int64)
+        out = np.full((m, width), -1, dtype=np.int64)   # -1: slot not filled yet
         count = np.zeros(m, dtype=np.int64)
         active = np.arange(m)
         while active.size:
             draw = np.searchsorted(cdf, rng.random(active.size), side="right")
             np.minimum(draw, n_items - 1, out=draw)
-            fresh = ~chosen[active, draw]
+            # a draw is new unless the basket already holds it
+            fresh = ~(out[active] == draw[:, None]).any(axis=1)
             a, d = active[fresh], draw[fresh]
-            chosen[a, d] = True
             out[a, count[a]] = d
             count[a] += 1
             active = active[count[active] < need[active]]
@@ -152,6 +151,66 @@ def serve_requests(n_requests, qps, seed=505):
         db = transactional(int(t[i]), int(items[i]), float(lam[i]), float(zipf[i]), seed * 1000 + i)
         out.append((float(arrival[i]), db, int(np.ceil(frac[i] * t[i]))))
     return out
+
+
+@dataclass
+class DbSpec:
+    """Parameters of one seeded database (generated by ``build``)."""
+    key: int
+    n_transactions: int
+    n_items: int
+    lam: float
+    zipf_s: float
+    seed: int
+
+    def build(self) -> Database:
+        return transactional(self.n_transactions, self.n_items, self.lam, self.zipf_s, self.seed)
+
+
+def serve_pool(n_db=48, seed=505):
+    """The databases served requests are drawn against: ``n_db`` specs with
+    T log-uniform in [10^4, 2*10^6] baskets, I uniform in [50, 2000] items,
+    lam uniform in [4, 12], Zipf s uniform in [0.9, 1.2]; database i is drawn
+    with seed ``seed * 1000 + i``.  Generate them with ``build_pool``."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = np.exp(rng.uniform(np.log(1e4), np.log(2e6), n_db)).round().astype(np.int64)
+    items = rng.integers(50, 2001, n_db)
+    lam = rng.uniform(4.0, 12.0, n_db)
+    zipf = rng.uniform(0.9, 1.2, n_db)
+    return [DbSpec(i, int(t[i]), int(items[i]), float(lam[i]), float(zipf[i]), seed * 1000 + i) for i in range(n_db)]
+
+
+def _build_spec(spec):
+    return spec.build()
+
+
+def build_pool(specs, workers=None):
+    """Databases of ``specs`` (in order), generated in worker processes."""
+    import multiprocessing as mp
+    import os
+    workers = workers or min(len(specs), max(1, min(16, os.cpu_count() or 1)))
+    if workers <= 1 or len(specs) <= 1:
+        return [s.build() for s in specs]
+    # largest first, so the long ones do not finish last
+    order = sorted(range(len(specs)), key=lambda i: -specs[i].n_transactions * specs[i].n_items)
+    with mp.get_context("fork").Pool(workers) as pool:
+        built = pool.map(_build_spec, [specs[i] for i in order], chunksize=1)
+    out = [None] * len(specs)
+    for i, db in zip(order, built):
+        out[i] = db
+    return out
+
+
+def serve_stream(n_requests, qps, n_db, seed=505):
+    """Served requests over a pool of ``n_db`` databases: Poisson arrivals at
+    ``qps``; each request takes a database uniformly at random and a relative
+    minimum support log-uniform in [0.001, 0.05].  Returns a list of
+    ``(arrival_seconds, database_index, minSup_fraction)``."""
+    rng = np.random.Generator(np.random.PCG64(seed * 7919 + 1))
+    arrival = np.cumsum(rng.exponential(1.0 / qps, n_requests))
+    db = rng.integers(0, n_db, n_requests)
+    frac = np.exp(rng.uniform(np.log(0.001), np.log(0.05), n_requests))
+    return [(float(a), int(d), float(f)) for a, d, f in zip(arrival, db, frac)]
 
 
 def label_columns(item_ids: np.ndarray, n_items_hint=None):

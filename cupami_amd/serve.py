@@ -1,37 +1,52 @@
-"""Serving loop: many mining requests on one GPU, with continuous batching
-and admission by HBM footprint.
+"""Serving loop: mining requests on one GPU with continuous batching,
+admission by HBM footprint and device-resident stores.
 
-Each request is one database (COO pairs in host memory) and an absolute
-minimum support.  The loop keeps a FIFO queue of arrived requests.  Whenever
-the device is free it admits, in arrival order, waiting requests while their
-footprints fit the HBM budget and their pairs fit the batch work cap (a
-request larger than the whole budget is rejected; one that does not fit
-waits for the next turn), builds all admitted stores with one batch call,
-mines them with one search (``cpm_eclat_mine_batch``) and hands each request
-its own patterns.  Requests that arrive while a batch runs join the next
-batch.  The work cap keeps batching where it pays: a request with millions
-of pairs fills the device by itself and runs alone (packing it with others
-would only make them wait for its deeper levels), while small requests,
-which leave the device mostly idle one at a time, share a search.
+A request is one database (pairs in host memory, or a BIN file's arrays)
+and an absolute minimum support.  Requests that name the same database
+(``Request.key``) share its vertical store, which stays resident in HBM
+after its first build.  The device work of a request is the reference's
+miner on that store (``cuEclat.mine``, ``cuEclat.py:168-245``); a request's
+itemsets are the same whichever way it is served.
 
-``mine_one_at_a_time`` serves the same arrival stream one request at a time
-(one store build and one search per request), the plain way to run the
-reference's miner repeatedly; both return per-request latencies.  A
-request's patterns are the same whichever way it is served.
+Three kinds of worker overlap on the device, each with its own HIP stream
+(streams of the default priority):
+
+* the dispatcher (the caller's thread) moves requests in as they arrive.  A
+  request whose store is resident (or being built) joins it at once; one
+  that needs a new store is admitted, in arrival order, as soon as its
+  footprint fits the HBM budget -- stores no request holds are evicted,
+  least recently used first, to make room.  A request whose footprint
+  exceeds the whole budget is rejected;
+* a builder thread builds admitted stores on its own context (and frees
+  evicted ones, as the context that built them);
+* ``slots`` slot threads, each with its own context, take every ready
+  request waiting (up to ``max_batch`` requests and ``max_batch_entries``
+  store entries) as one micro-batch, mine it -- one search for one request,
+  one batched search (``cpm_eclat_mine_batch``) for several -- and hand each
+  request its result.
+
+So a request that arrives while others are on the GPU does not wait for
+them: it is admitted as memory allows, its store is built beside the
+running searches, and it is mined by the next free slot, its work
+overlapping the other slots' on the device.
+
+``mine_one_at_a_time`` serves the same arrival stream the plain way -- one
+request at a time, its store built, mined and freed -- and
+``mine_sequential_resident`` one at a time on stores that stay resident
+(the residency without the overlap).
 """
 
 from __future__ import annotations
 
+import threading
 import time
-from collections import deque
+from collections import OrderedDict, deque
 from collections.abc import Mapping
 from dataclasses import dataclass, field
 
 import numpy as np
 
 from . import _hip
-
-SEARCH_RESERVE = 64 << 20   # bytes per request for its share of the search buffers
 
 
 def row_stride_words(n_transactions):
@@ -44,10 +59,10 @@ class Request:
     """A database to mine, as pairs (``rows``, ``cols``: item columns in
     label string order) or in the BIN file layout (``offsets`` u64[T+1] and
     raw item ``ids`` u32 up to ``universe``, ``rows``/``cols`` None): the
-    device builds the latter's store and labels itself, 4 bytes per entry
-    and 8 per basket over PCIe instead of 12 per pair.  Its columns are the
-    distinct ids in string order, the same numbering ``label_columns``
-    gives the pairs."""
+    device builds the latter's store and labels itself.  Its columns are the
+    distinct ids in string order, the numbering ``label_columns`` gives the
+    pairs.  Requests with equal ``key`` (not None) mine the same database
+    and share its resident store; ``key`` None is a database of its own."""
     rid: int
     arrival: float            # seconds after the start of the run
     rows: np.ndarray | None   # uint64 transaction index per pair
@@ -58,18 +73,25 @@ class Request:
     offsets: np.ndarray | None = None
     ids: np.ndarray | None = None
     universe: int = 0
+    key: object = None
 
     @property
     def entries(self):
         return int(self.ids.size if self.ids is not None else self.rows.size)
 
+    @property
+    def store_key(self):
+        return ("request", self.rid) if self.key is None else ("db", self.key)
+
+    def store_bytes(self):
+        return self.n_items * row_stride_words(self.n_transactions) * 8
+
     def footprint(self):
-        """Device bytes while it is being served: its store, its staged
-        input, and a share of the search buffers."""
-        store = self.n_items * row_stride_words(self.n_transactions) * 8
+        """Device bytes of its resident store plus the staging of its
+        input while the store is built."""
         staged = (4 * self.entries + 8 * (self.n_transactions + 1) if self.ids is not None
                   else 12 * self.entries)
-        return store + staged + SEARCH_RESERVE
+        return self.store_bytes() + staged
 
 
 def _build(ctx, r):
@@ -92,88 +114,255 @@ class Outcome:
         return self.finish - self.arrival
 
 
+class _Store:
+    """A resident (or building) store and the requests holding it."""
+
+    def __init__(self, key, nbytes):
+        self.key = key
+        self.nbytes = nbytes
+        self.bits = None
+        self.ready = False
+        self.error = None
+        self.holders = 0          # admitted requests not yet finished
+        self.waiting = []         # requests waiting for the build
+
+
 class Server:
-    def __init__(self, ctx=None, hbm_budget=8 << 30, max_batch=256, max_batch_pairs=4 << 20):
-        self.ctx = ctx or _hip.context(0)
+    """Continuous-batching server (see the module docstring)."""
+
+    def __init__(self, device=0, hbm_budget=8 << 30, slots=3, max_batch=64, max_batch_entries=16 << 20,
+                 contexts=None):
+        """``contexts(device)`` makes each worker's context (default: a new
+        ``_hip.Context`` -- its own HIP stream -- per worker)."""
+        self.device = int(device)
         self.hbm_budget = int(hbm_budget)
-        self.max_batch = int(max_batch)
-        self.max_batch_pairs = int(max_batch_pairs)
-        self.batches = []      # (start, end, request ids)
+        self.slots = max(1, int(slots))
+        self.max_batch = max(1, int(max_batch))
+        self.max_batch_entries = int(max_batch_entries)
+        make = contexts or _hip.Context
+        self.builder_ctx = make(self.device)
+        self.slot_ctx = [make(self.device) for _ in range(self.slots)]
+        self._stores = OrderedDict()   # key -> _Store, least recently used first
+        self._used = 0
+        self.batches = []              # (slot, start, end, request ids)
+        self.stats = {}
 
-    def _admit(self, queue):
-        """Admit in arrival order while the HBM budget and the work cap allow
-        (the first request always fits: oversize ones were rejected)."""
-        batch, used, pairs = [], 0, 0
-        while queue and len(batch) < self.max_batch:
-            fp = queue[0].footprint()
-            n = queue[0].entries
-            if batch and (used + fp > self.hbm_budget or pairs + n > self.max_batch_pairs):
-                break
-            batch.append(queue.popleft())
-            used += fp
-            pairs += n
-        return batch
+    def evict_all(self):
+        """Free every resident store (between runs)."""
+        for st in self._stores.values():
+            if st.bits is not None:
+                st.bits.close()
+        self._stores.clear()
+        self._used = 0
 
-    def _mine(self, batch):
-        if len(batch) == 1:  # nothing to share: the single-store path
-            r = batch[0]
-            bits = _build(self.ctx, r)
-            try:
-                return [_hip.PatternArrays.of(self.ctx.eclat_mine(bits, r.minsup))]
-            finally:
-                bits.close()
-        if all(r.ids is None for r in batch):
-            stores = self.ctx.build_bits_batch([(r.rows, r.cols, r.n_transactions, r.n_items) for r in batch])
-        else:
-            stores = []
-            try:
-                for r in batch:
-                    stores.append(_build(self.ctx, r))
-            except BaseException:
-                for st in stores:
-                    st.close()
-                raise
-        try:
-            forest = self.ctx.eclat_mine_batch(stores, [r.minsup for r in batch])
-        finally:
-            for s in stores:
-                s.close()
-        return forest.split_arrays(len(batch))
+    def close(self):
+        """Free the resident stores and the contexts."""
+        for st in self._stores.values():
+            if st.bits is not None:
+                st.bits.close()
+        self._stores.clear()
+        self._used = 0
+        for c in [self.builder_ctx, *self.slot_ctx]:
+            c.close()
 
+    # ---- the run ---------------------------------------------------------------
     def run(self, requests):
-        """Serve ``requests`` (any order; arrival times relative to now).
-        Returns one ``Outcome`` per request, in request-id order."""
+        """Serve ``requests`` (arrival times relative to now).  Returns one
+        ``Outcome`` per request, in request-id order.  Resident stores stay
+        for the next run."""
         pending = deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
         out = {r.rid: Outcome(r.rid, r.arrival) for r in requests}
-        queue = deque()
+        cv = threading.Condition()
+        admit = deque()          # arrived, needing a new store, in arrival order
+        ready = deque()          # (request, store) ready to mine
+        builds = deque()         # ("build", store, request) / ("free", bits)
+        left = [len(requests)]
+        stop = [False]
+        failure = []
+        counters = {"builds": 0, "evictions": 0, "resident_hits": 0, "busy_max": 0}
+        busy = [0]
+        self.batches = []
         t0 = time.perf_counter()
-        while pending or queue:
-            now = time.perf_counter() - t0
-            while pending and pending[0].arrival <= now:
-                r = pending.popleft()
-                if r.footprint() > self.hbm_budget:
-                    out[r.rid].rejected = f"footprint {r.footprint()} B exceeds the budget {self.hbm_budget} B"
-                    out[r.rid].finish = now
-                else:
-                    queue.append(r)
-            if not queue:
-                if pending:
-                    time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
-                continue
-            batch = self._admit(queue)
-            start = time.perf_counter() - t0
-            results = self._mine(batch)
-            end = time.perf_counter() - t0
-            k = len(self.batches)
-            self.batches.append((start, end, [r.rid for r in batch]))
-            for r, res in zip(batch, results):
-                o = out[r.rid]
-                o.finish, o.patterns, o.batch = end, res, k
+
+        def now():
+            return time.perf_counter() - t0
+
+        def finish(r, patterns=None, rejected=""):
+            o = out[r.rid]
+            o.finish = now()
+            if patterns is not None:
+                o.patterns = patterns
+            o.rejected = rejected
+            left[0] -= 1
+
+        def release(st):
+            st.holders -= 1
+
+        def builder():
+            while True:
+                with cv:
+                    while not builds and not stop[0]:
+                        cv.wait()
+                    if not builds:
+                        return
+                    job = builds.popleft()
+                if job[0] == "free":
+                    job[1].close()
+                    continue
+                _, st, r = job
+                try:
+                    bits = _build(self.builder_ctx, r)
+                    self.builder_ctx.synchronize()
+                except Exception as error:  # reported on the request
+                    with cv:
+                        st.error = error
+                        waiting, st.waiting = st.waiting, []
+                        for w in waiting:
+                            release(st)
+                            finish(w, rejected=f"store build failed: {error}")
+                        self._stores.pop(st.key, None)
+                        self._used -= st.nbytes
+                        cv.notify_all()
+                    continue
+                with cv:
+                    st.bits, st.ready = bits, True
+                    for w in st.waiting:
+                        ready.append((w, st))
+                    st.waiting = []
+                    counters["builds"] += 1
+                    cv.notify_all()
+
+        def slot(i):
+            ctx = self.slot_ctx[i]
+            while True:
+                with cv:
+                    while not ready and not stop[0]:
+                        cv.wait()
+                    if not ready:
+                        return
+                    batch = [ready.popleft()]
+                    entries = batch[0][0].entries
+                    while ready and len(batch) < self.max_batch and \
+                            entries + ready[0][0].entries <= self.max_batch_entries:
+                        entries += ready[0][0].entries
+                        batch.append(ready.popleft())
+                    busy[0] += 1
+                    counters["busy_max"] = max(counters["busy_max"], busy[0])
+                start = now()
+                try:
+                    if len(batch) == 1:
+                        r, st = batch[0]
+                        results = [_hip.PatternArrays.of(ctx.eclat_mine(st.bits, r.minsup))]
+                    else:
+                        res = ctx.eclat_mine_batch([st.bits for _, st in batch], [r.minsup for r, _ in batch])
+                        results = res.split_arrays(len(batch))
+                except Exception as error:
+                    with cv:
+                        failure.append(error)
+                        stop[0] = True
+                        busy[0] -= 1
+                        cv.notify_all()
+                    return
+                end = now()
+                with cv:
+                    busy[0] -= 1
+                    k = len(self.batches)
+                    self.batches.append((i, start, end, [r.rid for r, _ in batch]))
+                    for (r, st), pats in zip(batch, results):
+                        finish(r, pats)
+                        out[r.rid].batch = k
+                        release(st)
+                    cv.notify_all()
+
+        def try_admit():
+            """Admit waiting requests that need a new store, in arrival
+            order, while the budget allows (evicting idle stores)."""
+            while admit:
+                r = admit[0]
+                key = r.store_key
+                st = self._stores.get(key)
+                if st is not None:   # another request admitted it meanwhile
+                    admit.popleft()
+                    join(r, st)
+                    continue
+                need = r.footprint()
+                if self._used + need > self.hbm_budget:
+                    for k in list(self._stores):
+                        if self._used + need <= self.hbm_budget:
+                            break
+                        old = self._stores[k]
+                        if old.holders == 0 and old.ready:
+                            del self._stores[k]
+                            self._used -= old.nbytes
+                            builds.append(("free", old.bits))
+                            counters["evictions"] += 1
+                    if self._used + need > self.hbm_budget:
+                        return   # wait for running requests to release stores
+                admit.popleft()
+                st = _Store(key, need)
+                self._stores[key] = st
+                self._used += need
+                st.holders = 1
+                st.waiting.append(r)
+                builds.append(("build", st, r))
+                cv.notify_all()
+
+        def join(r, st):
+            self._stores.move_to_end(st.key)
+            st.holders += 1
+            if st.ready:
+                counters["resident_hits"] += 1
+                ready.append((r, st))
+            else:
+                st.waiting.append(r)
+            cv.notify_all()
+
+        threads = [threading.Thread(target=builder, daemon=True)]
+        threads += [threading.Thread(target=slot, args=(i,), daemon=True) for i in range(self.slots)]
+        for t in threads:
+            t.start()
+        try:
+            with cv:
+                while left[0] > 0 and not failure:
+                    t = now()
+                    while pending and pending[0].arrival <= t:
+                        r = pending.popleft()
+                        if r.footprint() > self.hbm_budget:
+                            finish(r, rejected=f"footprint {r.footprint()} B exceeds the budget "
+                                               f"{self.hbm_budget} B")
+                            continue
+                        st = self._stores.get(r.store_key)
+                        if st is not None:   # resident or being built: needs no new memory
+                            join(r, st)
+                        else:
+                            admit.append(r)
+                    try_admit()
+                    if left[0] == 0:
+                        break
+                    timeout = (pending[0].arrival - now()) if pending else None
+                    if timeout is None or timeout > 0:
+                        cv.wait(timeout=min(timeout, 0.05) if timeout is not None else 0.05)
+        finally:
+            with cv:
+                stop[0] = True
+                cv.notify_all()
+            for t in threads:
+                t.join()
+            # the stores of this run stay resident, built and idle
+            for key in [k for k, st in self._stores.items() if not st.ready]:
+                del self._stores[key]
+        if failure:
+            raise failure[0]
+        self.stats = dict(counters, resident_bytes=self._used, resident_stores=len(self._stores),
+                          batches=len(self.batches),
+                          max_batch=max((len(b[3]) for b in self.batches), default=0))
         return [out[rid] for rid in sorted(out)]
 
 
 def mine_one_at_a_time(requests, ctx=None):
-    """The same arrival stream, one request at a time in arrival order."""
+    """The same arrival stream the plain way: one request at a time in
+    arrival order, its store built, mined and freed (what a new miner object
+    per request does)."""
     ctx = ctx or _hip.context(0)
     order = sorted(requests, key=lambda r: (r.arrival, r.rid))
     out = {}
@@ -191,6 +380,31 @@ def mine_one_at_a_time(requests, ctx=None):
         o.finish = time.perf_counter() - t0
         o.patterns = pats
         out[r.rid] = o
+    return [out[rid] for rid in sorted(out)]
+
+
+def mine_sequential_resident(requests, ctx=None):
+    """One request at a time in arrival order on stores kept resident
+    (built on first use, never evicted): the residency without the overlap."""
+    ctx = ctx or _hip.context(0)
+    order = sorted(requests, key=lambda r: (r.arrival, r.rid))
+    stores, out = {}, {}
+    t0 = time.perf_counter()
+    try:
+        for r in order:
+            now = time.perf_counter() - t0
+            if r.arrival > now:
+                time.sleep(r.arrival - now)
+            bits = stores.get(r.store_key)
+            if bits is None:
+                bits = stores[r.store_key] = _build(ctx, r)
+            o = Outcome(r.rid, r.arrival)
+            o.patterns = _hip.PatternArrays.of(ctx.eclat_mine(bits, r.minsup))
+            o.finish = time.perf_counter() - t0
+            out[r.rid] = o
+    finally:
+        for b in stores.values():
+            b.close()
     return [out[rid] for rid in sorted(out)]
 
 

@@ -6,7 +6,10 @@
  * for the last failure.  Device memory belongs to the handle that allocated
  * it and is released by the matching *_destroy.  Host output buffers belong
  * to the caller.  A context is bound to one device and one HIP stream; use a
- * context from one host thread at a time.
+ * context from one host thread at a time.  Several contexts of one device
+ * may mine the same store concurrently once its build has completed on its
+ * own context (cpm_ctx_synchronize of that context); a store is destroyed
+ * from the thread of the context that built it, when no search uses it.
  *
  * Items are addressed by COLUMN: the index of the item's label in string
  * order (the reference sorts labels as strings, cuApriori.py:397).  Rows are
