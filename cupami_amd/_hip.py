@@ -20,7 +20,7 @@ import numpy as np
 _HERE = Path(__file__).resolve().parent
 LIB_PATH = Path(os.environ.get("CUPAMI_HIP_LIB", _HERE / "libcupami_hip.so"))
 
-OK, ERR_INVALID, ERR_RUNTIME, ERR_NOMEM, ERR_NODEVICE = 0, -1, -2, -3, -4
+OK, ERR_INVALID, ERR_RUNTIME, ERR_NOMEM, ERR_NODEVICE, ERR_LAYOUT = 0, -1, -2, -3, -4, -5
 NO_PARENT = 0xFFFFFFFF
 
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -51,6 +51,12 @@ SIGNATURES = {
     "cpm_bits_build_csr_device": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64,
                                                  ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(_vp)]),
     "cpm_bits_labels": (ctypes.c_int, [_vp, _vp, _u32p]),
+    "cpm_baskets_parse_text": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char,
+                                              ctypes.POINTER(_vp)]),
+    "cpm_baskets_info": (ctypes.c_int, [_vp, _u64p, _u64p, _u64p, _u32p]),
+    "cpm_baskets_download": (ctypes.c_int, [_vp, _vp, _u64p, _u32p]),
+    "cpm_bits_build_baskets": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "cpm_baskets_destroy": (ctypes.c_int, [_vp]),
     "cpm_ingest_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "cpm_patterns_keys": (ctypes.c_int, [_vp, ctypes.c_char_p, _u64p, ctypes.c_uint32, ctypes.c_char, _vp,
                                          ctypes.c_uint64, _u64p]),
@@ -247,6 +253,22 @@ class Context:
                                                ctypes.byref(handle)))
         bits = Bits(self, handle)
         return bits, bits.labels()
+
+    def parse_text(self, text, sep):
+        """Delimited text of decimal ids parsed on the device
+        (``cpm_baskets_parse_text``): ``Baskets``, or None when the text is
+        for the host reader (quotes, non-ASCII, fields that are not
+        canonical decimal ids)."""
+        data = bytes(text)
+        sep = sep.encode("latin-1") if isinstance(sep, str) else bytes(sep)
+        if len(sep) != 1:
+            return None
+        handle = _vp()
+        status = load().cpm_baskets_parse_text(self.handle, data, len(data), sep, ctypes.byref(handle))
+        if status == ERR_LAYOUT:
+            return None
+        check(status)
+        return Baskets(self, handle)
 
     def build_bits_device_batch(self, specs):
         """Several stores in one call: ``specs`` is a sequence of
@@ -506,6 +528,45 @@ class Bits:
         if out.size:
             check(load().cpm_bits_download(self.ctx.handle, self.handle, _p(out, _u64p)))
         return out
+
+
+class Baskets:
+    """Basket arrays parsed on the device (``cpm_baskets``)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+        nb, ni, ne = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        mx = ctypes.c_uint32()
+        check(load().cpm_baskets_info(handle, ctypes.byref(nb), ctypes.byref(ni), ctypes.byref(ne),
+                                      ctypes.byref(mx)))
+        self.n_baskets, self.n_ids, self.n_nonempty, self.max_id = nb.value, ni.value, ne.value, mx.value
+
+    def close(self):
+        if self.handle:
+            load().cpm_baskets_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def arrays(self):
+        """``(offsets u64[n_baskets + 1], ids u32[n_ids])`` on the host."""
+        off = np.zeros(self.n_baskets + 1, dtype=np.uint64)
+        ids = np.zeros(max(self.n_ids, 1), dtype=np.uint32)
+        check(load().cpm_baskets_download(self.ctx.handle, self.handle, _p(off, _u64p), _p(ids, _u32p)))
+        return off, ids[: self.n_ids]
+
+    def build_bits(self, keep_empty=False):
+        """The store and each column's id (``cpm_bits_build_baskets``)."""
+        handle = _vp()
+        check(load().cpm_bits_build_baskets(self.ctx.handle, self.handle, int(bool(keep_empty)),
+                                            ctypes.byref(handle)))
+        bits = Bits(self.ctx, handle)
+        return bits, bits.labels()
 
 
 class _PatternsHandle:

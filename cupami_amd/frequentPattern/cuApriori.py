@@ -17,17 +17,12 @@ is the popcount of the AND of its item rows (complete intersection,
 
 from __future__ import annotations
 
-import csv
-import io
 import math
-import struct
 import sys
 import time
-from itertools import chain, combinations
+from itertools import combinations
 from pathlib import Path
 from typing import Dict, Tuple
-from urllib.parse import urlparse
-from urllib.request import urlopen
 
 import numpy as np
 
@@ -36,23 +31,20 @@ if __name__ == "__main__" and not __package__:
     __package__ = "cupami_amd.frequentPattern"
 
 from .. import _hip
+from . import _inputs
 
 
 def binary_pairs(offsets, ids, keep_empty=False):
-    """Host (labels, rows, cols) of a BIN file's arrays (``cuApriori.py:381-396``):
-    distinct ids as string-sorted labels, each entry's row among the non-empty
-    baskets (every basket with ``keep_empty``) and its column.  The miners
-    build the store on the device instead (``cpm_bits_build_csr``); this
-    serves ``_getDevicePairs`` callers."""
-    unique = np.unique(ids)
-    items = sorted(str(int(v)) for v in unique)
-    index = {label: i for i, label in enumerate(items)}
-    remap = np.array([index[str(int(v))] for v in unique], dtype=np.uint32)
-    lengths = np.diff(offsets).astype(np.int64)
-    active = np.arange(lengths.size) if keep_empty else np.cumsum(lengths > 0) - 1
-    rows = np.repeat(active.astype(np.uint64), lengths)
-    cols = remap[np.searchsorted(unique, ids)] if ids.size else np.zeros(0, np.uint32)
-    return items, rows, cols.astype(np.uint32)
+    """Host ``(labels, rows, cols)`` of a BIN file's arrays
+    (``cuApriori.py:381-396``); see ``_inputs.basket_array_pairs``."""
+    return _inputs.basket_array_pairs(offsets, ids, keep_empty)
+
+
+_BACKENDS = {
+    "cpu": "Text or Parquet input",
+    "numpy-hip": "Bulk numeric reads; the store is built on the device from the arrays",
+    "hip-text": "Text parsed on the device into basket arrays; the store is built there",
+}
 
 
 class cuApriori:
@@ -61,159 +53,125 @@ class cuApriori:
     Parameters follow the reference: ``iFile`` is an iterable of baskets, a
     CSV/TSV path or http(s) URL, a ``CUPAMI01`` BIN file or a Parquet file with
     an ``items`` list<uint32> column; ``minSup`` is absolute (int, or float
-    > 1) or a fraction of the transactions (float in (0, 1]).
+    > 1) or a fraction of the transactions (float in (0, 1]).  ``use_gpu_io``
+    chooses where input is parsed: ``"auto"`` reads BIN and Parquet arrays in
+    bulk and parses text of decimal ids on the device (other text on the
+    host), ``True`` requires that, ``False`` parses everything on the host.
     """
 
     def __init__(self, iFile, minSup, sep=None, use_gds="auto", use_gpu_io="auto"):
-        loading_start = time.perf_counter()
-        if use_gds not in ("auto", True, False):
-            raise ValueError('use_gds must be "auto", True, or False')
-        if use_gpu_io not in ("auto", True, False):
-            raise ValueError('use_gpu_io must be "auto", True, or False')
-        self.iFile = iFile
-        self.minSup = minSup
-        self.sep = sep
-        self.use_gds = use_gds
-        self.use_gpu_io = use_gpu_io
+        started = time.perf_counter()
+        for name, value in (("use_gds", use_gds), ("use_gpu_io", use_gpu_io)):
+            if value not in ("auto", True, False):
+                raise ValueError(f'{name} must be "auto", True, or False')
+        self.iFile, self.minSup, self.sep = iFile, minSup, sep
+        self.use_gds, self.use_gpu_io = use_gds, use_gpu_io
+        # the input, in one of its forms: label rows (_Database), basket
+        # arrays on the host (_binaryArrays) or on the device (_deviceBaskets)
         self._binaryArrays = None
         self._binaryUniverse = 0
+        self._deviceBaskets = None
         self._transactionCount = 0
-        self._inputBackend = {"backend": "cpu", "reason": "Text or Parquet input"}
-        if use_gds is True and (not isinstance(iFile, (str, Path))
-                                or Path(iFile).suffix.lower() != ".bin"
-                                or urlparse(str(iFile)).scheme in ("http", "https")):
+        self._setBackend("cpu")
+        if use_gds is True and _inputs.source_kind(iFile) != "bin":
             raise ValueError("Required GDS input currently supports local BIN files only")
         self._Database = self._readTransactions(iFile)
         if self._Database is not None:
             self._transactionCount = len(self._Database)
         self._minSup = self._getSupportCount(minSup, self._transactionCount)
         self._finalPatterns: Dict[Tuple[str, ...], int] = {}
-        self._startTime = 0.0
-        self._endTime = 0.0
-        self._loadingTime = time.perf_counter() - loading_start
+        self._startTime = self._endTime = 0.0
+        self._loadingTime = time.perf_counter() - started
 
-    # ---- readers (cuApriori.py:202-450) ----------------------------------------
-    def _readTransactions(self, source):
-        if isinstance(source, str) and urlparse(source).scheme in ("http", "https"):
-            with urlopen(source, timeout=30) as response:
-                stream = io.TextIOWrapper(response, encoding="utf-8", newline="")
-                return self._readDelimitedFile(stream)
-        if isinstance(source, (str, Path)):
-            suffix = Path(source).suffix.lower()
-            if suffix == ".bin":
-                return self._readBinaryFile(Path(source))
-            if suffix == ".parquet":
-                self._requireCpuText()
-                return self._readParquetFile(source)
-            self._requireCpuText()
-            with Path(source).open("r", newline="", encoding="utf-8") as handle:
-                return self._readDelimitedFile(handle)
-        return self._cleanTransactions(source)
-
-    def _requireCpuText(self):
-        # Device-side text/Parquet parsing is not part of this build; the
-        # reference raises the same way when GPU input is required but
-        # unavailable (cuApriori.py:243-245).
-        if self.use_gpu_io is True:
-            raise RuntimeError("GPU input unavailable: text and Parquet inputs are parsed on "
-                               "the host")
-        self._inputBackend = {"backend": "cpu", "reason": "Text or Parquet input"}
-
-    def _readBinaryFile(self, path):
-        """The generator's ``CUPAMI01`` format (``cuApriori.py:256-323``)."""
-        with path.open("rb") as source:
-            header = source.read(32)
-            if len(header) != 32:
-                raise ValueError("Invalid BIN file: truncated header")
-            magic, count, total, universe = struct.unpack("<8sQQQ", header)
-            if magic != b"CUPAMI01":
-                raise ValueError("Invalid BIN file: expected CUPAMI01 format")
-            if not 1 <= universe <= 0xFFFFFFFF:
-                raise ValueError("Invalid BIN file: item universe must fit in uint32")
-            if path.stat().st_size != 32 + 8 * (count + 1) + 4 * total:
-                raise ValueError("Invalid BIN file: size does not match header")
-            if self.use_gds is True:
-                raise RuntimeError("GDS input requested but unavailable: no GPUDirect Storage "
-                                   "path on this platform")
-            offsets = np.fromfile(source, dtype="<u8", count=count + 1)
-            ids = np.fromfile(source, dtype="<u4", count=total)
-        if offsets.size != count + 1 or ids.size != total:
-            raise ValueError("Invalid BIN file: truncated arrays")
-        if offsets[0] != 0 or offsets[-1] != total or np.any(offsets[1:] < offsets[:-1]):
-            raise ValueError("Invalid BIN file: invalid transaction offsets")
-        if np.any(ids > universe):
-            raise ValueError("Invalid BIN file: item ID outside configured universe")
-        if self.use_gpu_io is False:
-            transactions = []
-            for start, end in zip(offsets[:-1].tolist(), offsets[1:].tolist()):
-                basket = self._cleanTransaction(ids[start:end].tolist())
-                if basket:
-                    transactions.append(basket)
-            return transactions
-        # Bulk numeric path: typed arrays go straight to the COO builder.
-        self._transactionCount = int(np.count_nonzero(offsets[1:] > offsets[:-1]))
-        self._binaryArrays = (offsets.astype(np.uint64), ids.astype(np.uint32))
-        self._binaryUniverse = int(universe)
-        self._inputBackend = {"backend": "numpy-hip",
-                              "reason": "Bulk numeric BIN reads; the store is built on the device "
-                                        "from the file's arrays"}
-        return None
+    def _setBackend(self, name, reason=None):
+        self._inputBackend = {"backend": name, "reason": reason or _BACKENDS[name]}
 
     def getInputBackend(self):
         return dict(self._inputBackend)
 
+    # ---- readers (cuApriori.py:202-450) ----------------------------------------
+    def _readTransactions(self, source):
+        """Label rows, or None when the input is kept as basket arrays."""
+        kind = _inputs.source_kind(source)
+        if kind == "rows":
+            return self._cleanTransactions(source)
+        if kind == "bin":
+            return self._readBinaryFile(Path(source))
+        if kind == "parquet":
+            return self._readParquetFile(source)
+        return self._readTextSource(source)
+
+    def _readTextSource(self, source):
+        data = _inputs.fetch_bytes(source)
+        if self.use_gpu_io is not False:
+            newline = data.find(b"\n")
+            first = (data if newline < 0 else data[:newline]).decode("utf-8", "replace")
+            sep = _inputs.separator_for(first, self.sep)
+            try:
+                ctx = self._getCudaModule()
+            except (ImportError, RuntimeError) as error:
+                if self.use_gpu_io is True:
+                    raise RuntimeError(f"GPU input unavailable: {error}") from error
+                self._setBackend("cpu", str(error))
+                return self._readDelimitedFile(data.decode("utf-8"))
+            baskets = ctx.parse_text(data, sep) if len(sep) == 1 else None
+            if baskets is not None:
+                self._deviceBaskets = baskets
+                self._transactionCount = int(baskets.n_nonempty)
+                self._setBackend("hip-text")
+                return None
+            if self.use_gpu_io is True:
+                raise RuntimeError("GPU input unavailable: the device reader takes delimited decimal item ids "
+                                   "(no quotes, ASCII only); this text needs the host reader")
+            self._setBackend("cpu", "Text that the device reader does not take (labels, quotes or non-ASCII)")
+        return self._readDelimitedFile(data.decode("utf-8"))
+
+    def _readBinaryFile(self, path):
+        """The generator's ``CUPAMI01`` format (``cuApriori.py:256-323``)."""
+        _inputs.bin_header(path, _inputs.CUPAMI01)
+        if self.use_gds is True:
+            raise RuntimeError("GDS input requested but unavailable: no GPUDirect Storage "
+                               "path on this platform")
+        _, offsets, ids, universe = _inputs.bin_arrays(path, _inputs.CUPAMI01)
+        if self.use_gpu_io is False:
+            return _inputs.arrays_to_label_rows(offsets, ids)
+        self._keepArrays(offsets, ids, universe)
+        return None
+
     def _readParquetFile(self, path):
-        try:
-            import pyarrow as pa
-            import pyarrow.parquet as pq
-        except ImportError as error:
-            raise ImportError("Parquet input requires pyarrow: pip install pyarrow") from error
-        parquet = pq.ParquetFile(path)
-        schema = parquet.schema_arrow
-        if "items" not in schema.names:
-            raise ValueError("Invalid Parquet file: missing items column")
-        item_type = schema.field("items").type
-        if not pa.types.is_list(item_type) or not pa.types.is_uint32(item_type.value_type):
-            raise ValueError("Invalid Parquet file: items must be a list of uint32 IDs")
-        transactions = []
-        for batch in parquet.iter_batches(columns=["items"]):
-            rows = batch.column(0).to_pylist()
-            if any(row is None or any(item is None for item in row) for row in rows):
-                raise ValueError("Invalid Parquet file: null transactions or item IDs")
-            transactions.extend(self._cleanTransactions(rows))
-        return transactions
+        """A Parquet file's ``items`` column (``cuApriori.py:406-426``), read as
+        basket arrays (label rows with ``use_gpu_io=False``)."""
+        offsets, ids, universe = _inputs.parquet_basket_arrays(path)
+        if self.use_gpu_io is False:
+            return _inputs.arrays_to_label_rows(offsets, ids)
+        self._keepArrays(offsets, ids, universe)
+        return None
+
+    def _keepArrays(self, offsets, ids, universe):
+        self._binaryArrays = (offsets, ids)
+        self._binaryUniverse = int(universe)
+        self._transactionCount = int(np.count_nonzero(np.diff(offsets.astype(np.int64))))
+        self._setBackend("numpy-hip")
 
     def _readDelimitedFile(self, stream):
-        first = stream.readline()
-        delimiter = self.sep
-        if delimiter is None:
-            delimiter = "\t" if "\t" in first else ","
-        return self._cleanTransactions(csv.reader(chain((first,), stream), delimiter=delimiter))
+        """Label rows of delimited text (a str or a text stream)."""
+        text = stream if isinstance(stream, str) else stream.read()
+        return _inputs.delimited_label_rows(text, self.sep)
 
     def _cleanTransactions(self, rows):
-        out = []
-        for row in rows:
-            basket = self._cleanTransaction(row)
-            if basket:
-                out.append(basket)
-        return out
+        return _inputs.kept_label_rows(rows)
 
     @staticmethod
     def _cleanTransaction(transaction):
         """``str().strip()``, drop empty labels and repeats (``cuApriori.py:444-450``)."""
-        items = []
-        for value in transaction:
-            item = str(value).strip()
-            if item and item not in items:
-                items.append(item)
-        return tuple(items)
+        return _inputs.clean_labels(transaction)
 
     @staticmethod
     def _getSupportCount(minSup, transactionCount):
-        """``cuApriori.py:452-456``: fractions round up, other values truncate."""
-        if isinstance(minSup, float) and 0 < minSup <= 1:
-            return math.ceil(minSup * transactionCount)
-        return int(minSup)
+        """``cuApriori.py:452-456``: a fraction in (0, 1] is rounded up, any
+        other value is truncated."""
+        fractional = isinstance(minSup, float) and 0.0 < minSup <= 1.0
+        return math.ceil(minSup * transactionCount) if fractional else int(minSup)
 
     @staticmethod
     def _getCudaModule():
@@ -230,36 +188,34 @@ class cuApriori:
     # ---- COO (cuApriori.py:378-404) -----------------------------------------------
     def _getDevicePairs(self, ctx=None):
         """Labels sorted as strings, and the (row, column) pairs of the store."""
+        if self._deviceBaskets is not None:
+            return binary_pairs(*self._deviceBaskets.arrays())
         if self._binaryArrays is not None:
             return binary_pairs(*self._binaryArrays)
-        items = sorted({item for row in self._Database for item in row})
-        index = {item: i for i, item in enumerate(items)}
-        lengths = [len(row) for row in self._Database]
-        rows = np.repeat(np.arange(len(self._Database), dtype=np.uint64),
-                         np.asarray(lengths, dtype=np.int64))
-        cols = np.fromiter((index[item] for row in self._Database for item in row),
-                           dtype=np.uint32, count=int(sum(lengths)))
-        return items, rows, cols
+        labels = sorted(set().union(*self._Database)) if self._Database else []
+        column = {label: i for i, label in enumerate(labels)}
+        sizes = np.fromiter(map(len, self._Database), dtype=np.int64, count=len(self._Database))
+        rows = np.repeat(np.arange(sizes.size, dtype=np.uint64), sizes)
+        cols = np.fromiter((column[label] for basket in self._Database for label in basket),
+                           dtype=np.uint32, count=int(sizes.sum()))
+        return labels, rows, cols
 
     def _getDeviceStore(self, ctx):
         """Labels and the device store (``None`` when there are no items).
-
-        A BIN file's arrays go to the device as stored and the store is
-        built there (``cpm_bits_build_csr``): the distinct ids, their string
-        order, and each entry's row and column never exist on the host.
-        Other inputs build it from the host pairs of ``_getDevicePairs``."""
-        if self._binaryArrays is not None:
-            offsets, ids = self._binaryArrays
-            bits, labelIds = ctx.build_bits_csr(offsets, ids, self._binaryUniverse)
-            items = [str(v) for v in labelIds.tolist()]
-            if not items:
-                bits.close()
-                return items, None
-            return items, bits
-        items, rows, columns = self._getDevicePairs(ctx)
-        if not items:
-            return items, None
-        return items, ctx.build_bits(rows, columns, self._transactionCount, len(items))
+        Basket arrays are built on the device as they are
+        (``cpm_bits_build_csr``), label rows from their host pairs."""
+        if self._deviceBaskets is not None:
+            bits, ids = self._deviceBaskets.build_bits()
+        elif self._binaryArrays is not None:
+            bits, ids = ctx.build_bits_csr(*self._binaryArrays, self._binaryUniverse)
+        else:
+            labels, rows, cols = self._getDevicePairs(ctx)
+            return labels, (ctx.build_bits(rows, cols, self._transactionCount, len(labels)) if labels else None)
+        labels = [str(v) for v in ids.tolist()]
+        if not labels:
+            bits.close()
+            bits = None
+        return labels, bits
 
     # ---- reference hooks, kept for subclasses and operator-level use --------------
     def _generateCandidates(self, previousLevel):
@@ -371,29 +327,35 @@ class cuApriori:
     def getTotalRuntime(self):
         return self.getLoadingTime() + self.getMiningTime()
 
-    def getMemoryRSS(self):
+    @staticmethod
+    def _memory(field):
         try:
             import psutil
         except ImportError:
             return 0
-        return int(psutil.Process().memory_info().rss)
+        process = psutil.Process()
+        info = process.memory_full_info() if field == "uss" else process.memory_info()
+        return int(getattr(info, field))
+
+    def getMemoryRSS(self):
+        return self._memory("rss")
 
     def getMemoryUSS(self):
-        try:
-            import psutil
-        except ImportError:
-            return 0
-        return int(psutil.Process().memory_full_info().uss)
+        return self._memory("uss")
+
+    def _patternCount(self):
+        pending = self._pending()
+        return len(pending[0]) if pending is not None else len(self._finalPatterns)
 
     def printResults(self):
-        pending = self._pending()
-        print("Total number of Frequent Patterns:",
-              len(pending[0]) if pending is not None else len(self._finalPatterns))
-        print("Total Memory in USS:", self.getMemoryUSS())
-        print("Total Memory in RSS:", self.getMemoryRSS())
-        print("Loading time (seconds):", self.getLoadingTime())
-        print("Mining time (seconds):", self.getMiningTime())
-        print("Total time (seconds):", self.getTotalRuntime())
+        lines = (("Total number of Frequent Patterns:", self._patternCount()),
+                 ("Total Memory in USS:", self.getMemoryUSS()),
+                 ("Total Memory in RSS:", self.getMemoryRSS()),
+                 ("Loading time (seconds):", self.getLoadingTime()),
+                 ("Mining time (seconds):", self.getMiningTime()),
+                 ("Total time (seconds):", self.getTotalRuntime()))
+        for label, value in lines:
+            print(label, value)
 
 
 Apriori = cuApriori

@@ -15,95 +15,65 @@ import csv
 import io
 import math
 import numbers
-import struct
 import time
 from pathlib import Path
-from urllib.parse import urlparse
-from urllib.request import urlopen
 
 import numpy as np
 
+from ..frequentPattern import _inputs
 from ..frequentPattern.cuApriori import binary_pairs, cuApriori
 
 _INT64_MAX = np.iinfo(np.int64).max
+_DIGITS = frozenset("0123456789")
 
 
 def read_temporal_arrays(path):
-    """The arrays of a ``CUPAMIT1`` file (``_temporal_io.py:48-72``), checked:
-    ``(timestamps i64[n], offsets u64[n+1], items u32[m], universe)``."""
-    header = struct.Struct("<8sQQQ")
-    with Path(path).open("rb") as source:
-        raw = source.read(header.size)
-        if len(raw) != header.size:
-            raise ValueError("Invalid temporal BIN header")
-        magic, count, total, universe = header.unpack(raw)
-        if magic != b"CUPAMIT1":
-            raise ValueError("Expected a temporal BIN file with timestamp data")
-        if not 1 <= universe <= 0xFFFFFFFF:
-            raise ValueError("Invalid temporal BIN item range")
-        if Path(path).stat().st_size != header.size + 8 * count + 8 * (count + 1) + 4 * total:
-            raise ValueError("Invalid temporal BIN file size")
-        ts = np.fromfile(source, dtype="<i8", count=count)
-        offsets = np.fromfile(source, dtype="<u8", count=count + 1)
-        items = np.fromfile(source, dtype="<u4", count=total)
-    if len(ts) != count or len(offsets) != count + 1 or len(items) != total:
-        raise ValueError("Truncated temporal BIN arrays")
-    if np.any(ts < 0):
-        raise ValueError("Temporal timestamps must be nonnegative")
-    if offsets[0] != 0 or offsets[-1] != total or np.any(offsets[1:] < offsets[:-1]):
-        raise ValueError("Invalid temporal BIN offsets")
-    if np.any(items > universe):
-        raise ValueError("Temporal BIN item ID exceeds header range")
-    return ts, offsets, items, int(universe)
+    """The arrays of a ``CUPAMIT1`` file (``extras/_temporal_io.py:48-72``),
+    checked: ``(timestamps i64[n], offsets u64[n+1], items u32[m], universe)``."""
+    return _inputs.bin_arrays(path, _inputs.CUPAMIT1)
 
 
 def read_temporal_binary(path):
-    """Rows ``[ts, *items]`` of a ``CUPAMIT1`` file (``_temporal_io.py:48-72``)."""
+    """Rows ``[ts, *items]`` of a ``CUPAMIT1`` file (``extras/_temporal_io.py:48-72``)."""
     ts, offsets, items, _ = read_temporal_arrays(path)
-    return [[int(t), *items[int(offsets[i]):int(offsets[i + 1])].tolist()]
-            for i, t in enumerate(ts.tolist())]
+    bounds = offsets.tolist()
+    return [[t, *items[bounds[i]:bounds[i + 1]].tolist()] for i, t in enumerate(ts.tolist())]
 
 
-def read_temporal_parquet(path):
-    """Rows of a temporal Parquet file (``timestamp`` int64, ``items`` list<uint32>)."""
-    try:
-        import pyarrow.parquet as pq
-    except ImportError as error:
-        raise ImportError("Temporal Parquet requires pyarrow: pip install pyarrow") from error
-    table = pq.read_table(path)
-    if "timestamp" not in table.column_names or "items" not in table.column_names:
-        raise ValueError("Temporal Parquet needs timestamp and items columns")
-    ts = table.column("timestamp").to_pylist()
-    items = table.column("items").to_pylist()
-    return [[t, *(row or [])] for t, row in zip(ts, items)]
+def _positive_real(value):
+    return (not isinstance(value, bool) and isinstance(value, numbers.Real) and math.isfinite(value))
 
 
 class TemporalInput(cuApriori):
-    """Timestamp-first CSV, temporal BIN/Parquet, or iterable input."""
+    """Timestamp-first CSV, temporal BIN/Parquet, or iterable input.
+
+    Rows keep their timestamps, equal ones included, and are ordered by
+    timestamp (stably); a row with a timestamp and no item is a basket of its
+    own and counts in N; the timeline starts at 0 and ends at the last
+    timestamp or at ``endTime`` (``_temporal.py:65-139``)."""
 
     def __init__(self, iFile, minSup, maxPer, sep=None, *, endTime=None):
-        loading_start = time.perf_counter()
+        started = time.perf_counter()
         self._timestamps = []
-        self.endTime = endTime
-        self.maxPer = maxPer
+        self.endTime, self.maxPer = endTime, maxPer
         super().__init__(iFile, minSup, sep, use_gds=False, use_gpu_io=False)
-        if (isinstance(minSup, bool) or not isinstance(minSup, numbers.Real)
-                or not math.isfinite(minSup) or minSup <= 0):
+        if not _positive_real(minSup) or minSup <= 0:
             raise ValueError("minSup must be a positive finite number")
-        if (isinstance(maxPer, bool) or not isinstance(maxPer, numbers.Real)
-                or not math.isfinite(maxPer) or maxPer < 0):
+        if not _positive_real(maxPer) or maxPer < 0:
             raise ValueError("maxPer must be a nonnegative finite number")
-        self._maxPer = (maxPer * self._endTimestamp
-                        if isinstance(maxPer, float) and 0 < maxPer <= 1 else maxPer)
+        # a fraction in (0, 1] is a share of the timeline, not of the rows
+        fraction = isinstance(maxPer, float) and 0.0 < maxPer <= 1.0
+        self._maxPer = maxPer * self._endTimestamp if fraction else maxPer
         self._periods = {}
-        self._loadingTime = time.perf_counter() - loading_start
+        self._loadingTime = time.perf_counter() - started
 
     @staticmethod
     def _timestamp(value):
+        """A digits-only non-negative int64 (``_temporal.py:89-98``)."""
         if isinstance(value, (bool, float)):
             raise ValueError("timestamps must be nonnegative int64 integers")
         text = str(value).strip()
-        if not text or any(c not in "0123456789" for c in text):
+        if not text or not _DIGITS.issuperset(text):
             raise ValueError("timestamps must be nonnegative int64 integers")
         stamp = int(text)
         if stamp > _INT64_MAX:
@@ -111,70 +81,60 @@ class TemporalInput(cuApriori):
         return stamp
 
     def _readTransactions(self, source):
-        if isinstance(source, (str, Path)):
-            suffix = Path(str(source)).suffix.lower()
-            is_url = urlparse(str(source)).scheme in ("http", "https")
-            if suffix in (".bin", ".parquet"):
-                if is_url:
-                    raise ValueError("Temporal BIN and Parquet input requires a local file")
-                if suffix == ".bin":
-                    return self._readTemporalArrays(source)
-                return self._cleanTemporalRows(read_temporal_parquet(source))
-            if is_url:
-                with urlopen(str(source), timeout=30) as response:
-                    text = response.read().decode("utf-8-sig")
-            else:
-                text = Path(source).read_text(encoding="utf-8-sig")
-            separator = self.sep
-            if separator is None:
-                first = next((line for line in text.splitlines() if line.strip()), "")
-                separator = "\t" if "\t" in first else "," if "," in first else None
-            rows = (csv.reader(io.StringIO(text), delimiter=separator) if separator
-                    else (line.split() for line in text.splitlines()))
-        else:
-            rows = source
-        return self._cleanTemporalRows(rows)
+        kind = _inputs.source_kind(source)
+        if kind == "rows":
+            return self._cleanTemporalRows(source)
+        if isinstance(source, (str, Path)) and Path(str(source)).suffix.lower() in (".bin", ".parquet"):
+            if kind == "url":
+                raise ValueError("Temporal BIN and Parquet input requires a local file")
+            arrays = (read_temporal_arrays(source) if kind == "bin"
+                      else _inputs.temporal_parquet_arrays(source))
+            return self._keepTemporalArrays(*arrays)
+        # a local file with universal newlines, a URL's body as sent
+        text = (_inputs.fetch_bytes(source).decode("utf-8-sig") if kind == "url"
+                else Path(source).read_text(encoding="utf-8-sig"))
+        return self._cleanTemporalRows(self._delimitedRows(text))
+
+    def _delimitedRows(self, text):
+        """Rows of temporal text: the separator given, else a tab or a comma
+        found on the first non-blank line, else runs of whitespace."""
+        sep = self.sep
+        if sep is None:
+            first = next(filter(str.strip, text.splitlines()), "")
+            sep = next((c for c in ("\t", ",") if c in first), None)
+        if sep is None:
+            return [line.split() for line in text.splitlines()]
+        return csv.reader(io.StringIO(text), delimiter=sep)
+
+    def _setTimeline(self, last):
+        end = last if self.endTime is None else self._timestamp(self.endTime)
+        if end < last:
+            raise ValueError("endTime must be at least the last timestamp")
+        self._endTimestamp = end
 
     def _cleanTemporalRows(self, rows):
+        """``(timestamp, labels)`` of every row that is not blank, ordered by
+        timestamp (stably); the label rows are returned, the timestamps kept."""
         records = []
-        for row in rows:
-            row = list(row)
-            if not row or not any(str(value).strip() for value in row):
-                continue
-            records.append((self._timestamp(row[0]), self._cleanTransaction(row[1:])))
-        records.sort(key=lambda record: record[0])
-        self._timestamps = [record[0] for record in records]
-        last = self._timestamps[-1] if records else 0
-        self._endTimestamp = last if self.endTime is None else self._timestamp(self.endTime)
-        if self._endTimestamp < last:
-            raise ValueError("endTime must be at least the last timestamp")
-        self._inputBackend = {"backend": "cpu", "reason": "Temporal reader; mining uses HIP"}
-        return [record[1] for record in records]
+        for row in map(list, rows):
+            if any(str(v).strip() for v in row):
+                records.append((self._timestamp(row[0]), _inputs.clean_labels(row[1:])))
+        records.sort(key=lambda rec: rec[0])
+        self._timestamps = [stamp for stamp, _ in records]
+        self._setTimeline(self._timestamps[-1] if records else 0)
+        self._setBackend("cpu", "Temporal reader; mining uses HIP")
+        return [labels for _, labels in records]
 
-    def _readTemporalArrays(self, source):
-        """A ``CUPAMIT1`` file's arrays, kept for the device build: every row
-        stays (timestamp-only rows too) and rows are ordered by timestamp,
-        stably, as ``_cleanTemporalRows`` orders them (``_temporal.py:125-139``)."""
-        ts, offsets, ids, universe = read_temporal_arrays(source)
-        if ts.size > 1 and np.any(ts[1:] < ts[:-1]):
-            order = np.argsort(ts, kind="stable")
-            lengths = np.diff(offsets.astype(np.int64))[order]
-            starts = offsets[:-1].astype(np.int64)[order]
-            sorted_off = np.zeros(ts.size + 1, dtype=np.uint64)
-            sorted_off[1:] = np.cumsum(lengths)
-            base = np.repeat(starts - sorted_off[:-1].astype(np.int64), lengths)
-            ids = ids[base + np.arange(ids.size, dtype=np.int64)]
-            offsets, ts = sorted_off, ts[order]
+    def _keepTemporalArrays(self, ts, offsets, ids, universe):
+        """Temporal basket arrays kept for the device build: every row stays
+        (timestamp-only rows too), rows ordered by timestamp (stably)."""
+        ts, offsets, ids = _inputs.stable_by_timestamp(ts, offsets, ids)
         self._timestamps = ts.astype(np.int64)
-        last = int(ts[-1]) if ts.size else 0
-        self._endTimestamp = last if self.endTime is None else self._timestamp(self.endTime)
-        if self._endTimestamp < last:
-            raise ValueError("endTime must be at least the last timestamp")
+        self._setTimeline(int(ts[-1]) if ts.size else 0)
         self._transactionCount = int(ts.size)
         self._binaryArrays = (offsets.astype(np.uint64), ids.astype(np.uint32))
-        self._binaryUniverse = universe
-        self._inputBackend = {"backend": "numpy-hip",
-                              "reason": "Temporal BIN arrays; the store is built on the device"}
+        self._binaryUniverse = int(universe)
+        self._setBackend("numpy-hip", "Temporal arrays; the store is built on the device")
         return None
 
     def _getDevicePairs(self, ctx=None):

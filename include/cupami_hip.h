@@ -30,11 +30,13 @@ extern "C" {
 #define CPM_ERR_RUNTIME    -2   /* RuntimeError: HIP call failed */
 #define CPM_ERR_NOMEM      -3   /* MemoryError: device allocation failed */
 #define CPM_ERR_NODEVICE   -4   /* RuntimeError: no usable gfx950 device */
+#define CPM_ERR_LAYOUT     -5   /* input the device reader does not take: read it on the host */
 
 typedef struct cpm_ctx cpm_ctx;
 typedef struct cpm_bits cpm_bits;
 typedef struct cpm_patterns cpm_patterns;
 typedef struct cpm_fptree cpm_fptree;
+typedef struct cpm_baskets cpm_baskets;
 
 /* ABI version (major*10000 + minor*100 + patch). */
 int cpm_version(void);
@@ -101,6 +103,27 @@ int cpm_bits_build_csr(cpm_ctx *ctx, const uint64_t *offsets, uint64_t n_baskets
 /* Same, with offsets/ids already in device memory. */
 int cpm_bits_build_csr_device(cpm_ctx *ctx, const uint64_t *d_offsets, uint64_t n_baskets, const uint32_t *d_ids,
                               uint64_t n_ids, uint32_t universe, int keep_empty, cpm_bits **out);
+/* Replaces the text side of the reference's input (_readDelimitedFile and
+ * basket cleaning, cuApriori.py:428-450, and the GPU text reader it tries
+ * first, cuApriori.py:223-247 with _gpu_io.py): delimited text (`text`,
+ * host memory, n_bytes) parsed on the device into basket arrays kept on the
+ * device.  Every line is a basket; the separator and '\n' end fields; fields
+ * are stripped of ASCII whitespace; empty fields are dropped; every other
+ * field must be a canonical decimal id (no sign or leading zero, at most
+ * 2^32 - 1), its label being its own text.  Returns CPM_ERR_LAYOUT (no
+ * baskets) for text the csv module would read differently (quotes, bytes
+ * >= 0x80, NUL, a '\r' not followed by '\n' before the end) or with other
+ * fields: the caller then reads it on the host. */
+int cpm_baskets_parse_text(cpm_ctx *ctx, const char *text, uint64_t n_bytes, char sep, cpm_baskets **out);
+/* Baskets (lines), ids, baskets with at least one id, largest id. */
+int cpm_baskets_info(const cpm_baskets *b, uint64_t *n_baskets, uint64_t *n_ids, uint64_t *n_nonempty,
+                     uint32_t *max_id);
+/* offsets[n_baskets + 1] and ids[n_ids] to host memory (either may be NULL). */
+int cpm_baskets_download(cpm_ctx *ctx, const cpm_baskets *b, uint64_t *offsets, uint32_t *ids);
+/* cpm_bits_build_csr_device on the parsed baskets (universe: the largest id). */
+int cpm_bits_build_baskets(cpm_ctx *ctx, const cpm_baskets *b, int keep_empty, cpm_bits **out);
+int cpm_baskets_destroy(cpm_baskets *b);
+
 /* The item id of each column of a store built by cpm_bits_build_csr*
  * (n_items values, host memory): the reference's `items` list
  * (cuApriori.py:384-385) as numbers. */
