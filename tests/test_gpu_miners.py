@@ -13,7 +13,7 @@ import pytest
 import generator
 import oracle
 from cupami_amd import cuApriori, cuEclat, cuFPGrowth, cuPFEclat
-from helpers import kats
+from helpers import kats, random_rows
 
 pytestmark = pytest.mark.gpu
 K = kats()
@@ -169,3 +169,52 @@ def test_periodic_random_and_boundaries():
         assert miner.getPatterns() == oracle.brute_periodic(rows, support, period, end)
         miner.mine()
         assert miner.getPatterns() == oracle.brute_periodic(rows, support, period, end)
+
+
+def test_overridden_hooks_take_the_host_search():
+    """A subclass that overrides cuFPGrowth._storePattern or cuPFEclat's
+    _evaluate / _accept gets a search that calls them: pass-through overrides
+    give the device search's result, and a False from _storePattern prunes
+    that pattern's conditional tree (cuFPGrowth.py:205)."""
+    from cupami_amd import cuFPGrowth, cuPFEclat
+    rng = random.Random(12)
+    rows = random_rows(rng, 400, 9, 6)
+
+    class Recording(cuFPGrowth):
+        def _storePattern(self, item_ids, support, labels):
+            super()._storePattern(item_ids, support, labels)
+
+    class Pruning(cuFPGrowth):
+        def _storePattern(self, item_ids, support, labels):
+            super()._storePattern(item_ids, support, labels)
+            return False
+
+    full = cuFPGrowth(rows, 5)
+    full.mine()
+    rec = Recording(rows, 5)
+    assert not rec._usesDeviceSearch()
+    rec.mine()
+    assert rec.getPatterns() == full.getPatterns() and len(full.getPatterns()) > 20
+    pruned = Pruning(rows, 5)
+    pruned.mine()
+    singles = {k: v for k, v in full.getPatterns().items() if "\t" not in k}
+    assert pruned.getPatterns() == singles
+
+    trows = [[rng.randrange(1, 60), *rng.sample(range(8), rng.randrange(4))] for _ in range(300)]
+
+    class Evaluating(cuPFEclat):
+        def _evaluate(self, ctx, candidates):
+            return super()._evaluate(ctx, candidates)
+
+    class Short(cuPFEclat):
+        def _accept(self, pattern, support, period):
+            return len(pattern) <= 2 and super()._accept(pattern, support, period)
+
+    pf = cuPFEclat(trows, 10, 20)
+    pf.mine()
+    ev = Evaluating(trows, 10, 20)
+    ev.mine()
+    assert ev.getPatterns() == pf.getPatterns() and len(pf.getPatterns()) > 10
+    sh = Short(trows, 10, 20)
+    sh.mine()
+    assert sh.getPatterns() == {k: v for k, v in pf.getPatterns().items() if k.count("\t") <= 1}
