@@ -42,36 +42,6 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measured_traffic(profile="eclat_1m_1k_hbm_traffic.md", key="k_support_count_hbm_bytes_per_launch"):
-    """HBM bytes per launch of a kernel from the committed counter runs
-    (profiles/<profile>), used only while the kernel sources are the ones
-    that were measured."""
-    prof = ROOT / "profiles" / profile
-    if not prof.exists():
-        return None
-    fields = dict(line.split(": ", 1) for line in prof.read_text().splitlines()
-                  if line.startswith((key + ":", "csrc_sha256:")))
-    if fields.get("csrc_sha256") != csrc_digest() or key not in fields:
-        return None
-    return float(fields[key])
-
-
-def csrc_digest(root=None):
-    """SHA-256 over the kernel sources (names and code, sorted), with comments
-    removed and whitespace runs folded, so that editing a comment keeps a
-    measurement valid while any change to the code does not."""
-    import hashlib
-    import re
-    token = re.compile(r'"(?:\\.|[^"\\])*"|\'(?:\\.|[^\'\\])*\'|//[^\n]*|/\*.*?\*/', re.S)
-    h = hashlib.sha256()
-    for path in sorted(((root or ROOT) / "cupami_amd" / "csrc").iterdir()):
-        if path.suffix in (".hip", ".h"):
-            text = token.sub(lambda m: m.group(0) if m.group(0)[0] in "\"'" else " ", path.read_text())
-            h.update(path.name.encode())
-            h.update(" ".join(text.split()).encode())
-    return h.hexdigest()
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,7 +57,8 @@ def main():
     ap.add_argument("--slots", type=int, default=3, help="serve: concurrent searches (each its own stream)")
     ap.add_argument("--hbm-budget-gb", type=float, default=8.0, help="serve: admission budget")
     ap.add_argument("--p99-bound-ms", type=float, default=25.0, help="serve: stated p99 latency bound")
-    ap.add_argument("--cpu-sample-s", type=float, default=15.0, help="CPU baseline: seconds of oracle work")
+    ap.add_argument("--cpu-sample-s", type=float, default=15.0, help="serve CPU baseline: seconds of oracle work")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="CPU baseline: timed runs (median)")
     ap.add_argument("--no-extras", action="store_true",
                     help="eclat_1m_1k: skip the batched and serving summaries")
     ap.add_argument("--copy-calibration", action="store_true")
@@ -129,20 +100,125 @@ def main():
             sys.exit(1)
         return
 
+    line, parity = run_search(args, args.workload)
+    if args.workload == "eclat_1m_1k" and not args.no_extras:
+        pf, pf_parity = run_search(argparse.Namespace(**{**vars(args), "steps": 10, "warmup": 2}), "pfeclat_2m_500")
+        line["pfeclat_2m_500"] = {k: pf[k] for k in ("value", "unit", "ms_per_step", "config", "roofline",
+                                                      "cpu_baseline", "parity")}
+        line["batched"], line["serve"] = extras(args)
+        line["quickstart"] = quickstart()
+        if any(v["patterns"] != 1275 for k, v in line["quickstart"].items() if k != "workload"):
+            parity = False
+        if False in (line["batched"]["parity"], line["serve"]["parity"], pf_parity):
+            parity = False
+    print(json.dumps(line), flush=True)
+    if parity is False:
+        sys.exit(1)
+
+
+# sources whose code a kernel's counter summary was measured on
+SUPPORT_SOURCES = ["cupami_amd/csrc/support.hip", "cupami_amd/csrc/support_kernels.h", "cupami_amd/csrc/search.hip",
+                   "cupami_amd/csrc/core.hip", "cupami_amd/csrc/cpm_internal.h"]
+GRAM_SOURCES = ["cupami_amd/csrc/gram.hip", "cupami_amd/csrc/cpm_internal.h"]
+MALL_BYTES = 256 << 20
+
+
+def support_roofline(workload, kernel, stats, steps, store_bytes):
+    """The support kernel against the level of the memory hierarchy that
+    bounds it.  ``achieved``: algorithmic bytes per launch (summed in the
+    kernel) / mean launch time (kernel clock stamps, live).  From the
+    committed summaries: the kernel's L2 request bytes and fabric bytes per
+    launch (counters) and the measured roofs.  Each level's time floor is
+    its bytes / its roof (L2 for requests; the Infinity Cache for fabric
+    reads when the store fits it, HBM otherwise); the larger floor names the
+    bound, and ``peak`` is that floor in algorithmic bytes, so ``frac`` <= 1
+    when the model holds.  ``traffic``: HBM bytes per launch from FETCH_SIZE
+    and WRITE_SIZE scaled by a copy of known size (an upper bound when the
+    store sits in the Infinity Cache)."""
+    from cupami_amd import profiling
+    launches = max(stats["launches"], 1)
+    avg_ms = stats["kernel_ms"] / launches
+    alg = stats["bytes"] / launches
+    achieved = alg / (avg_ms / 1e3) / 1e9 if avg_ms > 0 else 0.0
+    line = {"bound": None, "achieved": round(achieved, 1), "peak": None, "unit": "GB/s", "frac": None,
+            "traffic": None, "kernel": kernel, "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": alg,
+            "launches_per_step": launches / steps}
+    roofs = profiling.read_summary(ROOT / "profiles" / "roofs.md")
+    cnt = profiling.read_summary(ROOT / "profiles" / f"{workload}_{kernel}_counters.md")
+    if not roofs or "l2_request_bytes_per_launch" not in cnt or avg_ms <= 0:
+        line["basis"] = "no current counter summary for these sources: bound not placed"
+        return line
+    l2_b = float(cnt["l2_request_bytes_per_launch"])
+    fab_b = float(cnt["fabric_read_bytes_per_launch"])
+    fab_level = "mall" if store_bytes <= MALL_BYTES else "hbm"
+    r_l2, r_fab = float(roofs["l2_gb_s"]), float(roofs[f"{fab_level}_gb_s"])
+    t_l2, t_fab = l2_b / (r_l2 * 1e9), fab_b / (r_fab * 1e9)
+    bound, floor = ("l2", t_l2) if t_l2 >= t_fab else (fab_level, t_fab)
+    peak = alg / floor / 1e9
+    t = avg_ms / 1e3
+    line.update(bound=bound, peak=round(peak, 1), frac=round(achieved / peak, 4),
+                traffic=float(cnt["hbm_bytes_per_launch"]) if "hbm_bytes_per_launch" in cnt else None)
+    line["basis"] = {
+        "l2_request_bytes_per_launch": l2_b, "l2_hit_rate": float(cnt.get("l2_hit_rate", "nan")),
+        "l2_roof_gb_s": r_l2, "l2_gb_s": round(l2_b / t / 1e9, 1),
+        "fabric_read_bytes_per_launch": fab_b, f"{fab_level}_roof_gb_s": r_fab,
+        "fabric_gb_s": round(fab_b / t / 1e9, 1),
+        "fabric_frac_of_hbm_published": round(fab_b / t / 1e9 / HBM_PEAK_GBS, 4),
+        "summaries": ["profiles/roofs.md", f"profiles/{workload}_{kernel}_counters.md"]}
+    return line
+
+
+INGEST_SOURCES = ["cupami_amd/csrc/ingest.hip", "cupami_amd/csrc/core.hip", "cupami_amd/csrc/cpm_internal.h"]
+
+
+def ingest_traffic():
+    """HBM bytes per launch of k_ingest_tile from its committed counter
+    summary (None once the sources it measured change)."""
+    from cupami_amd import profiling
+    cnt = profiling.read_summary(ROOT / "profiles" / "ingest_1m_1k_k_ingest_tile_counters.md")
+    return float(cnt["hbm_bytes_per_launch"]) if "hbm_bytes_per_launch" in cnt else None
+
+
+def cpu_model():
+    """The host CPU's model name, logical CPU count and the oracle's threads."""
+    import oracle
+    model = ""
+    try:
+        model = next((l.split(":", 1)[1].strip() for l in Path("/proc/cpuinfo").read_text().splitlines()
+                      if l.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "threads": oracle.threads()}
+
+
+def timed_median(fn, runs):
+    """(median seconds, every run's seconds, the last result) of ``runs``
+    calls of ``fn``."""
+    times, out = [], None
+    for _ in range(runs):
+        t = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t)
+    return float(sorted(times)[len(times) // 2]), [round(x, 3) for x in times], out
+
+
+def run_search(args, workload):
+    """One database's store build + device search per step (eclat_1m_1k,
+    eclat_4m_1k, pfeclat_2m_500), its roofline and CPU baseline."""
     import numpy as np
     import torch
     from cupami_amd import _hip, datagen
 
-    periodic = args.workload == "pfeclat_2m_500"
+    periodic = workload == "pfeclat_2m_500"
+    big = workload == "eclat_4m_1k"
     t = time.perf_counter()
-    big = args.workload == "eclat_4m_1k"
     db = (datagen.temporal_2m_500() if periodic else
           datagen.transactional(4_000_000, 1000, 10.0, 1.1, 202) if big else datagen.eclat_1m_1k())
     labels, cols = datagen.label_columns(db.items)
     minsup = PF_MINSUP if periodic else 4 * MINSUP if big else MINSUP
     end = int(db.timestamps[-1]) if periodic else 0
     n, items, nnz = db.n_transactions, len(labels), db.rows.size
-    log(f"bench: generated {n} x {items} ({nnz} pairs) in {time.perf_counter() - t:.1f}s")
+    log(f"bench: {workload}: generated {n} x {items} ({nnz} pairs) in {time.perf_counter() - t:.1f}s")
 
     dev = torch.device("cuda:0")
     d_rows = torch.from_numpy(db.rows.view(np.int64)).to(dev)
@@ -178,40 +254,72 @@ def main():
     value = n_pat / (elapsed / args.steps)
     log(f"bench: {args.steps} steps, {ms_per_step:.2f} ms/step, {n_pat} patterns")
 
-    launches = max(stats["launches"], 1)
-    avg_ms = stats["kernel_ms"] / launches
-    bytes_per_launch = stats["bytes"] / launches
-    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9 if avg_ms > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_support_count", "avg_launch_ms": round(avg_ms, 4),
-                "bytes_per_launch": bytes_per_launch, "launches_per_step": launches / args.steps}
+    # the reference's mine() also builds its pattern dictionary: the same
+    # step followed by getPatterns()'s dict, timed apart from the headline
+    def step_with_dict():
+        p = step()
+        return dict(zip(p.keys(labels), p.support.tolist()))
+    dict_s, _, got_dict = timed_median(step_with_dict, 3)
+
+    kernel = "k_support_periodic" if periodic else "k_support_count"
+    roofline = support_roofline(workload, kernel, stats, args.steps, items * (((n + 63) // 64 + 63) // 64 * 64) * 8)
     gram_ms = stats["gram_ms"] / max(stats["gram_launches"], 1)
     gram_ops = stats["gram_ops"] / max(stats["gram_launches"], 1)
     # dense FP4 MFMA peak of the MI355X (the Gram's operands are E2M1 0 / 1)
     gram = {"kernel": "k_gram", "dtype": "fp4", "avg_launch_ms": round(gram_ms, 4),
             "tops": round(gram_ops / (gram_ms / 1e3) / 1e12, 2) if gram_ms > 0 else 0.0,
             "peak_tops": 10100.0}
+    gram["frac"] = round(gram["tops"] / gram["peak_tops"], 4)
+    from cupami_amd import profiling
+    roofs = profiling.read_summary(ROOT / "profiles" / "roofs.md")
+    if roofs.get("mfma_fp4_32x32x64_tops"):
+        gram["measured_loop_tops"] = float(roofs["mfma_fp4_32x32x64_tops"])
+        gram["frac_of_loop"] = round(gram["tops"] / gram["measured_loop_tops"], 4)
+    gcnt = profiling.read_summary(ROOT / "profiles" / f"{workload}_k_gram_counters.md")
+    if gcnt.get("avg_sq_valu_mfma_busy_cycles"):
+        # busy cycles summed over every SIMD; GRBM_GUI_ACTIVE summed over the 8 XCDs
+        simds = 4 * int(roofs.get("cu_count", 256) or 256)
+        gram["mfma_busy_frac"] = round(float(gcnt["avg_sq_valu_mfma_busy_cycles"]) /
+                                       (float(gcnt["avg_grbm_gui_active"]) / 8 * simds), 4)
 
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
         import oracle
         host = oracle.bits_build(db.rows, cols, n, items)
-        threads = oracle.threads()
-        t = time.perf_counter()
+        runs = args.cpu_runs
         if periodic:
-            expect = oracle.pfeclat(host, db.timestamps, n, end, minsup, PF_MAXPER)
+            search = lambda: oracle.pfeclat(host, db.timestamps, n, end, minsup, PF_MAXPER)
+            expect = search()   # parity first, then the timed runs
+            parity = pats.to_dict() == expect
+            med, all_s, _ = timed_median(search, runs)
+            cpu = {"value": round(len(expect) / med, 1), "unit": "itemsets/s", "kind": "port",
+                   "sample": f"whole workload ({len(expect)} itemsets), median of {runs} warm runs after the "
+                             "parity check: oracle/cpm_oracle.c vertical PF-Eclat, OpenMP over prefix classes",
+                   "median_s": round(med, 3), "runs_s": all_s}
         else:
-            expect = oracle.eclat(host, minsup)
-        cpu_s = time.perf_counter() - t
-        parity = pats.to_dict() == expect
-        cpu = {"value": round(len(expect) / cpu_s, 1), "unit": "itemsets/s", "cores": threads,
-               "kind": "port",
-               "sample": f"whole workload, one run ({len(expect)} itemsets in {cpu_s:.2f}s): "
-                         f"oracle/cpm_oracle.c vertical {'PF-' if periodic else ''}Eclat, "
-                         "OpenMP over prefix classes"}
-        log(f"bench: cpu baseline {cpu_s:.2f}s on {threads} threads, parity={parity}")
+            eclat = lambda: oracle.eclat(host, minsup)
+            apriori = lambda: oracle.apriori(host, minsup)
+            expect = eclat()
+            parity = pats.to_dict() == expect and oracle.apriori(host, minsup) == expect
+            if not big:
+                labelled = {"\t".join(labels[c] for c in k): v for k, v in expect.items()}
+                parity = parity and got_dict == labelled
+            e_med, e_all, _ = timed_median(eclat, runs)
+            a_med, a_all, _ = timed_median(apriori, runs)
+            best = min(e_med, a_med)
+            cpu = {"value": round(len(expect) / best, 1), "unit": "itemsets/s", "kind": "port",
+                   "sample": f"whole workload ({len(expect)} itemsets), median of {runs} warm runs each after the "
+                             "parity check; value from the faster of the clApriori restatement "
+                             "(u32 words, batches of 4096, _opencl_bitsets.py:21-42, clApriori.py:164-196) and "
+                             "the vertical Eclat oracle, both oracle/cpm_oracle.c with OpenMP",
+                   "clapriori_restatement": {"median_s": round(a_med, 3), "runs_s": a_all,
+                                             "itemsets_per_s": round(len(expect) / a_med, 1)},
+                   "eclat_oracle": {"median_s": round(e_med, 3), "runs_s": e_all,
+                                    "itemsets_per_s": round(len(expect) / e_med, 1)}}
+        cpu.update(cpu_model())
+        cpu["cores"] = cpu["threads"]
+        log(f"bench: cpu baseline {cpu['value']} itemsets/s on {cpu['threads']} threads, parity={parity}")
         if not parity:
             log("bench: PARITY FAILURE against the oracle")
 
@@ -223,16 +331,12 @@ def main():
         torch.cuda.synchronize()
         log("bench: copied 268435456 bytes device to device (counter calibration)")
 
-    kernel = "k_support_periodic" if periodic else "k_support_count"
-    roofline["kernel"] = kernel
-    roofline["traffic"] = (measured_traffic("pfeclat_2m_500_hbm_traffic.md", "k_support_periodic_hbm_bytes_per_launch")
-                           if periodic else measured_traffic())
     if periodic:
         config = {"workload": "pfeclat_2m_500", "transactions": n, "items": items, "lambda": 8,
                   "zipf_s": 1.05, "minsup": minsup, "maxper": PF_MAXPER, "patterns": n_pat,
                   "parallelism": "single GPU"}
     else:
-        config = {"workload": args.workload, "transactions": n, "items": items,
+        config = {"workload": workload, "transactions": n, "items": items,
                   "lambda": 10, "zipf_s": 1.1, "minsup": minsup, "patterns": n_pat,
                   "parallelism": "single GPU"}
     line = {
@@ -242,18 +346,11 @@ def main():
         "data": f"synthetic (numpy PCG64 seed {404 if periodic else 202})",
         "config": config,
         "roofline": roofline, "cpu_baseline": cpu, "gram": gram,
+        "with_get_patterns": {"ms_per_step": round(dict_s * 1e3, 3), "itemsets_per_s": round(n_pat / dict_s, 1),
+                              "note": "the step followed by getPatterns()'s dict, median of 3"},
         "levels_per_step": stats["levels"], "parity": parity,
     }
-    if not periodic and not big and not args.no_extras:
-        line["batched"], line["serve"] = extras(args)
-        line["quickstart"] = quickstart()
-        if any(v["patterns"] != 1275 for k, v in line["quickstart"].items() if k != "workload"):
-            parity = False
-        if line["batched"]["parity"] is False or line["serve"]["parity"] is False:
-            parity = False
-    print(json.dumps(line), flush=True)
-    if parity is False:
-        sys.exit(1)
+    return line, parity
 
 
 def quickstart():
@@ -488,10 +585,9 @@ def run_batch(args):
                          "oracle/cpm_oracle.c vertical Eclat, OpenMP over prefix classes"}
     log(f"bench: parity={parity}")
 
-    launches = max(stats["launches"], 1)
-    avg_ms = stats["kernel_ms"] / launches
-    bytes_per_launch = stats["bytes"] / launches
-    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9 if avg_ms > 0 else 0.0
+    store_bytes = sum(len(l) * (((db.n_transactions + 63) // 64 + 63) // 64 * 64) * 8
+                      for (db, _), l in zip(dbs, labels_all))
+    roofline = support_roofline("batch_256", "k_support_count", stats, args.steps, store_bytes)
     line = {
         "metric": "itemsets/s (batched Eclat, 256 databases) vs plain", "value": round(n_pat / batched_s, 1),
         "unit": "itemsets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -500,10 +596,7 @@ def run_batch(args):
         "config": {"workload": "batch_256", "databases": len(dbs), "transactions": sum(db.n_transactions for db, _ in dbs),
                    "pairs": sum(nnz), "patterns": n_pat, "schedule": {"group": config[0], "tile_words": config[1]},
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_support_count",
-                     "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": bytes_per_launch,
-                     "launches_per_step": launches / args.steps},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "plain": {"one_at_a_time_ms": round(seq_s * 1e3, 2),
                   "speedup_batched": round(seq_s / batched_s, 2),
@@ -619,7 +712,7 @@ def run_ingest(args):
     achieved = tile_bytes / (tile_ms / 1e3) / 1e9 if tile_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic("ingest_1m_1k_hbm_traffic.md", "k_ingest_tile_hbm_bytes_per_launch"),
+                "traffic": ingest_traffic(),
                 "kernel": "k_ingest_tile",
                 "avg_launch_ms": round(tile_ms, 4), "bytes_per_launch": tile_bytes}
     log(f"bench: {args.steps} builds, {ms_per_step:.3f} ms/step (device span {st['build_ms'] / builds:.3f} ms), "
@@ -794,6 +887,12 @@ def run_serve(args):
     server.run(warm)
     server.evict_all()
     mine_one_at_a_time(warm[:8], ctx)
+    # long-lived objects (the pool, earlier workloads' data) are moved out of
+    # the collector's reach, as a serving process does at start-up, so a
+    # full collection does not stall the request loop
+    import gc
+    gc.collect()
+    gc.freeze()
     t = time.perf_counter()
     served = server.run(reqs)
     served_s = time.perf_counter() - t
@@ -808,6 +907,7 @@ def run_serve(args):
     t = time.perf_counter()
     resident = mine_sequential_resident(reqs, ctx)
     resident_s = time.perf_counter() - t
+    gc.unfreeze()
     log(f"bench: one at a time, resident stores {resident_s:.2f}s")
 
     dig = [(o.patterns.digest(), a.patterns.digest(), q.patterns.digest())
