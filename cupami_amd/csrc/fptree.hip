@@ -16,26 +16,10 @@
 // _fptree.conditional_tree (_fptree.py:180-231): ancestor walks add each node's
 // count into its ancestors' items, frequent ancestors are emitted as a
 // weighted basket per node, and the basket set is built into a tree again.
-// The search (cuFPGrowth._search, cuFPGrowth.py:187-213) runs on the host over
-// device trees; single-path trees are enumerated directly.
-#include "sort.h"
+// The FP-growth search over these trees is fpgrowth.hip.
+#include "fptree.h"
 
 #include <string.h>
-#include <algorithm>
-
-struct cpm_fptree {
-    cpm_ctx *ctx = nullptr;
-    uint64_t n_nodes = 0;
-    uint32_t depth = 0;
-    bool single_path = true;
-    uint32_t item_count = 0;
-    cpm::DBuf parents, items, counts;  // int64, int64, uint64
-    std::vector<uint32_t> header_item;  // ascending (support, then descending id)
-    std::vector<uint64_t> header_sup;
-    bool links = false;
-    cpm::DBuf link_nodes;               // node ids grouped by item, ascending within an item
-    std::vector<uint32_t> link_off;     // item_count + 1
-};
 
 namespace cpm {
 
@@ -238,12 +222,6 @@ __global__ void k_root(int64_t *parents, int64_t *items, unsigned long long *cou
     counts[0] = 0;
 }
 
-__global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = v;
-}
-
 __global__ void k_max_u32(const uint32_t *__restrict__ v, uint64_t n, unsigned int *__restrict__ out) {
     uint32_t m = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -276,21 +254,6 @@ __global__ void k_ancestor_supports(const uint32_t *__restrict__ nodes, uint64_t
         const uint32_t node = nodes[i];
         const unsigned long long w = counts[node];
         for (int64_t p = parents[node]; p > 0; p = parents[p]) atomicAdd(sup + items[p], w);
-    }
-}
-
-// Conditional supports of every item of a tree at once: for each non-root
-// node n of item x and each ancestor a of n below the root,
-// M[x * I + item(a)] += count(n).  Row x is what k_ancestor_supports gives
-// for item x alone.
-__global__ void k_ancestor_matrix(const int64_t *__restrict__ parents, const int64_t *__restrict__ items,
-                                  const unsigned long long *__restrict__ counts, uint64_t n_nodes, uint32_t I,
-                                  unsigned long long *__restrict__ M) {
-    for (uint64_t n = 1 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; n < n_nodes;
-         n += (uint64_t)gridDim.x * blockDim.x) {
-        const unsigned long long w = counts[n];
-        unsigned long long *row = M + (uint64_t)items[n] * I;
-        for (int64_t p = parents[n]; p > 0; p = parents[p]) atomicAdd(row + items[p], w);
     }
 }
 
@@ -348,21 +311,9 @@ __global__ void k_unique_emit(const uint64_t *__restrict__ keys, const uint32_t 
         }
 }
 
-inline dim3 grid_of(uint64_t n) {
-    return dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 16384)));
-}
-
 }  // namespace
 
-// Working buffers of a build / projection / search.
-struct FpWs {
-    SortWs sort;
-    DBuf keys, vals, flag, pos, rows, cols, rows2, cols2, weights, rank, lengths, off, act, act2,
-        pfr, sup, freq, total, nodes, scalar;
-    HBuf host;
-};
-
-static int read_u64(cpm_ctx *ctx, FpWs &ws, const void *d, uint64_t *out) {
+int read_u64(cpm_ctx *ctx, FpWs &ws, const void *d, uint64_t *out) {
     CPM_CHECK(ws.host.ensure(64));
     CPM_HIP(hipMemcpyAsync(ws.host.p, d, 8, hipMemcpyDeviceToHost, ctx->stream));
     CPM_HIP(hipStreamSynchronize(ctx->stream));
@@ -371,7 +322,7 @@ static int read_u64(cpm_ctx *ctx, FpWs &ws, const void *d, uint64_t *out) {
 }
 
 // Two device regions to host memory with one wait (through the pinned buffer).
-static int read_two(cpm_ctx *ctx, FpWs &ws, const void *d0, size_t b0, void *h0, const void *d1, size_t b1, void *h1) {
+int read_two(cpm_ctx *ctx, FpWs &ws, const void *d0, size_t b0, void *h0, const void *d1, size_t b1, void *h1) {
     const size_t o1 = (b0 + 63) / 64 * 64;
     CPM_CHECK(ws.host.ensure(o1 + b1 + 64));
     unsigned char *h = ws.host.as<unsigned char>();
@@ -557,7 +508,7 @@ int fptree_build(cpm_ctx *ctx, FpWs &ws, const uint64_t *d_rows, const uint32_t 
     return CPM_OK;
 }
 
-static int build_links(cpm_ctx *ctx, FpWs &ws, cpm_fptree *t) {
+int build_links(cpm_ctx *ctx, FpWs &ws, cpm_fptree *t) {
     if (t->links) return CPM_OK;
     hipStream_t st = ctx->stream;
     const uint64_t n = t->n_nodes > 0 ? t->n_nodes - 1 : 0;
@@ -812,153 +763,6 @@ int cpm_fptree_destroy(cpm_fptree *t) {
     }
     delete t;
     return CPM_OK;
-}
-
-// FP-growth (cuFPGrowth._mineWithPool / _search, cuFPGrowth.py:170-213).
-int cpm_fpgrowth_mine(cpm_ctx *ctx, const uint64_t *rows, const uint32_t *cols, uint64_t nnz,
-                      uint64_t n_transactions, uint32_t n_items, uint64_t minsup, cpm_patterns **out) {
-    clear_error();
-    CPM_CHECK(bind(ctx));
-    if (!out) return fail(CPM_ERR_INVALID, "null out");
-    *out = nullptr;
-    if (nnz && (!rows || !cols)) return fail(CPM_ERR_INVALID, "null argument");
-    if (n_items && minsup < 1)
-        return fail(CPM_ERR_INVALID, "FP-growth requires a positive minimum support count");
-    for (uint64_t i = 0; i < nnz; ++i)
-        if (rows[i] >= n_transactions || cols[i] >= n_items)
-            return fail(CPM_ERR_INVALID, "pair %llu outside %llu x %u", (unsigned long long)i,
-                        (unsigned long long)n_transactions, n_items);
-    hipStream_t st = ctx->stream;
-    cpm_patterns *res = new cpm_patterns();
-    auto finish = [&](int s) {
-        if (s != CPM_OK) delete res;
-        else *out = res;
-        return s;
-    };
-    if (nnz == 0 || n_items == 0) return finish(CPM_OK);
-    FpWs ws;
-    DBuf dr, dc, ur, uc, w;
-    int s = CPM_OK;
-    if ((s = dr.ensure(nnz * 8)) != CPM_OK) return finish(s);
-    if ((s = dc.ensure(nnz * 4)) != CPM_OK) return finish(s);
-    if ((s = ur.ensure(nnz * 8)) != CPM_OK) return finish(s);
-    if ((s = uc.ensure(nnz * 4)) != CPM_OK) return finish(s);
-    if ((s = w.ensure(std::max<uint64_t>(n_transactions, 1) * 8)) != CPM_OK) return finish(s);
-    if (hipMemcpyAsync(dr.p, rows, nnz * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(dc.p, cols, nnz * 4, hipMemcpyHostToDevice, st) != hipSuccess)
-        return finish(fail(CPM_ERR_RUNTIME, "upload failed"));
-    uint64_t nu = 0;
-    if ((s = unique_pairs(ctx, ws, dr.as<uint64_t>(), dc.as<uint32_t>(), nnz, ur.as<uint64_t>(),
-                          uc.as<uint32_t>(), &nu, n_transactions ? n_transactions - 1 : 0)) != CPM_OK)
-        return finish(s);
-    hipLaunchKernelGGL(k_fill_u64, grid_of(n_transactions), dim3(256), 0, st, w.as<uint64_t>(),
-                       n_transactions, 1ull);
-    cpm_fptree *root = nullptr;
-    if ((s = fptree_build(ctx, ws, ur.as<uint64_t>(), uc.as<uint32_t>(), nu, w.as<uint64_t>(),
-                          n_transactions, n_items, minsup, nullptr, &root)) != CPM_OK)
-        return finish(s);
-    if (!root) return finish(CPM_OK);
-
-    const uint32_t kNoParent = 0xffffffffu;
-    auto emit = [&](uint32_t parent, uint32_t item, uint64_t support) {
-        res->parent.push_back(parent);
-        res->item.push_back(item);
-        res->support.push_back(support);
-        return (uint32_t)(res->parent.size() - 1);
-    };
-    // A tree's conditional supports for all its items, computed when the
-    // tree is first visited (items up to kMatrixItems): one launch and one
-    // read instead of one of each per item, and items whose conditional tree
-    // would be empty or a single node need no device work.
-    constexpr uint32_t kMatrixItems = 1448;  // 16 MB of u64 supports
-    struct Mat {
-        DBuf d;
-        std::vector<uint64_t> h;
-    };
-    struct Frame {
-        cpm_fptree *tree;
-        uint32_t pid;  // pattern id of the suffix (kNoParent for the empty suffix)
-        size_t position;
-        std::shared_ptr<Mat> mat;
-    };
-    std::vector<Frame> stack;
-    stack.push_back({root, kNoParent, 0, nullptr});
-    while (!stack.empty() && s == CPM_OK) {
-        Frame &f = stack.back();
-        cpm_fptree *t = f.tree;
-        if (t->single_path) {
-            // every non-empty subset of the path, support = smallest count
-            const uint64_t len = t->n_nodes - 1;
-            std::vector<int64_t> it(t->n_nodes);
-            std::vector<uint64_t> ct(t->n_nodes);
-            s = cpm_fptree_download(t, nullptr, it.data(), ct.data(), nullptr, nullptr);
-            if (s != CPM_OK) break;
-            if (len > 40) { s = fail(CPM_ERR_NOMEM, "single path of %llu items", (unsigned long long)len); break; }
-            // depth-first over subsets in position order; parent of a subset is
-            // the subset without its last position
-            struct Sub { uint64_t next; uint32_t pid; uint64_t minc; };
-            std::vector<Sub> subs;
-            subs.push_back({0, f.pid, ~0ull});
-            while (!subs.empty()) {
-                Sub &top = subs.back();
-                if (top.next >= len) { subs.pop_back(); continue; }
-                const uint64_t p = top.next++;
-                const uint64_t minc = std::min<uint64_t>(top.minc, ct[p + 1]);
-                const uint32_t id = emit(top.pid, (uint32_t)it[p + 1], minc);
-                subs.push_back({p + 1, id, minc});
-            }
-            cpm_fptree_destroy(t);
-            stack.pop_back();
-            continue;
-        }
-        if (f.position >= t->header_item.size()) {
-            cpm_fptree_destroy(t);
-            stack.pop_back();
-            continue;
-        }
-        if (f.position == 0 && !f.mat && t->item_count <= kMatrixItems) {
-            auto m = std::make_shared<Mat>();
-            const uint64_t I = t->item_count, cells = I * I;
-            if ((s = m->d.ensure(cells * 8)) != CPM_OK) break;
-            if (hipMemsetAsync(m->d.p, 0, cells * 8, st) != hipSuccess) {
-                s = fail(CPM_ERR_RUNTIME, "hipMemsetAsync failed");
-                break;
-            }
-            hipLaunchKernelGGL(k_ancestor_matrix, grid_of(t->n_nodes), dim3(256), 0, st, t->parents.as<int64_t>(),
-                               t->items.as<int64_t>(), t->counts.as<unsigned long long>(), t->n_nodes, (uint32_t)I,
-                               m->d.as<unsigned long long>());
-            m->h.resize(cells);
-            if ((s = copy_to_host(ctx, m->h.data(), m->d.p, cells * 8)) != CPM_OK) break;
-            f.mat = m;
-        }
-        const size_t pos = f.position++;
-        const uint32_t item = t->header_item[pos];
-        const uint32_t id = emit(f.pid, item, t->header_sup[pos]);
-        cpm_fptree *cond = nullptr;
-        if (f.mat) {
-            const uint64_t I = t->item_count;
-            const uint64_t *row = f.mat->h.data() + (uint64_t)item * I;
-            uint32_t nfreq = 0, only = 0;
-            for (uint32_t i = 0; i < I; ++i)
-                if (row[i] >= minsup) {
-                    ++nfreq;
-                    only = i;
-                }
-            if (nfreq == 0) continue;  // no frequent ancestor: no conditional tree
-            if (nfreq == 1) {          // a single-node conditional tree: its one pattern
-                emit(id, only, row[only]);
-                continue;
-            }
-            s = conditional_from(ctx, ws, t, item, minsup, f.mat->d.as<unsigned long long>() + (uint64_t)item * I,
-                                 row, &cond);
-        } else {
-            s = fptree_conditional(ctx, ws, t, item, minsup, &cond);
-        }
-        if (s == CPM_OK && cond) stack.push_back({cond, id, 0, nullptr});
-    }
-    for (auto &f : stack) cpm_fptree_destroy(f.tree);
-    res->n = res->parent.size();
-    return finish(s);
 }
 
 }  // extern "C"

@@ -104,3 +104,47 @@ def test_fpgrowth_mine_zipf_medium(ctx):
     expect = oracle.eclat(host, 60)
     assert len(expect) > 500
     assert as_sets == expect
+
+
+def _mine_sets(ctx, rows, minsup):
+    labels, rr, cc, n = coo(rows)
+    got = ctx.fpgrowth_mine(rr, cc, n, len(labels), minsup)
+    as_sets = {tuple(sorted(k)): v for k, v in got.to_dict().items()}
+    assert len(as_sets) == len(got)
+    return as_sets, oracle.eclat(oracle.bits_build(rr, cc, n, len(labels)), minsup)
+
+
+def test_fpgrowth_mine_wide_conditional(ctx):
+    # item 0 (support 70) sits under 70 more frequent items (support 72),
+    # each frequent with it (twice) while no two of them are: its
+    # conditional tree has more items than one workgroup's mask holds and
+    # takes the general projection
+    ring = [[0, 1 + i, 1 + (i + 1) % 70] for i in range(70)]
+    solo = [[1 + i] for i in range(70) for _ in range(70)]
+    got, expect = _mine_sets(ctx, ring + solo, 2)
+    assert any(len(k) == 2 for k in expect)
+    assert got == expect
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_fpgrowth_mine_deep(ctx, seed):
+    # dense baskets: patterns of up to ~9 items, conditional trees of
+    # conditional trees several levels down, all built one per workgroup
+    rng = random.Random(100 + seed)
+    rows = [[i for i in range(14) if rng.random() < 0.72] for _ in range(400)]
+    for minsup in (40, 90):
+        got, expect = _mine_sets(ctx, rows, minsup)
+        assert max(map(len, expect)) >= 5
+        assert got == expect
+
+
+def test_fpgrowth_mine_many_nodes_per_item(ctx):
+    # 30000 random subsets of 16 items: the first tree holds thousands of
+    # nodes of each low item (general projection), whose conditional trees
+    # are searched in the compact layout further down
+    rng = np.random.default_rng(5)
+    dense = rng.random((30000, 16)) < 0.5
+    rows = [np.flatnonzero(r).tolist() for r in dense]
+    got, expect = _mine_sets(ctx, rows, 900)
+    assert len(expect) > 1000
+    assert got == expect
