@@ -88,10 +88,18 @@ struct Range {
     uint32_t r0, r1, tile;
 };
 
-__device__ __forceinline__ Range range_of(const LevelView &v, uint64_t it) {
+// Item `it` is the k-th of its block.  When the grid is a multiple of the
+// tiles per range, item it would always land on tile it % te of the same
+// block; the tile is rotated by k (by 8 k when te is a multiple of 8, which
+// keeps a block on the tiles of its XCD) so that the tiles ranges of short
+// rows leave empty spread over all blocks.  Each run of gridDim.x items covers
+// whole ranges, so the map stays one to one.
+__device__ __forceinline__ Range range_of(const LevelView &v, uint64_t it, uint64_t k) {
     const uint64_t g = it / v.te;
     Range r;
-    r.tile = (uint32_t)(it - g * v.te);
+    uint64_t tile = it - g * v.te;
+    if (v.te > 1 && gridDim.x % v.te == 0) tile = (tile + k * (v.te % 8 == 0 ? 8 : 1)) % v.te;
+    r.tile = (uint32_t)tile;
     r.r0 = (uint32_t)(g * v.G);
     r.r1 = min(v.n_ext, r.r0 + v.G);
     return r;
@@ -128,8 +136,8 @@ __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
     constexpr uint32_t kQuadsPerTile = 256 * QPL;
     const unsigned long long t_start = wall_clock64();
     unsigned long long bytes = 0, cands = 0;
-    for (uint64_t it = blockIdx.x; it < v.total; it += gridDim.x) {
-        const Range r = range_of(v, it);
+    for (uint64_t it = blockIdx.x, k = 0; it < v.total; it += gridDim.x, ++k) {
+        const Range r = range_of(v, it, k);
         uint32_t t = find_task(v.ext_off, v.n_tasks, r.r0);
         // The AND of the first d - 1 prefix rows of the last task built here
         // (its parent's itemset), reused while the next tasks are its
@@ -330,8 +338,8 @@ __global__ __launch_bounds__(256) void k_support_periodic(SupportArgs a) {
     if (!level_view(a, lv)) return;
     const unsigned long long t_start = wall_clock64();
     unsigned long long bytes = 0, cands = 0;
-    for (uint64_t it = blockIdx.x; it < lv.total; it += gridDim.x) {
-        const Range r = range_of(lv, it);
+    for (uint64_t it = blockIdx.x, k = 0; it < lv.total; it += gridDim.x, ++k) {
+        const Range r = range_of(lv, it, k);
         uint32_t t = find_task(lv.ext_off, lv.n_tasks, r.r0);
         for (uint32_t e0 = r.r0; e0 < r.r1;) {
             while (lv.ext_off[t + 1] <= e0) ++t;  // empty tasks
