@@ -540,9 +540,9 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             psum = lane < d ? item_term(sv) : 0ull;
             for (int off = 32; off > 0; off >>= 1) psum += __shfl_xor(psum, off, 64);
         }
-        uint32_t f = 0;
+        uint32_t f = 0, e = 0;
         if (lane < nk) {
-            const uint32_t e = p.kept[a + lane];
+            e = p.kept[a + lane];
             f = p.L.ext[e];
             p.pat_parent[pb + lane] = p.L.pid[t];
             p.pat_item[pb + lane] = p.fcol ? p.fcol[f] : f;
@@ -592,6 +592,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
         if (live) {
             const uint32_t ct = cb + lane;
             p.N.par[ct] = t;
+            p.N.src[ct] = e;
             p.N.pid[ct] = (uint32_t)(pb + lane);
             p.N.db[ct] = dbi;
             uint32_t w = p.off_cext[t] + off;
@@ -650,10 +651,11 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
         for (uint32_t j0 = 0; j0 + 1 < nk; j0 += 256) {
             const uint32_t j = j0 + tid;
             const bool live = j + 1 < nk;
-            uint32_t cj = 0, fj = 0;
+            uint32_t cj = 0, fj = 0, ej = 0;
             const uint32_t *prow = nullptr;
             if (live) {
-                fj = p.L.ext[p.kept[a + j]];
+                ej = p.kept[a + j];
+                fj = p.L.ext[ej];
                 if (table) {
                     prow = table + (uint64_t)(fj - D.fbase) * D.pair_w32;
                     cj = child_count(lmask, prow, fj - D.fbase, wlast);
@@ -669,6 +671,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
                 for (uint32_t i = 0; i < d; ++i) dst[i] = spre[i];
                 dst[d] = fj;
                 p.N.par[ct] = t;
+                p.N.src[ct] = ej;
                 p.N.pid[ct] = (uint32_t)(pb + j);
                 p.N.db[ct] = dbi;
                 uint32_t w = xb + carry + off;
@@ -1094,13 +1097,15 @@ __global__ __launch_bounds__(256) void k_scatter(const PutRec *__restrict__ recs
 // ---- host side ---------------------------------------------------------------------
 
 struct LevelSet {
-    DBuf prefix, pid, par, db, ext_off, ext, sup, per;
+    DBuf prefix, pid, par, src, db, ext_off, ext, sup, per;
     DBuf kept, cnt[3], off[3], off2, ext2, live, pos, parts;
+    DBuf rows;  // retained itemset rows of this set's entries
     LevelPtrs ptrs(LevelSizes *sz) const {
         LevelPtrs p;
         p.prefix = prefix.as<uint32_t>();
         p.pid = pid.as<uint32_t>();
         p.par = par.as<uint32_t>();
+        p.src = src.as<uint32_t>();
         p.db = db.as<uint32_t>();
         p.ext_off = ext_off.as<uint32_t>();
         p.ext = ext.as<uint32_t>();
@@ -1114,6 +1119,7 @@ struct LevelSet {
         CPM_CHECK(prefix.ensure(t1 * kMaxDepth * 4));
         CPM_CHECK(pid.ensure(t1 * 4));
         CPM_CHECK(par.ensure(t1 * 4));
+        CPM_CHECK(src.ensure(t1 * 4));
         CPM_CHECK(db.ensure(t1 * 4));
         CPM_CHECK(ext_off.ensure(t1 * 4));
         CPM_CHECK(off2.ensure(t1 * 4));
@@ -1138,7 +1144,7 @@ struct LevelSet {
 
 struct GraphKey {
     const void *ptrs[64];
-    uint64_t caps[4];
+    uint64_t caps[4], slots;
     uint32_t G, te, tmask_w, tile_words, parts_tiles;
     int periodic, prune;
     bool operator==(const GraphKey &o) const { return memcmp(this, &o, sizeof(GraphKey)) == 0; }
@@ -1197,6 +1203,7 @@ struct Search {
     std::vector<uint32_t> nf;        // frequent items per database
     uint32_t parts_tiles = 0;
     Caps need{};                     // level 2's exact sizes when they overflowed
+    uint64_t slots = 0;              // retained rows per level set (0: none)
 
     Search(cpm_ctx *c, SearchWs &ws) : ctx(c), w(ws) {}
 
@@ -1270,6 +1277,11 @@ struct Search {
         a.te = te;
         a.stamps = w.stamps.as<unsigned long long>();
         a.siblings = 1;
+        if (slots) {
+            a.rin = w.set[c ^ 1].rows.as<uint64_t>();
+            a.rout = L.rows.as<uint64_t>();
+            a.slots = slots;
+        }
         return a;
     }
 
@@ -1310,7 +1322,7 @@ struct Search {
         auto add = [&](const DBuf &b) { k.ptrs[i++] = b.p; };
         for (int j = 0; j < 2; ++j) {
             const LevelSet &S = w.set[(c + j) & 1];
-            add(S.prefix); add(S.pid); add(S.par); add(S.db); add(S.ext_off); add(S.ext); add(S.sup); add(S.per); add(S.kept); add(S.off2); add(S.ext2); add(S.live);
+            add(S.prefix); add(S.pid); add(S.par); add(S.src); add(S.rows); add(S.db); add(S.ext_off); add(S.ext); add(S.sup); add(S.per); add(S.kept); add(S.off2); add(S.ext2); add(S.live);
             add(S.pos); add(S.parts);
             for (int q = 0; q < 3; ++q) { add(S.cnt[q]); add(S.off[q]); }
         }
@@ -1321,6 +1333,7 @@ struct Search {
         k.caps[1] = w.caps.ext;
         k.caps[2] = w.caps.patterns;
         k.caps[3] = w.caps.parts;
+        k.slots = slots;
         k.G = G;
         k.te = te;
         k.tmask_w = tmask_w;
@@ -1372,6 +1385,13 @@ struct Search {
 
     int reserve() {
         for (int i = 0; i < 2; ++i) CPM_CHECK(w.set[i].reserve(w.caps, periodic, parts_tiles));
+        // retained rows when a store has one-stride rows (counting only)
+        bool one_stride = false;
+        if (!periodic)
+            for (auto &d : in) one_stride = one_stride || d.bits->stride == kRetainWords;
+        slots = one_stride ? std::min<uint64_t>(w.caps.ext + 1, kRetainBytes / (kRetainWords * 8)) : 0;
+        if (slots)
+            for (int i = 0; i < 2; ++i) CPM_CHECK(w.set[i].rows.ensure(slots * kRetainWords * 8));
         CPM_CHECK(w.pat.ensure(4 * pat_pitch()));
         {
             void *before = w.filter.p;

@@ -55,11 +55,14 @@ __device__ __forceinline__ uint32_t col_of(cptr<uint32_t> fcol, uint32_t f) {
 // through the constant address space, so that the per-row lookups of a work
 // item (task offsets, prefix and entry items, columns) are scalar loads.
 struct LevelView {
-    cptr<uint32_t> ext_off, dbi, prefix, ext, fcol, par;
+    cptr<uint32_t> ext_off, dbi, prefix, ext, fcol, par, src;
     cptr<DbDev> dbs;
     uint32_t G, te, n_tasks, d, n_ext;
     int siblings;
     uint64_t total;
+    const uint64_t *rin;   // retained rows (SupportArgs)
+    uint64_t *rout;
+    uint64_t slots;
 };
 
 // false when the launch has nothing to do (the same answer in every block)
@@ -80,6 +83,10 @@ __device__ __forceinline__ bool level_view(const SupportArgs &a, LevelView &v) {
     v.te = a.te;
     v.siblings = a.siblings && a.L.par;
     v.par = v.siblings ? to_const(a.L.par) : nullptr;
+    v.rin = v.d >= 3 && a.L.src ? a.rin : nullptr;
+    v.src = v.rin ? to_const(a.L.src) : nullptr;
+    v.rout = a.rout;
+    v.slots = a.slots;
     return true;
 }
 
@@ -173,12 +180,28 @@ __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
             }
             auto row_of = [&](uint32_t f) { return store + (uint64_t)col_of(v.fcol, f) * stride_q; };
             const cptr<uint32_t> pre = v.prefix + (uint64_t)t * kMaxDepth;
+            // one-stride rows keep every counted itemset's row: read the
+            // prefix's from the previous launch when it has one
+            const bool kept_rows = db->stride == kRetainWords;
+            const uint32_t src = kept_rows && v.rin ? v.src[t] : ~0u;
+            const bool from_kept = src < v.slots;
+            uint4 P[QPL];
+            uint32_t rows_read = e1 - e0;
+            if (from_kept) {
+                const gptr<uint4> row = to_global(reinterpret_cast<const uint4 *>(v.rin)) + (uint64_t)src * stride_q;
+#pragma unroll
+                for (int q = 0; q < QPL; ++q) P[q] = row[off[q]];
+#pragma unroll
+                for (int q = 0; q < QPL; ++q)
+                    if (!valid[q]) P[q] = make_uint4(0, 0, 0, 0);
+                rows_read += 1;
+                pp_ok = false;
+            } else {
             // the parent PP was built for (siblings share the database and the
             // first d - 1 prefix items); a prefix of at most one item has
             // nothing to share
             const uint32_t par = v.siblings && v.d >= 2 ? v.par[t] : ~0u;
             const bool same = pp_ok && par != ~0u && par == pp_par;
-            uint32_t rows_read = e1 - e0;
             // the last prefix row first, so its load overlaps the parent's
             uint4 last[QPL];
 #pragma unroll
@@ -215,24 +238,37 @@ __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
             }
             pp_ok = true;
             pp_par = par;
-            uint4 P[QPL];
 #pragma unroll
             for (int q = 0; q < QPL; ++q) P[q] = and4(PP[q], last[q]);
-            bytes += 8ull * tile_words_of(db->words, r.tile, 2 * kQuadsPerTile) * rows_read;
+            }
+            // this launch's itemset rows, for the next level's prefixes
+            uint4 *const keep = reinterpret_cast<uint4 *>(v.rout);
+            const bool put = kept_rows && v.rout;
+            const uint64_t tile_w = tile_words_of(db->words, r.tile, 2 * kQuadsPerTile);
+            bytes += 8ull * tile_w * rows_read;
             uint32_t e = e0;
             for (; e + 1 < e1; e += 2) {
                 const gptr<uint4> ra = row_of(v.ext[e]), rb = row_of(v.ext[e + 1]);
                 uint4 A[QPL], B[QPL];
 #pragma unroll
                 for (int q = 0; q < QPL; ++q) {
-                    A[q] = ra[off[q]];
-                    B[q] = rb[off[q]];
+                    A[q] = and4(P[q], ra[off[q]]);
+                    B[q] = and4(P[q], rb[off[q]]);
                 }
                 uint32_t ca = 0, cb = 0;
 #pragma unroll
                 for (int q = 0; q < QPL; ++q) {
-                    ca += pop4(and4(P[q], A[q]));
-                    cb += pop4(and4(P[q], B[q]));
+                    ca += pop4(A[q]);
+                    cb += pop4(B[q]);
+                }
+                if (put) {
+#pragma unroll
+                    for (int q = 0; q < QPL; ++q)
+                        if (valid[q]) {
+                            if (e < v.slots) keep[(uint64_t)e * stride_q + off[q]] = A[q];
+                            if (e + 1 < v.slots) keep[(uint64_t)(e + 1) * stride_q + off[q]] = B[q];
+                        }
+                    bytes += 8ull * tile_w * ((e < v.slots) + (e + 1 < v.slots));
                 }
                 // one reduction for both: a wave's count is at most
                 // 64 lanes x 128 x QPL bits < 2^16 for QPL <= 4
@@ -245,9 +281,19 @@ __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
             }
             if (e < e1) {
                 const gptr<uint4> ra = row_of(v.ext[e]);
+                uint4 A[QPL];
                 uint32_t ca = 0;
 #pragma unroll
-                for (int q = 0; q < QPL; ++q) ca += pop4(and4(P[q], ra[off[q]]));
+                for (int q = 0; q < QPL; ++q) {
+                    A[q] = and4(P[q], ra[off[q]]);
+                    ca += pop4(A[q]);
+                }
+                if (put && e < v.slots) {
+#pragma unroll
+                    for (int q = 0; q < QPL; ++q)
+                        if (valid[q]) keep[(uint64_t)e * stride_q + off[q]] = A[q];
+                    bytes += 8ull * tile_w;
+                }
                 ca = wave_sum(ca);
                 if (lane == 0) part[wave][e - r.r0] = ca;
             }

@@ -10,6 +10,12 @@ namespace cpm {
 // search can reach is kMaxDepth + 1 items).
 constexpr uint32_t kMaxDepth = 64;
 
+// Databases whose rows are one 64-word stride (at most 4096 transactions)
+// keep each counted itemset's row, the AND of its items' rows, for the next
+// level: a task then reads its prefix as one row instead of d.
+constexpr uint32_t kRetainWords = 64;
+constexpr uint64_t kRetainBytes = 256ull << 20;   // per level set
+
 // One database of a search.  Frequent items of every database live in one
 // global index space; database i owns indexes [fbase, fbase + nf).
 struct DbDev {
@@ -44,6 +50,7 @@ struct LevelPtrs {
     uint32_t *prefix = nullptr;
     uint32_t *pid = nullptr;
     uint32_t *par = nullptr;  // parent task: its index in the previous level (levels >= 3)
+    uint32_t *src = nullptr;  // the previous level's entry whose itemset is the prefix (levels >= 3)
     uint32_t *db = nullptr;
     uint32_t *ext_off = nullptr;
     uint32_t *ext = nullptr;
@@ -88,6 +95,14 @@ struct SupportArgs {
     // L.par holds each task's parent: tasks with one parent share the AND
     // of their first d - 1 prefix rows
     int siblings = 0;
+    // Retained rows (kRetainWords words per slot, `slots` slots each; null:
+    // none).  rout[e] receives entry e's itemset row when its database's
+    // stride is kRetainWords and e < slots; a task of such a database whose
+    // prefix has 3 or more items reads its prefix row as rin[L.src[t]] (the
+    // previous launch's rout) when L.src[t] < slots.
+    const uint64_t *rin = nullptr;
+    uint64_t *rout = nullptr;
+    uint64_t slots = 0;
 };
 
 // Entries per work range for a level of n_ext entries: at most G, and small
