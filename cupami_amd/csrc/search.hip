@@ -201,6 +201,41 @@ __device__ __forceinline__ uint32_t pair_bit(const uint32_t *prow, uint32_t f) {
     return (prow[f >> 5] >> (f & 31)) & 1u;
 }
 
+// Child entry counts of a small task, one wave: wf[0 .. nk) are its kept
+// items (local indexes, ascending); lane j < nk - 1 gets the number of kept
+// items above wf[j] whose pair with wf[j] is kept.  The kept items are
+// grouped into runs sharing a pair-table word (rw[r], bits rm[r]), so a lane
+// reads one word of its pair row per run instead of one per item.  The whole
+// wave calls it.
+__device__ __forceinline__ uint32_t small_child_count(const uint32_t *wf, uint32_t *rw, uint32_t *rm, uint32_t nk,
+                                                      const uint32_t *table, uint32_t pair_w32) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t f = lane < nk ? wf[lane] : 0u;
+    const uint32_t q = f >> 5;
+    const bool head = lane < nk && (lane == 0 || (wf[lane - 1] >> 5) != q);
+    const unsigned long long heads = __ballot(head);
+    const uint32_t nrun = (uint32_t)__popcll(heads);
+    const uint32_t run = (uint32_t)__popcll(heads & ((2ull << lane) - 1)) - 1u;  // lane < nk: its run
+    if (head) {
+        rw[run] = q;
+        rm[run] = 0;
+    }
+    wave_sync();
+    if (lane < nk) atomicOr(&rm[run], 1u << (f & 31));
+    wave_sync();
+    uint32_t c = 0;
+    if (lane + 1 < nk) {
+        const uint32_t *prow = table + (uint64_t)f * pair_w32;
+        for (uint32_t r = run; r < nrun; ++r) {
+            uint32_t m = rm[r] & prow[rw[r]];
+            if (r == run) m &= above(f);
+            c += __popc(m);
+        }
+    }
+    wave_sync();  // the runs are read before the wave's next task rewrites them
+    return c;
+}
+
 // sum of part[0 .. b), in every lane of the block
 __device__ __forceinline__ unsigned long long sum_before(const unsigned long long *part, uint32_t b,
                                                          unsigned long long *red) {
@@ -277,7 +312,8 @@ __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
     __shared__ uint32_t lds[8];
     __shared__ unsigned long long red[4];
     __shared__ uint32_t lmask[kMaskWords];
-    __shared__ uint32_t wf[4][kSmallTask];  // per wave: the task's kept items
+    __shared__ uint32_t wf[4][kSmallTask];  // per wave: the task's kept items (local indexes)
+    __shared__ uint32_t rw[4][kSmallTask], rm[4][kSmallTask];  // per wave: their pair-table word runs
     if (p.stamps && blockIdx.x == gridDim.x - 1) fold_stamps(p.stamps, p.stamp_blocks, &p.ctl->acc);
     if (p.ctl->overflow) return;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -301,20 +337,15 @@ __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
         if (k) {
             const uint32_t pos = (uint32_t)__popcll(bal & ((1ull << lane) - 1));
             p.kept[a + pos] = e;
-            wf[wave][pos] = p.L.ext[e];
+            wf[wave][pos] = p.L.ext[e] - D.fbase;
         }
         wave_sync();
         const uint32_t nc = (p.emit_children && nk >= 2) ? nk - 1 : 0;
         uint32_t c = 0;
-        if (lane < nc) {
-            if (p.pairbits) {
-                const uint32_t fj = wf[wave][lane] - D.fbase;
-                const uint32_t *prow = p.pairbits + D.pair_off + (uint64_t)fj * D.pair_w32;
-                for (uint32_t l = lane + 1; l < nk; ++l) c += pair_bit(prow, wf[wave][l] - D.fbase);
-            } else {
-                c = nk - 1 - lane;
-            }
-        }
+        if (nc && p.pairbits)
+            c = small_child_count(wf[wave], rw[wave], rm[wave], nk, p.pairbits + D.pair_off, D.pair_w32);
+        else if (lane < nc)
+            c = nk - 1 - lane;
         uint32_t cext;
         (void)wave_scan(c, &cext);
         if (lane == 0) {
@@ -507,6 +538,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
     __shared__ uint32_t lds[8];
     __shared__ uint32_t lmask[kMaskWords];
     __shared__ uint32_t wf[4][kSmallTask];  // per wave: the task's kept items
+    __shared__ uint32_t rw[4][kSmallTask], rm[4][kSmallTask];  // per wave: their pair-table word runs
     __shared__ uint32_t spre[kMaxDepth];    // a large task's prefix
     if (p.ctl->overflow) return;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -573,13 +605,10 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
         const uint32_t fl = f - D.fbase;
         const uint32_t *prow = table && live ? table + (uint64_t)fl * D.pair_w32 : nullptr;
         uint32_t cj = 0;
-        if (live) {
-            if (table) {
-                for (uint32_t l = lane + 1; l < nk; ++l) cj += pair_bit(prow, wf[wave][l]);
-            } else {
-                cj = nk - 1 - lane;
-            }
-        }
+        if (table)
+            cj = small_child_count(wf[wave], rw[wave], rm[wave], nk, table, D.pair_w32);
+        else if (live)
+            cj = nk - 1 - lane;
         uint32_t tot;
         const uint32_t off = wave_scan(cj, &tot);
         // child c's prefix row: this task's prefix and item f of lane c, one
