@@ -357,10 +357,21 @@ __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
             s_cext += cext;
         }
     }
-    // large tasks: the whole block
-    for (uint32_t t = (uint32_t)lo; t < hi; ++t) {
+    // large tasks: the whole block.  The chunk's large tasks are found 256 at
+    // a time by all lanes (a scan by the block would wait on one load per task).
+    __shared__ uint32_t big[256];
+    __shared__ uint32_t n_big;
+    for (uint64_t t0 = lo; t0 < hi; t0 += 256) {
+    __syncthreads();  // every lane has read the previous n_big
+    if (tid == 0) n_big = 0;
+    __syncthreads();
+    if (t0 + tid < hi && p.L.ext_off[t0 + tid + 1] - p.L.ext_off[t0 + tid] > kSmallTask)
+        big[atomicAdd(&n_big, 1u)] = (uint32_t)(t0 + tid);
+    __syncthreads();
+    const uint32_t nb = n_big;
+    for (uint32_t bi = 0; bi < nb; ++bi) {
+        const uint32_t t = big[bi];
         const uint32_t a = p.L.ext_off[t], b = p.L.ext_off[t + 1];
-        if (b - a <= kSmallTask) continue;
         const DbDev &D = p.dbs[p.L.db[t]];
         uint32_t nk = 0;
         for (uint32_t i0 = a; i0 < b; i0 += 256) {
@@ -400,6 +411,7 @@ __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
             s_cext += cext;
         }
         __syncthreads();
+    }
     }
     s_pat = block_sum64(s_pat, red);
     s_child = block_sum64(s_child, red);
@@ -637,10 +649,20 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             }
         }
     }
-    // large tasks: the whole block
-    for (uint32_t t = blockIdx.x; t < n_tasks; t += gridDim.x) {
+    // large tasks: the whole block, found 256 at a time by all lanes
+    __shared__ uint32_t big[256];
+    __shared__ uint32_t n_big;
+    for (uint64_t k0 = 0; (uint64_t)blockIdx.x + k0 * gridDim.x < n_tasks; k0 += 256) {
+    __syncthreads();
+    if (tid == 0) n_big = 0;
+    __syncthreads();
+    const uint64_t tt = (uint64_t)blockIdx.x + (k0 + tid) * gridDim.x;
+    if (tt < n_tasks && p.cnt_pat[tt] > kSmallTask) big[atomicAdd(&n_big, 1u)] = (uint32_t)tt;
+    __syncthreads();
+    const uint32_t nb = n_big;
+    for (uint32_t bi = 0; bi < nb; ++bi) {
+        const uint32_t t = big[bi];
         const uint32_t nk = p.cnt_pat[t];
-        if (nk <= kSmallTask) continue;
         const uint32_t dbi = p.L.db[t];
         const DbDev &D = p.dbs[dbi];
         const uint32_t a = p.L.ext_off[t];
@@ -741,6 +763,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             mask_clear(lmask, p.kept, p.L.ext, a, nk, D.fbase);
             __syncthreads();
         }
+    }
     }
 }
 
