@@ -346,7 +346,8 @@ namespace cpm {
 constexpr size_t kBounceBytes = 8u << 20;
 
 // ---- host block cache ------------------------------------------------------
-// A large block carries a 64-byte header with its capacity.  The cache keeps
+// A large block is pinned host memory (plain memory if pinning fails) with a
+// 64-byte header: its capacity and whether it is pinned.  The cache keeps
 // up to kHostCacheBlocks blocks and kHostCacheBytes in all (held until the
 // process ends); a request takes the smallest cached block that fits and is
 // at most twice its size.
@@ -377,9 +378,18 @@ void *host_block_take(size_t bytes) {
             return static_cast<unsigned char *>(b) + kHostHeader;
         }
     }
-    void *b = malloc(bytes + kHostHeader);
+    // pinned, so that results are copied from the device straight into
+    // them; plain memory when pinning fails
+    void *b = nullptr;
+    size_t pinned = 1;
+    if (hipHostMalloc(&b, bytes + kHostHeader, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        b = malloc(bytes + kHostHeader);
+        pinned = 0;
+    }
     if (!b) return nullptr;
-    *static_cast<size_t *>(b) = bytes;
+    static_cast<size_t *>(b)[0] = bytes;
+    static_cast<size_t *>(b)[1] = pinned;
     return static_cast<unsigned char *>(b) + kHostHeader;
 }
 
@@ -399,7 +409,8 @@ void host_block_give(void *p, size_t bytes) {
             return;
         }
     }
-    free(b);
+    if (static_cast<size_t *>(b)[1]) (void)hipHostFree(b);
+    else free(b);
 }
 
 hipEvent_t take_event(cpm_ctx *ctx) {

@@ -164,9 +164,10 @@ struct cpm_bits {
 };
 
 namespace cpm {
-// Large host blocks are recycled through a small process-wide cache, so a
-// search's result arrays land in memory that is already mapped (fresh pages
-// cost the kernel a zeroing pass: ~9 ms per 120 MB).  Blocks under 1 MiB go
+// Large host blocks are pinned and recycled through a small process-wide
+// cache, so a search's result arrays are copied from the device straight
+// into memory that is already mapped (fresh pages cost the kernel a zeroing
+// pass: ~9 ms per 120 MB, and pinning costs more).  Blocks under 1 MiB go
 // straight to malloc / free.
 void *host_block_take(size_t bytes);
 void host_block_give(void *p, size_t bytes);  // bytes: as passed to host_block_take

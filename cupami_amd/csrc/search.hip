@@ -33,8 +33,6 @@
 #include <string.h>
 #include <algorithm>
 #include <mutex>
-#include <system_error>
-#include <thread>
 
 namespace cpm {
 
@@ -43,7 +41,6 @@ namespace {
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kMaskWords = 2048;   // LDS kept mask: up to 65536 frequent items per database
 constexpr uint64_t kTaskMaskBudget = 256ull << 20;  // bytes of per-task kept masks
-constexpr uint64_t kParallelCopyOut = 1 << 20;      // patterns from which the host copy-out uses threads
 
 enum Overflow : uint32_t {
     kOvTasks = 1, kOvEntries = 2, kOvPatterns = 4, kOvDepth = 8, kOvParts = 32,
@@ -1126,6 +1123,15 @@ __global__ void k_pair_table(PlanArgs p, uint32_t *bits) {
 // `bytes` (a multiple of 4) from the blob at `src` to `dst`, or zeroes them
 // when src == kZeroFill.
 constexpr uint64_t kZeroFill = ~0ull;
+__global__ void k_widen_u32(const uint32_t *__restrict__ in, uint64_t n, unsigned long long *__restrict__ out) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
+inline dim3 grid_of_n(uint64_t n) {
+    return dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 16384)));
+}
+
 struct PutRec {
     uint32_t *dst;
     uint64_t src;
@@ -1212,7 +1218,7 @@ struct SearchWs {
     HBuf host;
     HBuf stage;           // pinned staging of small host <-> device copies
     size_t stage_off = 0; // in use since the last stream sync
-    HBuf down;            // pinned landing area of the pattern download
+    DBuf sup64;           // supports widened to u64 for the download
     Caps caps{};
     bool graphs_ok[2] = {false, false};
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -1862,29 +1868,9 @@ struct Search {
         p->n = n;
         p->periodic = periodic;
         hipStream_t st = ctx->stream;
-        const unsigned char *h = nullptr;
-        if (n) {
-            const size_t row = periodic ? 24 : 16;
-            int sc = w.down.ensure(n * row);
-            if (sc != CPM_OK) {
-                delete p;
-                return sc;
-            }
-            unsigned char *hd = w.down.as<unsigned char>();
-            // parent, item, support and database columns in one strided copy
-            hipError_t e = hipMemcpy2DAsync(hd, n * 4, w.pat.p, pat_pitch(), n * 4, 4, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess && periodic)
-                e = hipMemcpyAsync(hd + n * 16, w.pat_per.p, n * 8, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
-            if (e != hipSuccess) {
-                delete p;
-                return fail(CPM_ERR_RUNTIME, "pattern download failed: %s", hipGetErrorString(e));
-            }
-            h = hd;
-        }
         try {
-            // the result's arrays come from recycled host blocks and are not
-            // cleared first: every element is written below
+            // the result's arrays come from recycled (pinned) host blocks and
+            // are not cleared first: the copies below write every element
             p->parent.resize(n);
             p->item.resize(n);
             p->support.resize(n);
@@ -1895,32 +1881,27 @@ struct Search {
             return fail(CPM_ERR_NOMEM, "host memory for %llu patterns", (unsigned long long)n);
         }
         if (n) {
-            const uint32_t *hp = reinterpret_cast<const uint32_t *>(h);
-            // independent column copies: on four threads for large results
-            auto widen = [&](uint64_t lo, uint64_t hi) {
-                for (uint64_t i = lo; i < hi; ++i) p->support[i] = hp[2 * n + i];
-            };
-            auto parts = [&](int k) {
-                switch (k) {
-                case 0: memcpy(p->parent.data(), hp, n * 4); break;
-                case 1: memcpy(p->item.data(), hp + n, n * 4); break;
-                case 2: memcpy(p->db.data(), hp + 3 * n, n * 4); break;
-                default:
-                    widen(0, n);
-                    if (periodic) memcpy(p->period.data(), h + n * 16, n * 8);
-                }
-            };
-            std::thread helpers[3];
-            int started = 0;
-            if (n >= kParallelCopyOut) {
-                try {
-                    for (; started < 3; ++started) helpers[started] = std::thread(parts, started);
-                } catch (const std::system_error &) {
-                    // no thread: this one does the rest
-                }
+            // supports widened on the device, then every column straight
+            // into the result's arrays, one wait
+            int sc = w.sup64.ensure(n * 8);
+            if (sc != CPM_OK) {
+                delete p;
+                return sc;
             }
-            for (int k = started; k < 4; ++k) parts(k);
-            for (int k = 0; k < started; ++k) helpers[k].join();
+            hipLaunchKernelGGL(k_widen_u32, grid_of_n(n), dim3(256), 0, st, pat_col(2), n,
+                               w.sup64.as<unsigned long long>());
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess) e = hipMemcpyAsync(p->parent.data(), pat_col(0), n * 4, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(p->item.data(), pat_col(1), n * 4, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(p->db.data(), pat_col(3), n * 4, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(p->support.data(), w.sup64.p, n * 8, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess && periodic)
+                e = hipMemcpyAsync(p->period.data(), w.pat_per.p, n * 8, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                delete p;
+                return fail(CPM_ERR_RUNTIME, "pattern download failed: %s", hipGetErrorString(e));
+            }
         }
         // statistics
         int wall_khz = 0;
