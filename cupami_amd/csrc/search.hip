@@ -68,6 +68,7 @@ struct PlanArgs {
     const uint32_t *fcol;       // null at level 1 (entries are columns)
     const uint32_t *pairbits;   // pair tables, null for none
     uint32_t *kept, *cnt_pat, *cnt_child, *cnt_cext, *off_pat, *off_child, *off_cext;
+    uint32_t *ccount;           // child entries of each kept entry (kept order; with a pair table)
     unsigned long long *filter; // Bloom filter of this level's itemsets for the subset test
     uint32_t *off2, *ext2, *live, *pos;  // next level's staging for the subset test
     unsigned long long *part;   // plan/count chunk sums: 3 x (plan grid)
@@ -202,10 +203,11 @@ __device__ __forceinline__ uint32_t pair_bit(const uint32_t *prow, uint32_t f) {
 // items (local indexes, ascending); lane j < nk - 1 gets the number of kept
 // items above wf[j] whose pair with wf[j] is kept.  The kept items are
 // grouped into runs sharing a pair-table word (rw[r], bits rm[r]), so a lane
-// reads one word of its pair row per run instead of one per item.  The whole
+// reads one word of its pair row per run instead of one per item.  When trow
+// is not null it receives the task's kept mask (pair_w32 words).  The whole
 // wave calls it.
 __device__ __forceinline__ uint32_t small_child_count(const uint32_t *wf, uint32_t *rw, uint32_t *rm, uint32_t nk,
-                                                      const uint32_t *table, uint32_t pair_w32) {
+                                                      const uint32_t *table, uint32_t pair_w32, uint32_t *trow) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t f = lane < nk ? wf[lane] : 0u;
     const uint32_t q = f >> 5;
@@ -220,6 +222,15 @@ __device__ __forceinline__ uint32_t small_child_count(const uint32_t *wf, uint32
     wave_sync();
     if (lane < nk) atomicOr(&rm[run], 1u << (f & 31));
     wave_sync();
+    if (trow)
+        for (uint32_t w = lane; w < pair_w32; w += 64) {
+            uint32_t lo = 0, hi = nrun;  // the run of word w, if any
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (rw[mid] < w) lo = mid + 1; else hi = mid;
+            }
+            trow[w] = lo < nrun && rw[lo] == w ? rm[lo] : 0u;
+        }
     uint32_t c = 0;
     if (lane + 1 < nk) {
         const uint32_t *prow = table + (uint64_t)f * pair_w32;
@@ -339,10 +350,13 @@ __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
         wave_sync();
         const uint32_t nc = (p.emit_children && nk >= 2) ? nk - 1 : 0;
         uint32_t c = 0;
-        if (nc && p.pairbits)
-            c = small_child_count(wf[wave], rw[wave], rm[wave], nk, p.pairbits + D.pair_off, D.pair_w32);
-        else if (lane < nc)
+        if (nc && p.pairbits) {
+            uint32_t *trow = p.tmask ? p.tmask + (uint64_t)t * p.tmask_w : nullptr;
+            c = small_child_count(wf[wave], rw[wave], rm[wave], nk, p.pairbits + D.pair_off, D.pair_w32, trow);
+            if (lane < nc) p.ccount[a + lane] = c;
+        } else if (lane < nc) {
             c = nk - 1 - lane;
+        }
         uint32_t cext;
         (void)wave_scan(c, &cext);
         if (lane == 0) {
@@ -386,11 +400,16 @@ __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
             if (p.pairbits) {
                 mask_set(lmask, p.kept, p.L.ext, a, nk, D.fbase);
                 __syncthreads();
+                if (p.tmask)
+                    for (uint32_t q = tid; q < D.pair_w32; q += blockDim.x)
+                        p.tmask[(uint64_t)t * p.tmask_w + q] = lmask[q];
                 const uint32_t wlast = (p.L.ext[p.kept[a + nk - 1]] - D.fbase) >> 5;
                 const uint32_t *table = p.pairbits + D.pair_off;
                 for (uint32_t j = tid; j + 1 < nk; j += 256) {
                     const uint32_t f = p.L.ext[p.kept[a + j]] - D.fbase;
-                    c += child_count(lmask, table + (uint64_t)f * D.pair_w32, f, wlast);
+                    const uint32_t cj = child_count(lmask, table + (uint64_t)f * D.pair_w32, f, wlast);
+                    p.ccount[a + j] = cj;
+                    c += cj;
                 }
             } else {
                 for (uint32_t j = tid; j + 1 < nk; j += 256) c += nk - 1 - j;
@@ -547,7 +566,6 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
     __shared__ uint32_t lds[8];
     __shared__ uint32_t lmask[kMaskWords];
     __shared__ uint32_t wf[4][kSmallTask];  // per wave: the task's kept items
-    __shared__ uint32_t rw[4][kSmallTask], rm[4][kSmallTask];  // per wave: their pair-table word runs
     __shared__ uint32_t spre[kMaxDepth];    // a large task's prefix
     if (p.ctl->overflow) return;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -593,31 +611,19 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             if (stage) filter_insert(p.filter, fmask, set_hash(psum + item_term(f)));
         }
         if (!(children && nk >= 2)) continue;
-        wave_sync();  // the previous task's reads of wf are done
-        if (lane < nk) wf[wave][lane] = f - D.fbase;
-        wave_sync();
-        if (fill) {
-            // kept-item mask, word q: the run of kept items in that word
-            for (uint32_t q = lane; q < D.pair_w32; q += 64) {
-                uint32_t lo = 0, hi = nk;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if ((wf[wave][mid] >> 5) < q) lo = mid + 1; else hi = mid;
-                }
-                uint32_t m = 0;
-                for (uint32_t l = lo; l < nk && (wf[wave][l] >> 5) == q; ++l) m |= 1u << (wf[wave][l] & 31);
-                p.tmask[(uint64_t)t * p.tmask_w + q] = m;
-            }
+        // plan/count left each kept entry's child count (and the kept mask
+        // plan/fill reads); the entries are written here only without a mask
+        if (!fill) {
+            wave_sync();  // the previous task's reads of wf are done
+            if (lane < nk) wf[wave][lane] = f - D.fbase;
+            wave_sync();
         }
         const uint32_t *table = p.pairbits ? p.pairbits + D.pair_off : nullptr;
         const bool live = lane + 1 < nk;
         const uint32_t fl = f - D.fbase;
         const uint32_t *prow = table && live ? table + (uint64_t)fl * D.pair_w32 : nullptr;
         uint32_t cj = 0;
-        if (table)
-            cj = small_child_count(wf[wave], rw[wave], rm[wave], nk, table, D.pair_w32);
-        else if (live)
-            cj = nk - 1 - lane;
+        if (live) cj = table ? p.ccount[a + lane] : nk - 1 - lane;
         uint32_t tot;
         const uint32_t off = wave_scan(cj, &tot);
         // child c's prefix row: this task's prefix and item f of lane c, one
@@ -684,15 +690,15 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             if (stage) filter_insert(p.filter, fmask, set_hash(psum + item_term(f)));
         }
         if (!(children && nk >= 2)) continue;
-        if (p.pairbits) {
+        // the kept mask only for writing the entries here (plan/count wrote
+        // the child counts and plan/fill's masks)
+        const bool inline_mask = p.pairbits && !fill;
+        if (inline_mask) {
             mask_set(lmask, p.kept, p.L.ext, a, nk, D.fbase);
             __syncthreads();
-            if (fill)
-                for (uint32_t q = tid; q < D.pair_w32; q += blockDim.x)
-                    p.tmask[(uint64_t)t * p.tmask_w + q] = lmask[q];
         }
         const uint32_t *table = p.pairbits ? p.pairbits + D.pair_off : nullptr;
-        const uint32_t wlast = (p.L.ext[p.kept[a + nk - 1]] - D.fbase) >> 5;
+        const uint32_t wlast = inline_mask ? (p.L.ext[p.kept[a + nk - 1]] - D.fbase) >> 5 : 0u;
         const uint32_t cb = p.off_child[t];
         const uint32_t xb = p.off_cext[t];
         uint32_t carry = 0;
@@ -706,7 +712,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
                 fj = p.L.ext[ej];
                 if (table) {
                     prow = table + (uint64_t)(fj - D.fbase) * D.pair_w32;
-                    cj = child_count(lmask, prow, fj - D.fbase, wlast);
+                    cj = p.ccount[a + j];
                 } else {
                     cj = nk - 1 - j;
                 }
@@ -755,7 +761,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
             }
             carry += tot;
         }
-        if (p.pairbits) {
+        if (inline_mask) {
             __syncthreads();
             mask_clear(lmask, p.kept, p.L.ext, a, nk, D.fbase);
             __syncthreads();
@@ -1157,7 +1163,8 @@ __global__ __launch_bounds__(256) void k_scatter(const PutRec *__restrict__ recs
 struct LevelSet {
     DBuf prefix, pid, par, src, db, ext_off, ext, sup, per;
     DBuf kept, cnt[3], off[3], off2, ext2, live, pos, parts;
-    DBuf rows;  // retained itemset rows of this set's entries
+    DBuf rows;    // retained itemset rows of this set's entries
+    DBuf ccount;  // child entries of each kept entry
     LevelPtrs ptrs(LevelSizes *sz) const {
         LevelPtrs p;
         p.prefix = prefix.as<uint32_t>();
@@ -1189,6 +1196,7 @@ struct LevelSet {
         CPM_CHECK(ext2.ensure(e1 * 4));
         CPM_CHECK(sup.ensure(e1 * 4));
         CPM_CHECK(kept.ensure(e1 * 4));
+        CPM_CHECK(ccount.ensure(e1 * 4));
         CPM_CHECK(live.ensure(e1 * 4));
         CPM_CHECK(pos.ensure(e1 * 4));
         if (periodic) {
@@ -1289,6 +1297,7 @@ struct Search {
         p.fcol = emit_children ? w.fcol.as<uint32_t>() : nullptr;
         p.pairbits = emit_children && w.pairbits.p ? w.pairbits.as<uint32_t>() : nullptr;
         p.kept = L.kept.as<uint32_t>();
+        p.ccount = L.ccount.as<uint32_t>();
         p.cnt_pat = L.cnt[0].as<uint32_t>();
         p.cnt_child = L.cnt[1].as<uint32_t>();
         p.cnt_cext = L.cnt[2].as<uint32_t>();
@@ -1380,7 +1389,7 @@ struct Search {
         auto add = [&](const DBuf &b) { k.ptrs[i++] = b.p; };
         for (int j = 0; j < 2; ++j) {
             const LevelSet &S = w.set[(c + j) & 1];
-            add(S.prefix); add(S.pid); add(S.par); add(S.src); add(S.rows); add(S.db); add(S.ext_off); add(S.ext); add(S.sup); add(S.per); add(S.kept); add(S.off2); add(S.ext2); add(S.live);
+            add(S.prefix); add(S.pid); add(S.par); add(S.src); add(S.rows); add(S.ccount); add(S.db); add(S.ext_off); add(S.ext); add(S.sup); add(S.per); add(S.kept); add(S.off2); add(S.ext2); add(S.live);
             add(S.pos); add(S.parts);
             for (int q = 0; q < 3; ++q) { add(S.cnt[q]); add(S.off[q]); }
         }
