@@ -128,38 +128,24 @@ struct BuildJob {
     const uint32_t *cols;
     uint64_t nnz, n_trans, stride;
     uint64_t *store;        // tile build only
-    unsigned int *flags;    // [0] rows descend, [1] last row out of range, [2] bad pair
+    unsigned int *flags;    // [0] rows not sorted into the store's range, [2] a column out of range
     uint32_t n_items, B;    // B: words per workgroup of the tile build
     uint64_t block0;        // first workgroup of this database in the tile launch
 };
-
-// Is rows[] non-decreasing, and does every pair lie inside the store?
-// Raises flags[0] when a descent is found, and flags[1] when the last row is
-// >= n_trans (for sorted rows the last is the largest) or a column is >=
-// n_items.  blockIdx.y picks the database.
-__global__ __launch_bounds__(256) void k_rows_sorted(const BuildJob *__restrict__ jobs) {
-    const BuildJob &j = jobs[blockIdx.y];
-    const uint64_t *__restrict__ rows = j.rows;
-    const uint32_t *__restrict__ cols = j.cols;
-    const uint64_t nnz = j.nnz;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && nnz && rows[nnz - 1] >= j.n_trans) atomicOr(j.flags + 1, 1u);
-    bool bad_col = false;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        bad_col |= cols[i] >= j.n_items;
-        if (i + 1 < nnz && rows[i] > rows[i + 1]) {
-            atomicOr(j.flags, 1u);  // unsorted: the scatter build checks its own pairs
-            return;
-        }
-    }
-    if (bad_col) atomicOr(j.flags + 1, 1u);
-}
 
 // Store build for row-sorted pairs, any number of databases per launch:
 // workgroup b of database d owns words [b*B, b*B+B) of every item row of d.
 // Its pairs are one contiguous run of the input; they are ORed into an LDS
 // tile (n_items x B words) and the tile is written out once, so the store
 // needs no clearing and HBM sees no atomics.
+//
+// The kernel checks its own input instead of a separate pass over it: each
+// workgroup tests that its run is non-decreasing (its last pair against the
+// next run's first too) and that its rows fall in its word range, and the
+// last workgroup that the runs end at nnz.  The runs of consecutive
+// workgroups meet (each end is the same search), so when no workgroup raises
+// flags[0] the rows are sorted and below n_trans and the store is right;
+// otherwise the caller builds that store again by scatter.
 // First i in [0, n) with rows[i] >= x (n if none), for non-decreasing rows,
 // by one wave: each step probes 64 positions that split the interval into 65
 // parts, so 10^7 rows take four dependent loads.
@@ -206,9 +192,12 @@ __global__ __launch_bounds__(512) void k_bits_tile(const BuildJob *__restrict__ 
     }
     __syncthreads();
     const uint64_t lo = range[0], lo2 = range[1];
+    const bool last_block = blk + 1 == j.block0 + (stride + B - 1) / B;
+    bool unsorted = lo2 < lo || (last_block && lo2 != nnz);
     constexpr int kPairBatch = 4;  // loads in flight per lane before its LDS atomics
+    const uint32_t lane = tid & 63;
     for (uint64_t i0 = lo; i0 < lo2; i0 += (uint64_t)kPairBatch * blockDim.x) {
-        uint64_t r[kPairBatch];
+        uint64_t r[kPairBatch], nx[kPairBatch];
         uint32_t c[kPairBatch];
         bool v[kPairBatch];
 #pragma unroll
@@ -217,17 +206,29 @@ __global__ __launch_bounds__(512) void k_bits_tile(const BuildJob *__restrict__ 
             v[q] = i < lo2;
             r[q] = v[q] ? rows[i] : 0;
             c[q] = v[q] ? cols[i] : 0;
+            // the next row: the next lane's, or loaded at the end of a wave or of the run
+            const bool own = v[q] && (lane == 63 || i + 1 == lo2) && i + 1 < nnz;
+            nx[q] = own ? rows[i + 1] : ~0ull;
         }
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) {
+            const uint64_t i = i0 + (uint64_t)q * blockDim.x + tid;
+            const uint64_t down = __shfl_down(r[q], 1, 64);
+            const uint64_t next = (lane == 63 || i + 1 == lo2) ? nx[q] : down;
             if (!v[q]) continue;
+            const uint64_t wr = (r[q] >> 6) - w0;
+            if (r[q] > next || wr >= B) {
+                unsorted = true;
+                continue;
+            }
             if (c[q] >= n_items) {
                 atomicOr(j.flags + 2, 1u);
                 continue;
             }
-            atomicOr(&tile[c[q] * B + (uint32_t)((r[q] >> 6) - w0)], 1ull << (r[q] & 63));
+            atomicOr(&tile[c[q] * B + (uint32_t)wr], 1ull << (r[q] & 63));
         }
     }
+    if (unsorted) atomicOr(j.flags, 1u);
     __syncthreads();
     uint64_t *__restrict__ store = j.store;
     for (uint32_t i = tid; i < cells; i += blockDim.x) {
@@ -510,11 +511,11 @@ struct BuildIn {
     uint32_t n_items;
 };
 
-// Build n stores from device COO.  The sortedness and range checks of the
-// whole batch are read back in one copy, so a batch costs one host
-// synchronization whatever its size (two when a store has unsorted rows: the
-// scatter build's own range check is read back at the end).  batch = true
-// puts all stores in one arena.
+// Build n stores from device COO.  Every store with few enough items takes
+// the tile build, which checks its own input; the flags of the whole batch are
+// read back in one copy, so a batch costs one host synchronization whatever
+// its size (two when a store's rows turn out unsorted and it is built again
+// by scatter).  batch = true puts all stores in one arena.
 int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bits **out) {
     for (uint32_t i = 0; i < n; ++i) {
         out[i] = nullptr;
@@ -533,40 +534,12 @@ int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bi
     unsigned int *hflags = ctx->host_scratch.as<unsigned int>();
     BuildJob *h_jobs = reinterpret_cast<BuildJob *>(ctx->host_scratch.as<unsigned char>() + flag_bytes);
 
-    // 1. sortedness and range of every store's rows (one launch)
-    uint32_t n_check = 0;
-    uint64_t max_nnz = 0;
-    std::vector<uint32_t> check_of(n, 0);
-    for (uint32_t i = 0; i < n; ++i) {
-        if (!in[i].nnz || !tile_words_for(in[i].n_items)) continue;
-        BuildJob &j = h_jobs[n_check];
-        memset(&j, 0, sizeof(j));
-        j.rows = in[i].rows;
-        j.cols = in[i].cols;
-        j.nnz = in[i].nnz;
-        j.n_trans = in[i].n_trans;
-        j.n_items = in[i].n_items;
-        j.flags = flags + 3 * i;
-        check_of[i] = n_check++;
-        max_nnz = std::max(max_nnz, in[i].nnz);
-    }
-    CPM_HIP(hipMemsetAsync(flags, 0, (size_t)3 * n * sizeof(unsigned int), st));
-    if (n_check) {
-        CPM_HIP(hipMemcpyAsync(d_jobs, h_jobs, (size_t)n_check * sizeof(BuildJob), hipMemcpyHostToDevice, st));
-        const uint64_t gx = std::max<uint64_t>(1, std::min<uint64_t>((max_nnz + 255) / 256,
-                                                                    (uint64_t)ctx->cu_count * 16 / std::max(1u, n_check) + 1));
-        hipLaunchKernelGGL(k_rows_sorted, dim3((unsigned)gx, n_check), dim3(256), 0, st, d_jobs);
-        CPM_HIP(hipGetLastError());
-    }
-    CPM_HIP(hipMemcpyAsync(hflags, flags, (size_t)3 * n * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-    CPM_HIP(hipStreamSynchronize(st));
+    // 1. which stores take the tile build: all with pairs and few enough
+    // items for the LDS tile; the tile kernel checks the rows are sorted, and
+    // a store whose rows are not is built again by scatter (step 4)
     std::vector<char> tiled(n, 0);
-    for (uint32_t i = 0; i < n; ++i) {
-        tiled[i] = in[i].nnz && tile_words_for(in[i].n_items) && hflags[3 * i] == 0;
-        if (tiled[i] && hflags[3 * i + 1])
-            return fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
-                        (unsigned long long)in[i].n_trans, in[i].n_items);
-    }
+    for (uint32_t i = 0; i < n; ++i) tiled[i] = in[i].nnz && tile_words_for(in[i].n_items);
+    CPM_HIP(hipMemsetAsync(flags, 0, (size_t)3 * n * sizeof(unsigned int), st));
     // 2. allocation
     std::shared_ptr<StoreArena> arena;
     std::vector<uint64_t> offset(n, 0);
@@ -654,15 +627,42 @@ int build_stores(cpm_ctx *ctx, const BuildIn *in, uint32_t n, bool batch, cpm_bi
     hipError_t e = hipGetLastError();
     if (status == CPM_OK && e != hipSuccess)
         status = fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(e));
-    // tile builds had their range checked in step 1: only scatter builds
-    // report out-of-range pairs afterwards
-    if (status == CPM_OK && scattered) {
-        e = hipMemcpyAsync(hflags, flags, (size_t)3 * n * sizeof(unsigned int), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) status = fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(e));
+    // 4. the flags of every store, one read-back: unsorted rows are built
+    // again by scatter (one more read-back for its range check)
+    auto read_flags = [&]() {
+        hipError_t r = hipMemcpyAsync(hflags, flags, (size_t)3 * n * sizeof(unsigned int), hipMemcpyDeviceToHost, st);
+        if (r == hipSuccess) r = hipStreamSynchronize(st);
+        if (r != hipSuccess) status = fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(r));
+    };
+    bool any_tiled = false;
+    for (uint32_t i = 0; i < n; ++i) any_tiled = any_tiled || tiled[i];
+    if (status == CPM_OK && (any_tiled || scattered)) read_flags();
+    bool rebuilt = false;
+    for (uint32_t i = 0; i < n && status == CPM_OK; ++i) {
+        if (!tiled[i] || !hflags[3 * i]) continue;
+        const size_t bytes = (size_t)out[i]->n_items * out[i]->stride * 8;
+        if (hipMemsetAsync(out[i]->rows(), 0, bytes, st) != hipSuccess) {
+            status = fail(CPM_ERR_RUNTIME, "hipMemsetAsync failed");
+            break;
+        }
+        hflags[3 * i + 2] = 0;
+        if (hipMemsetAsync(flags + 3 * i + 2, 0, sizeof(unsigned int), st) != hipSuccess) {
+            status = fail(CPM_ERR_RUNTIME, "hipMemsetAsync failed");
+            break;
+        }
+        const uint64_t g = std::min<uint64_t>((in[i].nnz + 255) / 256, (uint64_t)ctx->cu_count * 32);
+        hipLaunchKernelGGL(k_bits_scatter, dim3((unsigned)g), dim3(256), 0, st, in[i].rows, in[i].cols,
+                           in[i].nnz, in[i].n_trans, in[i].n_items, out[i]->stride,
+                           reinterpret_cast<unsigned long long *>(out[i]->rows()), flags + 3 * i + 2);
+        rebuilt = true;
     }
-    for (uint32_t i = 0; i < n && status == CPM_OK && scattered; ++i)
-        if (!tiled[i] && hflags[3 * i + 2])
+    if (status == CPM_OK && rebuilt) {
+        e = hipGetLastError();
+        if (e != hipSuccess) status = fail(CPM_ERR_RUNTIME, "store build failed: %s", hipGetErrorString(e));
+        else read_flags();
+    }
+    for (uint32_t i = 0; i < n && status == CPM_OK; ++i)
+        if (in[i].nnz && hflags[3 * i + 2])
             status = fail(CPM_ERR_INVALID, "a (row, column) pair lies outside %llu x %u",
                           (unsigned long long)in[i].n_trans, in[i].n_items);
     if (status != CPM_OK) {

@@ -39,6 +39,28 @@ def test_bits_build(ctx, n, row_sorted):
     assert np.array_equal(bits.download(), oracle.bits_build(rows, cols, n, 7))
 
 
+def test_bits_build_single_descent(ctx):
+    # sorted rows with one descent: at the start, at the end, and where the
+    # tile build's workgroups meet (512 rows apart for 7 items), so the
+    # check that a workgroup's last pair precedes the next run's first is what
+    # finds it; the store must come out as a scatter build gives it
+    n = 100_003
+    rows, cols = random_store(7, n, 7, 0.3)
+    order = np.argsort(rows, kind="stable")
+    rows, cols = rows[order], cols[order]
+    starts = np.searchsorted(rows, np.arange(0, n, 512))
+    picks = [0, rows.size - 2] + [int(b) - 1 for b in starts[1:40:7]] + [int(starts[100])]
+    steps = np.flatnonzero(rows[1:] != rows[:-1])  # j with rows[j] < rows[j + 1]
+    for i in picks:
+        r, c = rows.copy(), cols.copy()
+        j = int(steps[min(np.searchsorted(steps, i), steps.size - 1)])
+        r[j], r[j + 1] = r[j + 1], r[j]
+        c[j], c[j + 1] = c[j + 1], c[j]
+        bits = ctx.build_bits(r, c, n, 7)
+        assert np.array_equal(bits.download(), oracle.bits_build(r, c, n, 7)), i
+        bits.close()
+
+
 def test_bits_build_many_items_and_reuse(ctx):
     # item counts on both sides of the LDS tile limit, store buffers reused
     for items in (1, 1000, 4096, 4097, 7168, 7169, 9000):
