@@ -41,6 +41,10 @@ namespace {
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kMaskWords = 2048;   // LDS kept mask: up to 65536 frequent items per database
 constexpr uint64_t kTaskMaskBudget = 256ull << 20;  // bytes of per-task kept masks
+// Results whose columns are host blocks of at least this many bytes (pinned,
+// recycled) are copied from the device straight into them; smaller ones go
+// through a pinned stage, as plain host memory is slow to copy into.
+constexpr uint64_t kDirectDownload = 1 << 20;
 
 enum Overflow : uint32_t {
     kOvTasks = 1, kOvEntries = 2, kOvPatterns = 4, kOvDepth = 8, kOvParts = 32,
@@ -1226,7 +1230,8 @@ struct SearchWs {
     HBuf host;
     HBuf stage;           // pinned staging of small host <-> device copies
     size_t stage_off = 0; // in use since the last stream sync
-    DBuf sup64;           // supports widened to u64 for the download
+    DBuf sup64;           // supports widened to u64 for the download of large results
+    HBuf down;            // pinned stage of the download of small results
     Caps caps{};
     bool graphs_ok[2] = {false, false};
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -1889,9 +1894,9 @@ struct Search {
             delete p;
             return fail(CPM_ERR_NOMEM, "host memory for %llu patterns", (unsigned long long)n);
         }
-        if (n) {
-            // supports widened on the device, then every column straight
-            // into the result's arrays, one wait
+        if (n && n * 4 >= kDirectDownload) {
+            // large results: supports widened on the device, then every
+            // column straight into the result's (pinned) arrays, one wait
             int sc = w.sup64.ensure(n * 8);
             if (sc != CPM_OK) {
                 delete p;
@@ -1911,6 +1916,30 @@ struct Search {
                 delete p;
                 return fail(CPM_ERR_RUNTIME, "pattern download failed: %s", hipGetErrorString(e));
             }
+        } else if (n) {
+            // small results (plain host memory): the four columns in one
+            // strided copy to the pinned stage, then copied out
+            const size_t row = periodic ? 24 : 16;
+            int sc = w.down.ensure(n * row);
+            if (sc != CPM_OK) {
+                delete p;
+                return sc;
+            }
+            unsigned char *hd = w.down.as<unsigned char>();
+            hipError_t e = hipMemcpy2DAsync(hd, n * 4, w.pat.p, pat_pitch(), n * 4, 4, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess && periodic)
+                e = hipMemcpyAsync(hd + n * 16, w.pat_per.p, n * 8, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                delete p;
+                return fail(CPM_ERR_RUNTIME, "pattern download failed: %s", hipGetErrorString(e));
+            }
+            const uint32_t *hp = reinterpret_cast<const uint32_t *>(hd);
+            memcpy(p->parent.data(), hp, n * 4);
+            memcpy(p->item.data(), hp + n, n * 4);
+            memcpy(p->db.data(), hp + 3 * n, n * 4);
+            for (uint64_t i = 0; i < n; ++i) p->support[i] = hp[2 * n + i];
+            if (periodic) memcpy(p->period.data(), hd + n * 16, n * 8);
         }
         // statistics
         int wall_khz = 0;

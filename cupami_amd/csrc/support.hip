@@ -133,7 +133,10 @@ __device__ __forceinline__ void launch_epilogue(const SupportArgs &a, unsigned l
 
 // ---- frequency ----------------------------------------------------------------
 
-template <int QPL>  // uint4 per lane per tile: tile = 512 * QPL words
+// QPL: uint4 per lane per tile (tile = 512 * QPL words).  KEEP: the launch
+// has retained rows (SupportArgs::rout); without them the kernel carries none
+// of their code.
+template <int QPL, bool KEEP>
 __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
     LevelView v;
     if (!level_view(a, v)) return;
@@ -182,9 +185,9 @@ __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
             const cptr<uint32_t> pre = v.prefix + (uint64_t)t * kMaxDepth;
             // one-stride rows keep every counted itemset's row: read the
             // prefix's from the previous launch when it has one
-            const bool kept_rows = db->stride == kRetainWords;
+            const bool kept_rows = KEEP && db->stride == kRetainWords;
             const uint32_t src = kept_rows && v.rin ? v.src[t] : ~0u;
-            const bool from_kept = src < v.slots;
+            const bool from_kept = KEEP && src < v.slots;
             uint4 P[QPL];
             uint32_t rows_read = e1 - e0;
             if (from_kept) {
@@ -507,11 +510,18 @@ uint32_t persistent_grid(const cpm_ctx *ctx, uint32_t per_cu, uint32_t grid) {
 
 int launch_support_count(cpm_ctx *ctx, const SupportArgs &a, uint32_t tile_words, uint32_t grid) {
     if (a.te == 0) return fail(CPM_ERR_INVALID, "zero tiles per work range");
-    if (tile_words == 1024)
-        hipLaunchKernelGGL(k_support_count<2>, dim3(persistent_grid(ctx, 7, grid)), dim3(256), 0,
+    const bool keep = a.rout != nullptr;
+    if (tile_words == 1024 && keep)
+        hipLaunchKernelGGL((k_support_count<2, true>), dim3(persistent_grid(ctx, 7, grid)), dim3(256), 0,
+                           ctx->stream, a);
+    else if (tile_words == 1024)
+        hipLaunchKernelGGL((k_support_count<2, false>), dim3(persistent_grid(ctx, 7, grid)), dim3(256), 0,
+                           ctx->stream, a);
+    else if (tile_words == 2048 && keep)
+        hipLaunchKernelGGL((k_support_count<4, true>), dim3(persistent_grid(ctx, 8, grid)), dim3(256), 0,
                            ctx->stream, a);
     else if (tile_words == 2048)
-        hipLaunchKernelGGL(k_support_count<4>, dim3(persistent_grid(ctx, 8, grid)), dim3(256), 0,
+        hipLaunchKernelGGL((k_support_count<4, false>), dim3(persistent_grid(ctx, 8, grid)), dim3(256), 0,
                            ctx->stream, a);
     else
         return fail(CPM_ERR_INVALID, "tile of %u words", tile_words);
