@@ -65,6 +65,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch level steps directly")
     ap.add_argument("--group", type=int, default=32, help="entries per support work item")
     ap.add_argument("--tile", type=int, default=1024, help="words per support tile (1024|2048)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="batch_256: the batched steps alone, on the schedule --group/--tile (no autotune, "
+                         "no one-at-a-time or per-class comparison; for profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -490,7 +493,11 @@ def run_batch(args):
 
     ctx = _hip.context(0)
     miner = BatchMiner(ctx)
-    config, table = miner.tune(specs, minsups)
+    if args.batched_only:
+        config, table = (args.group, args.tile), {}
+        miner.use(config)
+    else:
+        config, table = miner.tune(specs, minsups)
     log(f"bench: autotune {({f'{g}x{w}': round(v * 1e3, 2) for (g, w), v in table.items()})} -> {config}")
     # each step releases the previous result first, as a consumer that hands
     # results on would (its host blocks are then reused)
@@ -509,6 +516,15 @@ def run_batch(args):
     stats = ctx.stats()
     n_pat = len(pats)
     log(f"bench: batched {batched_s * 1e3:.2f} ms/step, {n_pat} patterns")
+    if args.batched_only:
+        return {"metric": "itemsets/s (batched Eclat, 256 databases)", "value": round(n_pat / batched_s, 1),
+                "unit": "itemsets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": round(batched_s * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "u64", "data": "synthetic (numpy PCG64 seeds 303, 303000+i)",
+                "config": {"workload": "batch_256", "databases": len(dbs), "patterns": n_pat,
+                           "schedule": {"group": config[0], "tile_words": config[1]},
+                           "parallelism": "single GPU"},
+                "roofline": None, "cpu_baseline": None, "parity": None}
 
     # plain: the same engine, one database at a time
     def sequential(idx):
