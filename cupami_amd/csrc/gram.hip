@@ -15,10 +15,10 @@
 // than 2^24 transactions; the planner cuts K so that it does.
 //
 // An MFMA step takes 32 K elements per lane: lanes 0-31 feed rows (columns)
-// 0-31 with one 32-bit piece of the tile, lanes 32-63 the same rows with
-// another.  Over four steps a lane uses one 16-byte LDS read per 32-row tile
-// (u32 8j + 4h .. +3 of its row's tile, h = lane / 32), and spreads each u32
-// to nibbles with shift-and-mask.  A (row items) and B (column items) use the
+// 0-31, lanes 32-63 the same rows with other K elements.  Over four steps a
+// lane uses one 16-byte LDS read per 32-row tile (u32 8j + 4h .. +3 of its
+// row's tile, h = lane / 32): step p takes bit plane p of those four u32 (the
+// bits 4m + p, masked in place, see plane_fp4).  A (row items) and B (column items) use the
 // same lane -> bit map, so every bit is paired with itself exactly once
 // whatever order the hardware gives the K slots.  128 accumulator registers
 // per wave leave room for two waves per SIMD, so one wave's barrier and LDS
@@ -51,15 +51,26 @@ constexpr int kKW = 16;                      // words per K tile: one 128-byte l
 constexpr int kGroups = kKW / 4;             // 16-byte LDS reads (four MFMA steps) per K tile
 constexpr int kLdsW = 2 * kKW + 4;           // u32 per LDS row (stride = 4 mod 32: 8-lane groups of
                                              // 16-byte reads cover all banks)
-constexpr uint32_t kE2M1One = 0x22222222u;   // bit 1 of each nibble: 1.0 in E2M1
 constexpr int kFmtFp4 = 4;                   // f8f6f4 operand format code of E2M1
-constexpr int kScaleOne = 0x7f7f7f7f;        // E8M0 block scales of 2^0
 static_assert(kKW % 2 == 0, "K tiles hold whole word pairs");
 
-// 32 bits -> 32 E2M1 values 0 / 1.0: bit 4j + k lands in nibble j of dword k.
-__device__ __forceinline__ v8i spread_fp4(uint32_t x) {
-    return (v8i){(int)((x << 1) & kE2M1One), (int)(x & kE2M1One), (int)((x >> 1) & kE2M1One),
-                 (int)((x >> 2) & kE2M1One), 0, 0, 0, 0};
+// Bit planes.  Four u32 of a row (128 K elements) feed four MFMAs, one per
+// plane p: plane p takes bit 4m + p of every u32, left in place where that is
+// a positive E2M1 value -- bit 0 of a nibble is 0.5, bit 1 is 1.0, bit 2 is
+// 2.0 -- and bit 3 (the sign) shifted down to bit 2.  An E8M0 block scale per
+// plane (2^1, 2^0, 2^-1, 2^-1) brings every set value to 1.0 in both
+// operands, so each product is still 0 or 1 and a plane costs one AND (two
+// for the last) per u32 instead of a shift and an AND per nibble position.
+__device__ __forceinline__ v8i plane_fp4(uint4 q, int p) {
+    constexpr uint32_t kMask[4] = {0x11111111u, 0x22222222u, 0x44444444u, 0x44444444u};
+    const uint32_t m = kMask[p];
+    if (p == 3) return (v8i){(int)((q.x >> 1) & m), (int)((q.y >> 1) & m), (int)((q.z >> 1) & m),
+                             (int)((q.w >> 1) & m), 0, 0, 0, 0};
+    return (v8i){(int)(q.x & m), (int)(q.y & m), (int)(q.z & m), (int)(q.w & m), 0, 0, 0, 0};
+}
+// E8M0 scales of the planes, the same byte in all four lanes of the register
+__device__ __forceinline__ int plane_scale(int p) {
+    return p == 0 ? 0x80808080 : p == 1 ? 0x7f7f7f7f : 0x7e7e7e7e;
 }
 
 __global__ __launch_bounds__(kThreads) void k_gram(const GramJob *__restrict__ jobs, const GramDb *__restrict__ dbs,
@@ -155,16 +166,17 @@ __global__ __launch_bounds__(kThreads) void k_gram(const GramJob *__restrict__ j
             for (int t = 0; t < kTY; ++t) qb[t] = *reinterpret_cast<const uint4 *>(rowB + t * 32 * kLdsW + 8 * j);
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
+                const int sc = plane_scale(c);
                 v8i fb[kTY];
 #pragma unroll
-                for (int y = 0; y < kTY; ++y) fb[y] = spread_fp4((&qb[y].x)[c]);
+                for (int y = 0; y < kTY; ++y) fb[y] = plane_fp4(qb[y], c);
 #pragma unroll
                 for (int x = 0; x < kTX; ++x) {
-                    const v8i fa = spread_fp4((&qa[x].x)[c]);
+                    const v8i fa = plane_fp4(qa[x], c);
 #pragma unroll
                     for (int y = 0; y < kTY; ++y)
                         acc[x][y] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                            fa, fb[y], acc[x][y], kFmtFp4, kFmtFp4, 0, kScaleOne, 0, kScaleOne);
+                            fa, fb[y], acc[x][y], kFmtFp4, kFmtFp4, 0, sc, 0, sc);
                 }
             }
         }
