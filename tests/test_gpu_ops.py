@@ -158,7 +158,7 @@ def test_intersect_extensions(ctx):
         assert np.array_equal(idx, keep) and np.array_equal(rows, expect_rows[keep])
 
 
-@pytest.mark.parametrize("n,items", [(64, 3), (1000, 17), (70_000, 65), (300_001, 130)])
+@pytest.mark.parametrize("n,items", [(64, 3), (1000, 17), (70_000, 65), (300_001, 130), (5000, 257)])
 def test_pair_gram_equals_popcount(ctx, n, items):
     rows, cols = random_store(n + items, n, items, 0.35)
     bits = ctx.build_bits(rows, cols, n, items)
