@@ -17,6 +17,7 @@
 #include "support_kernels.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace cpm {
 
@@ -133,21 +134,53 @@ __device__ __forceinline__ void launch_epilogue(const SupportArgs &a, unsigned l
 
 // ---- frequency ----------------------------------------------------------------
 
+// Row sums of a wave: DPP adds within each row of 16 lanes (the two quad
+// swaps, then the half-row and row mirrors), after which every lane of a row
+// holds the sum of its row.  No LDS round trip and no wait between steps.
+// All 64 lanes must be active.
+__device__ __forceinline__ uint32_t row_sum16(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xb1, 0xf, 0xf, false);   // quad_perm [1,0,3,2]
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4e, 0xf, 0xf, false);   // quad_perm [2,3,0,1]
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xf, 0xf, false);  // row_half_mirror
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xf, 0xf, false);  // row_mirror
+    return v;
+}
+
 // QPL: uint4 per lane per tile (tile = 512 * QPL words).  KEEP: the launch
 // has retained rows (SupportArgs::rout); without them the kernel carries none
 // of their code.
+//
+// A work item's entries are at most 64: lane i loads the column of entry
+// r0 + i once, and the entry loop reads it with readlane, so no scalar load
+// chain (entry -> item -> column) sits between one row load and the next.
+// Counts are summed per row of 16 lanes (row_sum16); the four row leaders of
+// a wave store them, two entries packed per word, and the block adds the 16
+// partial sums of each entry at the end of the work item.
+// The register hint keeps the 1024-word kernel without retained rows at 7
+// waves per SIMD (72 VGPRs; the persistent grid is 7 blocks per CU) and the
+// others at 4 at most.
 template <int QPL, bool KEEP>
-__global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QPL == 2 && !KEEP ? 7 : 4)))
+void k_support_count(SupportArgs a) {
     LevelView v;
     if (!level_view(a, v)) return;
     uint32_t *__restrict__ sup = a.L.sup;
-    __shared__ uint32_t part[4][64];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ uint32_t part[16][64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t slot = tid >> 4;              // wave * 4 + row of 16 lanes
+    const bool leader = (lane & 15) == 0;
     constexpr uint32_t kQuadsPerTile = 256 * QPL;
+    // entries counted per step: row loads in flight per lane = U x QPL
+    constexpr int U = QPL == 2 ? 4 : 2;
+    // a row of 16 lanes counts at most 16 x 128 x QPL bits: two entries'
+    // counts share one 32-bit word through the row sums
+    static_assert(16 * 128 * QPL < 65536, "packed row counts");
     const unsigned long long t_start = wall_clock64();
     unsigned long long bytes = 0, cands = 0;
     for (uint64_t it = blockIdx.x, k = 0; it < v.total; it += gridDim.x, ++k) {
         const Range r = range_of(v, it, k);
+        const uint32_t ne = r.r1 - r.r0;  // <= 64 (the group is at most 64)
+        const uint32_t colv = lane < ne ? col_of(v.fcol, v.ext[r.r0 + lane]) : 0u;
         uint32_t t = find_task(v.ext_off, v.n_tasks, r.r0);
         // The AND of the first d - 1 prefix rows of the last task built here
         // (its parent's itemset), reused while the next tasks are its
@@ -161,8 +194,8 @@ __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
             const cptr<DbDev> db = v.dbs + v.dbi[t];
             if (r.tile == 0) cands += e1 - e0;
             if (r.tile >= db->tiles) {
-                if (lane == 0)
-                    for (uint32_t e = e0; e < e1; ++e) part[wave][e - r.r0] = 0;
+                if (leader)
+                    for (uint32_t e = e0; e < e1; ++e) part[slot][e - r.r0] = 0;
                 e0 = e1;
                 continue;
             }
@@ -182,6 +215,10 @@ __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
                 off[q] = valid[q] ? o : stride_q - 1;
             }
             auto row_of = [&](uint32_t f) { return store + (uint64_t)col_of(v.fcol, f) * stride_q; };
+            // entry e's row (e in this work item's range)
+            auto entry_row = [&](uint32_t e) {
+                return store + (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)colv, (int)(e - r.r0)) * stride_q;
+            };
             const cptr<uint32_t> pre = v.prefix + (uint64_t)t * kMaxDepth;
             // one-stride rows keep every counted itemset's row: read the
             // prefix's from the previous launch when it has one
@@ -249,62 +286,64 @@ __global__ __launch_bounds__(256) void k_support_count(SupportArgs a) {
             const bool put = kept_rows && v.rout;
             const uint64_t tile_w = tile_words_of(db->words, r.tile, 2 * kQuadsPerTile);
             bytes += 8ull * tile_w * rows_read;
-            uint32_t e = e0;
-            for (; e + 1 < e1; e += 2) {
-                const gptr<uint4> ra = row_of(v.ext[e]), rb = row_of(v.ext[e + 1]);
-                uint4 A[QPL], B[QPL];
+            // NE entries from e: loads first (NE x QPL in flight per lane),
+            // then the counts, packed two per row sum
+            auto count = [&](uint32_t e, auto ne_c) {
+                constexpr int NE = decltype(ne_c)::value;
+                uint4 X[NE][QPL];
 #pragma unroll
-                for (int q = 0; q < QPL; ++q) {
-                    A[q] = and4(P[q], ra[off[q]]);
-                    B[q] = and4(P[q], rb[off[q]]);
+                for (int u = 0; u < NE; ++u) {
+                    const gptr<uint4> row = entry_row(e + u);
+#pragma unroll
+                    for (int q = 0; q < QPL; ++q) X[u][q] = row[off[q]];
                 }
-                uint32_t ca = 0, cb = 0;
+                uint32_t c[NE];
 #pragma unroll
-                for (int q = 0; q < QPL; ++q) {
-                    ca += pop4(A[q]);
-                    cb += pop4(B[q]);
+                for (int u = 0; u < NE; ++u) {
+                    c[u] = 0;
+#pragma unroll
+                    for (int q = 0; q < QPL; ++q) {
+                        X[u][q] = and4(P[q], X[u][q]);
+                        c[u] += pop4(X[u][q]);
+                    }
                 }
                 if (put) {
 #pragma unroll
-                    for (int q = 0; q < QPL; ++q)
-                        if (valid[q]) {
-                            if (e < v.slots) keep[(uint64_t)e * stride_q + off[q]] = A[q];
-                            if (e + 1 < v.slots) keep[(uint64_t)(e + 1) * stride_q + off[q]] = B[q];
+                    for (int u = 0; u < NE; ++u)
+                        if (e + u < v.slots) {
+#pragma unroll
+                            for (int q = 0; q < QPL; ++q)
+                                if (valid[q]) keep[(uint64_t)(e + u) * stride_q + off[q]] = X[u][q];
+                            bytes += 8ull * tile_w;
                         }
-                    bytes += 8ull * tile_w * ((e < v.slots) + (e + 1 < v.slots));
                 }
-                // one reduction for both: a wave's count is at most
-                // 64 lanes x 128 x QPL bits < 2^16 for QPL <= 4
-                static_assert(64 * 128 * QPL < 65536, "packed wave counts");
-                const uint32_t cab = wave_sum(ca | (cb << 16));
-                if (lane == 0) {
-                    part[wave][e - r.r0] = cab & 0xffffu;
-                    part[wave][e + 1 - r.r0] = cab >> 16;
-                }
-            }
-            if (e < e1) {
-                const gptr<uint4> ra = row_of(v.ext[e]);
-                uint4 A[QPL];
-                uint32_t ca = 0;
 #pragma unroll
-                for (int q = 0; q < QPL; ++q) {
-                    A[q] = and4(P[q], ra[off[q]]);
-                    ca += pop4(A[q]);
+                for (int u = 0; u + 1 < NE; u += 2) {
+                    const uint32_t s = row_sum16(c[u] | (c[u + 1] << 16));
+                    if (leader) {
+                        part[slot][e + u - r.r0] = s & 0xffffu;
+                        part[slot][e + u + 1 - r.r0] = s >> 16;
+                    }
                 }
-                if (put && e < v.slots) {
-#pragma unroll
-                    for (int q = 0; q < QPL; ++q)
-                        if (valid[q]) keep[(uint64_t)e * stride_q + off[q]] = A[q];
-                    bytes += 8ull * tile_w;
+                if (NE & 1) {
+                    const uint32_t s = row_sum16(c[NE - 1]);
+                    if (leader) part[slot][e + NE - 1 - r.r0] = s;
                 }
-                ca = wave_sum(ca);
-                if (lane == 0) part[wave][e - r.r0] = ca;
+            };
+            uint32_t e = e0;
+            for (; e + U <= e1; e += U) count(e, std::integral_constant<int, U>{});
+            if (U > 2 && e + 2 <= e1) {
+                count(e, std::integral_constant<int, 2>{});
+                e += 2;
             }
+            if (e < e1) count(e, std::integral_constant<int, 1>{});
             e0 = e1;
         }
         __syncthreads();
-        if (tid < r.r1 - r.r0) {
-            const uint32_t c = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+        if (tid < ne) {
+            uint32_t c = 0;
+#pragma unroll
+            for (int s = 0; s < 16; ++s) c += part[s][tid];
             if (c) atomicAdd(sup + r.r0 + tid, c);
         }
         __syncthreads();
