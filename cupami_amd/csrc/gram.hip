@@ -77,7 +77,8 @@ __global__ __launch_bounds__(kThreads) void k_gram(const GramJob *__restrict__ j
                                                    int32_t *__restrict__ out) {
     __shared__ uint32_t sA[2][kBT][kLdsW];  // two buffers of A and B tiles: 144 KB
     __shared__ uint32_t sB[2][kBT][kLdsW];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar loop control
     const GramJob job = jobs[blockIdx.x];
     const GramDb db = dbs[job.db];
     const gptr<uint64_t> store = to_global(db.store);
@@ -91,30 +92,38 @@ __global__ __launch_bounds__(kThreads) void k_gram(const GramJob *__restrict__ j
 
     // loader: lanes 4r .. 4r+3 of a wave stage the 64 bytes (kKW words) of
     // one row, so one load instruction touches 16 rows; wave w's pass q
-    // covers rows 16 (4w + q) .. +15 of the 512 (A then B)
+    // covers rows 16 (4w + q) .. +15 of the 512 (A then B).  Loads are
+    // unconditional: a block row past the last item reads item n - 1's row,
+    // which only reaches outputs past the last item (never stored), and a
+    // tile reaches past k_end only in the last slice, where the words past
+    // the row's data are the store's zero padding (slices start and end on
+    // kKW-word tiles, and rows are padded to a multiple of 64 words).
     constexpr int kLanesPerRow = kKW / 2;            // uint4 per row per tile
     constexpr int kRowsPerPass = 64 / kLanesPerRow;  // rows per load instruction
     constexpr int kPasses = 2 * kBT / (kRowsPerPass * (kThreads / 64));
     static_assert(kPasses * kRowsPerPass * (kThreads / 64) == 2 * kBT, "passes cover A and B");
     const uint32_t part = lane % kLanesPerRow;
-    gptr<uint4> pr[kPasses];
+    uint32_t col[kPasses];
     uint32_t *srow[kPasses];
 #pragma unroll
-    for (int q = 0; q < kPasses; ++q) {
+    for (int q = 0; q < kPasses; ++q) {  // all column loads in flight together
         const uint32_t R = (wave * kPasses + q) * kRowsPerPass + lane / kLanesPerRow;  // 0 .. 511
         const bool isB = R >= (uint32_t)kBT;
         const uint32_t lr = R % kBT;
         const uint32_t item = (isB ? bj : bi) * kBT + lr;
-        pr[q] = item < n ? (gptr<uint4>)(store + (uint64_t)cols[item] * stride) : nullptr;
+        col[q] = cols[min(item, n - 1)];
         srow[q] = (isB ? &sB[0][lr][0] : &sA[0][lr][0]) + 4 * part;
     }
-    uint4 rr[kPasses];
+    gptr<uint4> pr[kPasses];
+#pragma unroll
+    for (int q = 0; q < kPasses; ++q) pr[q] = (gptr<uint4>)(store + (uint64_t)col[q] * stride);
+    uint4 rr[kPasses] = {};
     auto fetch = [&](uint64_t k0) {
-        const uint64_t w = k0 + 2 * part;  // even; w + 1 < stride (zero padded)
+        const uint64_t w = k0 + 2 * part;  // even; w + 1 < stride
 #pragma unroll
         for (int q = 0; q < kPasses; ++q) {
             rr[q] = make_uint4(0, 0, 0, 0);
-            if (pr[q] && w < k_end) rr[q] = pr[q][w / 2];
+            if (w < k_end) rr[q] = pr[q][w / 2];
         }
     };
     constexpr uint32_t kBufU32 = kBT * kLdsW;  // u32 per LDS buffer of A (or B)
