@@ -1,6 +1,8 @@
 // Internal definitions shared by the libcupami_hip.so translation units.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <stdint.h>
@@ -296,6 +298,61 @@ __device__ __forceinline__ gptr<T> to_global(const T *p) {
 template <class T>
 __device__ __forceinline__ cptr<T> to_const(const T *p) {
     return (cptr<T>)p;
+}
+
+// ---- wave sums and scans on DPP ------------------------------------------------
+// Data-parallel moves between lanes of a row of 16 inside one VALU
+// instruction: no LDS round trip and no wait between steps (a __shfl is an
+// LDS permute with a wait after it).  Every lane of the wave must be active.
+
+// Row sums: after the two quad swaps and the half-row and row mirrors every
+// lane of a row of 16 holds the sum of its row.
+__device__ __forceinline__ uint32_t dpp_row_sum(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xb1, 0xf, 0xf, false);   // quad_perm [1,0,3,2]
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4e, 0xf, 0xf, false);   // quad_perm [2,3,0,1]
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xf, 0xf, false);  // row_half_mirror
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xf, 0xf, false);  // row_mirror
+    return v;
+}
+
+// Wave sum, uniform across the wave (the four row sums read as scalars).
+__device__ __forceinline__ uint32_t dpp_wave_sum(uint32_t v) {
+    v = dpp_row_sum(v);
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 0) + (uint32_t)__builtin_amdgcn_readlane((int)v, 16) +
+           (uint32_t)__builtin_amdgcn_readlane((int)v, 32) + (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+}
+
+__device__ __forceinline__ unsigned long long dpp_wave_sum64(unsigned long long v) {
+    auto step = [&](auto ctrl) {
+        constexpr int c = decltype(ctrl)::value;
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)v, c, 0xf, 0xf, false);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(v >> 32), c, 0xf, 0xf, false);
+        v += ((unsigned long long)hi << 32) | lo;
+    };
+    step(std::integral_constant<int, 0xb1>{});
+    step(std::integral_constant<int, 0x4e>{});
+    step(std::integral_constant<int, 0x141>{});
+    step(std::integral_constant<int, 0x140>{});
+    unsigned long long t = 0;
+#pragma unroll
+    for (int r = 0; r < 64; r += 16)
+        t += ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), r) << 32) |
+             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, r);
+    return t;
+}
+
+// Inclusive scan over the wave: shifts by 1, 2, 4 and 8 lanes inside each row
+// (lanes shifted in from outside the row read 0), then the last lane of rows
+// 0 and 2 added into rows 1 and 3, then the last lane of row 1 into rows 2
+// and 3 (rows outside a move's row mask take 0).
+__device__ __forceinline__ uint32_t dpp_wave_scan_incl(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);  // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false); // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false); // row_bcast:31
+    return x;
 }
 
 struct GramDb {

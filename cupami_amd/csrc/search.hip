@@ -104,12 +104,7 @@ __device__ __forceinline__ bool keep_entry(const PlanArgs &p, const DbDev &D, ui
 // the block sum.  Uses lds[0..4].
 __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *lds, uint32_t *total) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off, 64);
-        if (lane >= (uint32_t)off) x += y;
-    }
+    const uint32_t x = dpp_wave_scan_incl(v);
     if (lane == 63) lds[wave] = x;
     __syncthreads();
     uint32_t base = 0;
@@ -187,14 +182,8 @@ __device__ __forceinline__ void wave_sync() {
 
 // exclusive scan over the 64 lanes of a wave; *total gets the wave sum
 __device__ __forceinline__ uint32_t wave_scan(uint32_t v, uint32_t *total) {
-    const uint32_t lane = threadIdx.x & 63;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= (uint32_t)o) x += y;
-    }
-    *total = __shfl(x, 63, 64);
+    const uint32_t x = dpp_wave_scan_incl(v);
+    *total = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
     return x - v;
 }
 
@@ -599,10 +588,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
         // wave instead of a dependent load per item in every lane
         const uint32_t sv = lane < d ? p.L.prefix[(uint64_t)t * kMaxDepth + lane] : 0u;
         unsigned long long psum = 0;  // sum of the prefix items' filter terms
-        if (stage) {
-            psum = lane < d ? item_term(sv) : 0ull;
-            for (int off = 32; off > 0; off >>= 1) psum += __shfl_xor(psum, off, 64);
-        }
+        if (stage) psum = dpp_wave_sum64(lane < d ? item_term(sv) : 0ull);
         uint32_t f = 0, e = 0;
         if (lane < nk) {
             e = p.kept[a + lane];
@@ -634,7 +620,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
         // coalesced store per child (d + 1 <= 64 when children are emitted)
         const uint32_t cb = p.off_child[t];
         for (uint32_t c = 0; c + 1 < nk; ++c) {
-            const uint32_t fc = __shfl(f, c, 64);
+            const uint32_t fc = (uint32_t)__builtin_amdgcn_readlane((int)f, (int)c);
             if (lane <= d) p.N.prefix[(uint64_t)(cb + c) * kMaxDepth + lane] = lane < d ? sv : fc;
         }
         if (live) {
@@ -679,10 +665,7 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
         if (tid < d) spre[tid] = p.L.prefix[(uint64_t)t * kMaxDepth + tid];
         __syncthreads();
         unsigned long long psum = 0;  // sum of the prefix items' filter terms
-        if (stage) {
-            psum = lane < d ? item_term(spre[lane]) : 0ull;
-            for (int off = 32; off > 0; off >>= 1) psum += __shfl_xor(psum, off, 64);
-        }
+        if (stage) psum = dpp_wave_sum64(lane < d ? item_term(spre[lane]) : 0ull);
         for (uint32_t j = tid; j < nk; j += 256) {
             const uint32_t e = p.kept[a + j];
             const uint32_t f = p.L.ext[e];
@@ -1359,7 +1342,7 @@ struct Search {
 
     uint32_t grid() const { return support_grid(ctx); }
     uint64_t work_target() const { return 4ull * grid(); }
-    uint32_t grid_tasks() const { return (uint32_t)ctx->cu_count * 4; }
+    uint32_t grid_tasks() const { return (uint32_t)ctx->cu_count * 6; }
 
     int count(int c, bool level1) {
         SupportArgs a = support_args(c, level1);

@@ -134,28 +134,6 @@ __device__ __forceinline__ void launch_epilogue(const SupportArgs &a, unsigned l
 
 // ---- frequency ----------------------------------------------------------------
 
-// Row sums of a wave: DPP adds within each row of 16 lanes (the two quad
-// swaps, then the half-row and row mirrors), after which every lane of a row
-// holds the sum of its row.  No LDS round trip and no wait between steps.
-// All 64 lanes must be active.
-__device__ __forceinline__ uint32_t row_sum16(uint32_t v) {
-    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xb1, 0xf, 0xf, false);   // quad_perm [1,0,3,2]
-    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4e, 0xf, 0xf, false);   // quad_perm [2,3,0,1]
-    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xf, 0xf, false);  // row_half_mirror
-    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xf, 0xf, false);  // row_mirror
-    return v;
-}
-
-// QPL: uint4 per lane per tile (tile = 512 * QPL words).  KEEP: the launch
-// has retained rows (SupportArgs::rout); without them the kernel carries none
-// of their code.
-//
-// A work item's entries are at most 64: lane i loads the column of entry
-// r0 + i once, and the entry loop reads it with readlane, so no scalar load
-// chain (entry -> item -> column) sits between one row load and the next.
-// Counts are summed per row of 16 lanes (row_sum16); the four row leaders of
-// a wave store them, two entries packed per word, and the block adds the 16
-// partial sums of each entry at the end of the work item.
 // The register hint keeps the 1024-word kernel without retained rows at 7
 // waves per SIMD (72 VGPRs; the persistent grid is 7 blocks per CU) and the
 // others at 4 at most.
@@ -319,14 +297,14 @@ void k_support_count(SupportArgs a) {
                 }
 #pragma unroll
                 for (int u = 0; u + 1 < NE; u += 2) {
-                    const uint32_t s = row_sum16(c[u] | (c[u + 1] << 16));
+                    const uint32_t s = dpp_row_sum(c[u] | (c[u + 1] << 16));
                     if (leader) {
                         part[slot][e + u - r.r0] = s & 0xffffu;
                         part[slot][e + u + 1 - r.r0] = s >> 16;
                     }
                 }
                 if (NE & 1) {
-                    const uint32_t s = row_sum16(c[NE - 1]);
+                    const uint32_t s = dpp_row_sum(c[NE - 1]);
                     if (leader) part[slot][e + NE - 1 - r.r0] = s;
                 }
             };
