@@ -134,6 +134,101 @@ __device__ __forceinline__ void launch_epilogue(const SupportArgs &a, unsigned l
 
 // ---- frequency ----------------------------------------------------------------
 
+// Short rows.  A database of at most 4096 transactions has one 64-word
+// stride per row: on the tile path it fills 32 / QPL lanes of a 256-lane
+// block per entry, so its deep levels (many small tasks) are walked one entry
+// pair at a time by nearly idle blocks.  A work item whose tasks all belong
+// to such databases, and whose prefixes are each one retained row (levels
+// past 3) or at most two item rows, is counted by lane groups instead: S =
+// 32 / QPL lanes per entry, 256 / S entries at a time, each group reading its
+// own task's prefix.  emap (64 entries) and ebytes are the block's LDS.
+// Returns false, with nothing counted, when the work item does not qualify
+// (the same answer for every tile of its range).
+template <int QPL>
+__device__ bool short_rows(const SupportArgs &a, const LevelView &v, const Range &r, uint32_t t0, uint32_t colv,
+                           uint32_t *emap, unsigned long long *ebytes, unsigned long long &bytes,
+                           unsigned long long &cands) {
+    constexpr uint32_t kQuads = kRetainWords / 2;  // uint4 per row
+    constexpr uint32_t S = kQuads / QPL;           // lanes per entry
+    constexpr uint32_t NG = 256 / S;               // entries at a time
+    static_assert(S == 8 || S == 16, "group sums cover 8 or 16 lanes");
+    const uint32_t tid = threadIdx.x;
+    // the range's tasks, one per lane: entry -> task map, qualification,
+    // algorithmic bytes (entry row, prefix rows, retained row written)
+    if (tid == 0) *ebytes = 0;
+    __syncthreads();
+    int ok = 1;
+    unsigned long long my_bytes = 0;
+    const uint32_t prefix_rows = v.d == 0 ? 0u : v.d <= 2 ? v.d : 1u;
+    for (uint32_t t = t0 + tid; t < v.n_tasks && v.ext_off[t] < r.r1; t += blockDim.x) {
+        const uint32_t e0 = max(v.ext_off[t], r.r0), e1 = min(v.ext_off[t + 1], r.r1);
+        if (e0 >= e1) continue;
+        const cptr<DbDev> db = v.dbs + v.dbi[t];
+        const bool kept = v.rin && v.src[t] < v.slots;
+        if (db->stride != kRetainWords || (v.d > 2 && !kept)) ok = 0;
+        for (uint32_t e = e0; e < e1; ++e) {
+            emap[e - r.r0] = t;
+            my_bytes += 8ull * db->words * (1 + prefix_rows + (a.rout && e < v.slots ? 1 : 0));
+        }
+    }
+    if (my_bytes && r.tile == 0) atomicAdd(ebytes, my_bytes);
+    if (!__syncthreads_and(ok)) return false;
+    if (r.tile == 0) {
+        bytes += *ebytes;
+        cands += r.r1 - r.r0;
+        const uint32_t g = tid / S, sl = tid % S;
+        const uint32_t ne = r.r1 - r.r0;
+        uint4 *const keep = reinterpret_cast<uint4 *>(a.rout);
+        for (uint32_t base = 0; base < ne; base += NG) {  // the same trip count in every lane
+            const uint32_t i = base + g;
+            const bool live = i < ne;
+            const uint32_t ii = live ? i : ne - 1;
+            const uint32_t e = r.r0 + ii;
+            const uint32_t t = emap[ii];
+            const uint32_t col = (uint32_t)__shfl((int)colv, (int)ii, 64);
+            const cptr<DbDev> db = v.dbs + v.dbi[t];
+            const gptr<uint4> store = to_global(reinterpret_cast<const uint4 *>(db->store));
+            const uint32_t q0 = sl * QPL;
+            uint4 P[QPL];
+            if (v.d == 0) {
+#pragma unroll
+                for (int q = 0; q < QPL; ++q) P[q] = make_uint4(~0u, ~0u, ~0u, ~0u);
+            } else if (v.d <= 2) {
+                const cptr<uint32_t> pre = v.prefix + (uint64_t)t * kMaxDepth;
+                const gptr<uint4> r0 = store + (uint64_t)col_of(v.fcol, pre[0]) * kQuads;
+                const gptr<uint4> r1 = store + (uint64_t)col_of(v.fcol, pre[v.d - 1]) * kQuads;
+#pragma unroll
+                for (int q = 0; q < QPL; ++q) P[q] = and4(r0[q0 + q], r1[q0 + q]);
+            } else {
+                const gptr<uint4> row = to_global(reinterpret_cast<const uint4 *>(v.rin)) + (uint64_t)v.src[t] * kQuads;
+#pragma unroll
+                for (int q = 0; q < QPL; ++q) P[q] = row[q0 + q];
+            }
+            const gptr<uint4> xr = store + (uint64_t)col * kQuads;
+            uint32_t c = 0;
+            uint4 X[QPL];
+#pragma unroll
+            for (int q = 0; q < QPL; ++q) {
+                X[q] = and4(P[q], xr[q0 + q]);
+                c += pop4(X[q]);
+            }
+            if (live && keep && e < v.slots) {
+#pragma unroll
+                for (int q = 0; q < QPL; ++q) keep[(uint64_t)e * kQuads + q0 + q] = X[q];
+            }
+            // sum over the group's S lanes (quad swaps, half-row mirror, and
+            // for 16 lanes the row mirror)
+            c += (uint32_t)__builtin_amdgcn_mov_dpp((int)c, 0xb1, 0xf, 0xf, false);
+            c += (uint32_t)__builtin_amdgcn_mov_dpp((int)c, 0x4e, 0xf, 0xf, false);
+            c += (uint32_t)__builtin_amdgcn_mov_dpp((int)c, 0x141, 0xf, 0xf, false);
+            if (S == 16) c += (uint32_t)__builtin_amdgcn_mov_dpp((int)c, 0x140, 0xf, 0xf, false);
+            if (live && sl == 0) a.L.sup[e] = c;
+        }
+    }
+    __syncthreads();  // emap and ebytes are read before the next work item rewrites them
+    return true;
+}
+
 // The register hint keeps the 1024-word kernel without retained rows at 7
 // waves per SIMD (72 VGPRs; the persistent grid is 7 blocks per CU) and the
 // others at 4 at most.
@@ -145,6 +240,8 @@ void k_support_count(SupportArgs a) {
     uint32_t *__restrict__ sup = a.L.sup;
     __shared__ uint32_t part[16][64];
     const uint32_t tid = threadIdx.x, lane = tid & 63;
+    __shared__ uint32_t emap[64];                // short rows: entry -> task
+    __shared__ unsigned long long ebytes;
     const uint32_t slot = tid >> 4;              // wave * 4 + row of 16 lanes
     const bool leader = (lane & 15) == 0;
     constexpr uint32_t kQuadsPerTile = 256 * QPL;
@@ -160,6 +257,7 @@ void k_support_count(SupportArgs a) {
         const uint32_t ne = r.r1 - r.r0;  // <= 64 (the group is at most 64)
         const uint32_t colv = lane < ne ? col_of(v.fcol, v.ext[r.r0 + lane]) : 0u;
         uint32_t t = find_task(v.ext_off, v.n_tasks, r.r0);
+        if (KEEP && short_rows<QPL>(a, v, r, t, colv, emap, &ebytes, bytes, cands)) continue;
         // The AND of the first d - 1 prefix rows of the last task built here
         // (its parent's itemset), reused while the next tasks are its
         // siblings: children of one task are stored next to each other.
