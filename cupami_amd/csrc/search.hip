@@ -1215,6 +1215,8 @@ struct SearchWs {
     size_t stage_off = 0; // in use since the last stream sync
     DBuf sup64;           // supports widened to u64 for the download of large results
     HBuf down;            // pinned stage of the download of small results
+    hipStream_t copy = nullptr;       // streams large results to the host (default priority)
+    hipEvent_t copy_ev = nullptr;     // a chunk's patterns are final and widened
     Caps caps{};
     bool graphs_ok[2] = {false, false};
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -1222,6 +1224,11 @@ struct SearchWs {
     ~SearchWs() {
         for (auto &g : graph)
             if (g) (void)hipGraphExecDestroy(g);
+        if (copy) {
+            (void)hipStreamSynchronize(copy);
+            (void)hipStreamDestroy(copy);
+        }
+        if (copy_ev) (void)hipEventDestroy(copy_ev);
     }
 };
 
@@ -1258,8 +1265,68 @@ struct Search {
     uint32_t parts_tiles = 0;
     Caps need{};                     // level 2's exact sizes when they overflowed
     uint64_t slots = 0;              // retained rows per level set (0: none)
+    // Large results are streamed: once a search has found enough patterns
+    // for the direct download, each finished chunk's patterns are copied into
+    // the result's pinned arrays (sized to the pattern capacity) on a second
+    // stream while the next chunk runs; download() copies only the rest.
+    cpm_patterns *res = nullptr;
+    uint64_t copied = 0;             // patterns queued for copying into res
 
     Search(cpm_ctx *c, SearchWs &ws) : ctx(c), w(ws) {}
+    ~Search() { drop_stream(); }
+
+    // the streamed result of an attempt that will not be used (copies waited for)
+    void drop_stream() {
+        if (res) {
+            (void)hipStreamSynchronize(w.copy);
+            delete res;
+            res = nullptr;
+        }
+        copied = 0;
+    }
+
+    // queue the copy of patterns [copied, n_now): the support column widened
+    // on the search stream, then every column on the copy stream
+    int queue_copy(uint64_t n_now) {
+        if (n_now <= copied) return CPM_OK;
+        if (!res) {
+            if (n_now * 4 < kDirectDownload || n_now > w.caps.patterns) return CPM_OK;
+            if (!w.copy) CPM_HIP(hipStreamCreateWithFlags(&w.copy, hipStreamNonBlocking));
+            if (!w.copy_ev) CPM_HIP(hipEventCreateWithFlags(&w.copy_ev, hipEventDisableTiming));
+            CPM_CHECK(w.sup64.ensure(w.caps.patterns * 8));
+            const uint64_t cap = w.caps.patterns;
+            res = new cpm_patterns();
+            res->periodic = periodic;
+            try {
+                res->parent.resize(cap);
+                res->item.resize(cap);
+                res->support.resize(cap);
+                res->db.resize(cap);
+                if (periodic) res->period.resize(cap);
+            } catch (const std::bad_alloc &) {
+                delete res;
+                res = nullptr;
+                return fail(CPM_ERR_NOMEM, "host memory for %llu patterns", (unsigned long long)cap);
+            }
+        }
+        const uint64_t a = copied, m = n_now - a;
+        hipStream_t cs = w.copy;
+        hipLaunchKernelGGL(k_widen_u32, grid_of_n(m), dim3(256), 0, ctx->stream, pat_col(2) + a, m,
+                           w.sup64.as<unsigned long long>() + a);
+        CPM_HIP(hipGetLastError());
+        CPM_HIP(hipEventRecord(w.copy_ev, ctx->stream));
+        CPM_HIP(hipStreamWaitEvent(cs, w.copy_ev, 0));
+        CPM_HIP(hipMemcpyAsync(res->parent.data() + a, pat_col(0) + a, m * 4, hipMemcpyDeviceToHost, cs));
+        CPM_HIP(hipMemcpyAsync(res->item.data() + a, pat_col(1) + a, m * 4, hipMemcpyDeviceToHost, cs));
+        CPM_HIP(hipMemcpyAsync(res->db.data() + a, pat_col(3) + a, m * 4, hipMemcpyDeviceToHost, cs));
+        CPM_HIP(hipMemcpyAsync(res->support.data() + a, w.sup64.as<unsigned long long>() + a, m * 8,
+                               hipMemcpyDeviceToHost, cs));
+        if (periodic)
+            CPM_HIP(hipMemcpyAsync(res->period.data() + a, w.pat_per.as<int64_t>() + a, m * 8,
+                                   hipMemcpyDeviceToHost, cs));
+        copied = n_now;
+        return CPM_OK;
+    }
 
     Ctl *ctl() const { return w.ctl.as<Ctl>(); }
     // words of the subset-test filter: 32 bits per itemset of the pattern
@@ -1797,6 +1864,7 @@ struct Search {
                 *overflow = hc->overflow;
                 return CPM_OK;
             }
+            CPM_CHECK(queue_copy(hc->n_patterns));
             if (hc->sz[c].n_tasks == 0) {
                 // level c is empty: its support launch wrote no stamps, and
                 // the plan that made it folded the previous launch's
@@ -1827,6 +1895,7 @@ struct Search {
         for (int tries = 0;; ++tries) {
             uint32_t ov = 0;
             need = Caps{};
+            drop_stream();  // before buffers are resized
             const int rc = attempt(&ov);
             if (rc != CPM_OK || ov) w.filter_dirty = true;  // stopped between insert and clear
             CPM_CHECK(rc);
@@ -1861,6 +1930,23 @@ struct Search {
             CPM_CHECK(read_ctl(hc));
         }
         const uint64_t n = hc->n_patterns;
+        if (res) {
+            // streamed: the last chunk's patterns, then one wait
+            CPM_CHECK(queue_copy(n));
+            const hipError_t e = hipStreamSynchronize(w.copy);
+            if (e != hipSuccess) return fail(CPM_ERR_RUNTIME, "pattern download failed: %s", hipGetErrorString(e));
+            res->n = n;
+            res->parent.resize(n);
+            res->item.resize(n);
+            res->support.resize(n);
+            res->db.resize(n);
+            if (periodic) res->period.resize(n);
+            record_stats(hc);
+            *out = res;
+            res = nullptr;
+            copied = 0;
+            return CPM_OK;
+        }
         cpm_patterns *p = new cpm_patterns();
         p->n = n;
         p->periodic = periodic;
@@ -1924,7 +2010,12 @@ struct Search {
             for (uint64_t i = 0; i < n; ++i) p->support[i] = hp[2 * n + i];
             if (periodic) memcpy(p->period.data(), hd + n * 16, n * 8);
         }
-        // statistics
+        record_stats(hc);
+        *out = p;
+        return CPM_OK;
+    }
+
+    void record_stats(const Ctl *hc) {
         int wall_khz = 0;
         (void)hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, ctx->device);
         const double tick_ms = wall_khz > 0 ? 1.0 / wall_khz : 0.0;
@@ -1933,8 +2024,6 @@ struct Search {
         ctx->stats[2] += (double)hc->acc.bytes;
         ctx->stats[3] += (double)hc->acc.candidates;
         ctx->stats[7] = (double)hc->levels + 2;
-        *out = p;
-        return CPM_OK;
     }
 };
 
