@@ -129,15 +129,17 @@ MALL_BYTES = 256 << 20
 def support_roofline(workload, kernel, stats, steps, store_bytes):
     """The support kernel against the level of the memory hierarchy that
     bounds it.  ``achieved``: algorithmic bytes per launch (summed in the
-    kernel) / mean launch time (kernel clock stamps, live).  From the
-    committed summaries: the kernel's L2 request bytes and fabric bytes per
-    launch (counters) and the measured roofs.  Each level's time floor is
-    its bytes / its roof (L2 for requests; the Infinity Cache for fabric
-    reads when the store fits it, HBM otherwise); the larger floor names the
-    bound, and ``peak`` is that floor in algorithmic bytes, so ``frac`` <= 1
-    when the model holds.  ``traffic``: HBM bytes per launch from FETCH_SIZE
-    and WRITE_SIZE scaled by a copy of known size (an upper bound when the
-    store sits in the Infinity Cache)."""
+    kernel) / mean launch time (kernel clock stamps, live; launches past a
+    search's last level exit at once and are not counted).  From the
+    committed summaries: the kernel's L2 request bytes and fabric read bytes
+    per search (counters) and the measured roofs.  Each level's time floor
+    for a search is its bytes / its roof (L2 for requests; the Infinity Cache
+    for fabric reads when the store fits it, HBM otherwise); the larger floor
+    names the bound, and ``peak`` is that floor in algorithmic bytes, so
+    ``frac`` = floor / kernel time per search <= 1 when the model holds.
+    ``traffic``: HBM bytes per launch from FETCH_SIZE and WRITE_SIZE scaled by
+    a copy of known size (an upper bound when the store sits in the Infinity
+    Cache)."""
     from cupami_amd import profiling
     launches = max(stats["launches"], 1)
     avg_ms = stats["kernel_ms"] / launches
@@ -148,25 +150,29 @@ def support_roofline(workload, kernel, stats, steps, store_bytes):
             "launches_per_step": launches / steps}
     roofs = profiling.read_summary(ROOT / "profiles" / "roofs.md")
     cnt = profiling.read_summary(ROOT / "profiles" / f"{workload}_{kernel}_counters.md")
-    if not roofs or "l2_request_bytes_per_launch" not in cnt or avg_ms <= 0:
+    if not roofs or "l2_request_bytes_per_search" not in cnt or avg_ms <= 0:
         line["basis"] = "no current counter summary for these sources: bound not placed"
         return line
-    l2_b = float(cnt["l2_request_bytes_per_launch"])
-    fab_b = float(cnt["fabric_read_bytes_per_launch"])
+    l2_b = float(cnt["l2_request_bytes_per_search"])
+    fab_b = float(cnt["fabric_read_bytes_per_search"])
     fab_level = "mall" if store_bytes <= MALL_BYTES else "hbm"
     r_l2, r_fab = float(roofs["l2_gb_s"]), float(roofs[f"{fab_level}_gb_s"])
     t_l2, t_fab = l2_b / (r_l2 * 1e9), fab_b / (r_fab * 1e9)
     bound, floor = ("l2", t_l2) if t_l2 >= t_fab else (fab_level, t_fab)
-    peak = alg / floor / 1e9
-    t = avg_ms / 1e3
+    t = stats["kernel_ms"] / steps / 1e3    # kernel time per search
+    alg_s = stats["bytes"] / steps
+    peak = alg_s / floor / 1e9
+    per_launch = launches / steps
     line.update(bound=bound, peak=round(peak, 1), frac=round(achieved / peak, 4),
-                traffic=float(cnt["hbm_bytes_per_launch"]) if "hbm_bytes_per_launch" in cnt else None)
+                traffic=(float(cnt["hbm_bytes_per_search"]) / per_launch
+                         if "hbm_bytes_per_search" in cnt else None))
     line["basis"] = {
-        "l2_request_bytes_per_launch": l2_b, "l2_hit_rate": float(cnt.get("l2_hit_rate", "nan")),
+        "l2_request_bytes_per_search": l2_b, "l2_hit_rate": float(cnt.get("l2_hit_rate", "nan")),
         "l2_roof_gb_s": r_l2, "l2_gb_s": round(l2_b / t / 1e9, 1),
-        "fabric_read_bytes_per_launch": fab_b, f"{fab_level}_roof_gb_s": r_fab,
+        "fabric_read_bytes_per_search": fab_b, f"{fab_level}_roof_gb_s": r_fab,
         "fabric_gb_s": round(fab_b / t / 1e9, 1),
         "fabric_frac_of_hbm_published": round(fab_b / t / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel_ms_per_search": round(t * 1e3, 4),
         "summaries": ["profiles/roofs.md", f"profiles/{workload}_{kernel}_counters.md"]}
     return line
 

@@ -88,26 +88,55 @@ def _dispatches(run_dir):
     return out
 
 
+# one dispatch per search attempt: the level-1 setup kernel
+_SEARCH = re.compile(r"\bk_level1\(")
+
+
 def counter_averages(run_dirs, kernel):
     """Per-launch averages of every counter over the dispatches of
-    ``kernel`` (a name fragment), and the calibration copy's readings."""
-    sums, counts, copy = defaultdict(float), defaultdict(int), {}
+    ``kernel`` (a name fragment), the calibration copy's readings, and each
+    counter's total per search in the runs that also traced ``k_level1``.
+
+    ``k_level1`` starts every attempt of a search.  A search whose buffers
+    overflow stops and starts again with larger ones (a fresh context's first
+    search always does once, after level 1), so an attempt is counted as a
+    search only when it made as many launches of the kernel as the run's
+    longest attempt (every search of a run mines the same input); the
+    attempts that stopped early are left out of the totals.  A search's
+    launches past its last level exit at once: they dilute the per-launch
+    averages but not the per-search totals."""
+    sums, counts, copy, per_search = defaultdict(float), defaultdict(int), {}, {}
     for run in run_dirs:
-        for name, counters in _dispatches(run).values():
+        attempts = []   # per attempt: {counter: total} and the kernel's launch count
+        for _, (name, counters) in sorted(_dispatches(run).items()):
             if kernel in name:
                 for c, v in counters.items():
                     sums[c] += v
                     counts[c] += 1
+                if attempts:
+                    tot, n = attempts[-1]
+                    for c, v in counters.items():
+                        tot[c] += v
+                    attempts[-1] = (tot, n + 1)
             elif "copyBuffer" in name:
                 for c, v in counters.items():
                     copy[c] = max(copy.get(c, 0.0), v)   # the largest copy is the calibration
-    return {c: sums[c] / counts[c] for c in sums}, counts, copy
+            elif _SEARCH.search(name):
+                attempts.append((defaultdict(float), 0))
+        if attempts:
+            longest = max(n for _, n in attempts)
+            full = [tot for tot, n in attempts if n == longest and n > 0]
+            for c in full[0] if full else ():
+                per_search[c] = (sum(t[c] for t in full) / len(full), len(full))
+    return {c: sums[c] / counts[c] for c in sums}, counts, copy, per_search
 
 
 def counters_summary(kernel, run_dirs, command, sources, workload):
-    avg, counts, copy = counter_averages(run_dirs, kernel)
+    avg, counts, copy, per_search = counter_averages(run_dirs, kernel)
     lines = [f"# {kernel} counters, {workload}", "",
-             "Per-launch averages over every dispatch of the kernel in the runs below (rocprofv3 --pmc).",
+             "Per-launch averages over every dispatch of the kernel in the runs below (rocprofv3 --pmc),",
+             "and totals per search (over the searches of each run that ran to their last level;",
+             "launches past a search's last level exit at once and only dilute the per-launch averages).",
              "", "```", command, "```", ""]
     f = {"kernel": kernel, "workload": workload, "sources": " ".join(sources),
          "sources_sha256": sources_digest(sources)}
@@ -132,6 +161,22 @@ def counters_summary(kernel, run_dirs, command, sources, workload):
     if "fetch_size_bytes_per_launch" in f:
         f["hbm_bytes_per_launch"] = str(int(float(f["fetch_size_bytes_per_launch"])) +
                                         int(float(f.get("write_size_bytes_per_launch", 0))))
+    if per_search:
+        f["searches_per_run"] = " ".join(f"{c.lower()}={n}" for c, (_, n) in sorted(per_search.items()))
+        tot = {c: v for c, (v, _) in per_search.items()}
+        if "TCC_REQ_sum" in tot:
+            f["l2_request_bytes_per_search"] = f"{tot['TCC_REQ_sum'] * 128:.0f}"
+        if "TCC_EA0_RDREQ_sum" in tot:
+            f["fabric_read_bytes_per_search"] = f"{tot['TCC_EA0_RDREQ_sum'] * 128:.0f}"
+        if "TCC_EA0_WRREQ_sum" in tot:
+            f["fabric_write_bytes_per_search"] = f"{tot['TCC_EA0_WRREQ_sum'] * 64:.0f}"
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            if c in tot and copy.get(c):
+                scale = COPY_BYTES / (copy[c] * 1024)
+                f[f"{c.lower()}_bytes_per_search"] = f"{tot[c] * 1024 * scale:.0f}"
+        if "fetch_size_bytes_per_search" in f:
+            f["hbm_bytes_per_search"] = str(int(float(f["fetch_size_bytes_per_search"])) +
+                                            int(float(f.get("write_size_bytes_per_search", 0))))
     lines += [f"{k}: {v}" for k, v in f.items()]
     return "\n".join(lines) + "\n"
 
