@@ -1199,7 +1199,7 @@ struct GraphKey {
     const void *ptrs[64];
     uint64_t caps[4], slots;
     uint32_t G, te, tmask_w, tile_words, parts_tiles;
-    int periodic, prune;
+    int periodic, prune, ranges;
     bool operator==(const GraphKey &o) const { return memcmp(this, &o, sizeof(GraphKey)) == 0; }
 };
 
@@ -1208,6 +1208,7 @@ struct SearchWs {
     DBuf ctl, dbs, fcol, pairbits, tmask, gram, goff, gjobs, gdbs, l2, part, partl, scatter, ts, stamps;
     DBuf pat;             // pattern forest: parent, item, support and database columns, pat_pitch() bytes apart
     DBuf filter;          // subset-test Bloom filter (filter_words() words)
+    DBuf ranges;          // per work range of the level being counted: first task, most tiles
     bool filter_dirty = true;  // holds bits of an attempt that stopped early
     DBuf pat_per;
     HBuf host;
@@ -1265,6 +1266,9 @@ struct Search {
     uint32_t parts_tiles = 0;
     Caps need{};                     // level 2's exact sizes when they overflowed
     uint64_t slots = 0;              // retained rows per level set (0: none)
+    // Work items on tiles some databases do not reach exist (a batch of
+    // databases with different row lengths): count through a range table.
+    bool use_ranges = false;
     // Large results are streamed: once a search has found enough patterns
     // for the direct download, each finished chunk's patterns are copied into
     // the result's pinned arrays (sized to the pattern capacity) on a second
@@ -1404,6 +1408,10 @@ struct Search {
             a.rout = L.rows.as<uint64_t>();
             a.slots = slots;
         }
+        if (use_ranges) {
+            a.ranges = w.ranges.as<uint32_t>();
+            a.rcap = w.caps.ext + 1;   // a level has at most caps.ext entries, so as many ranges
+        }
         return a;
     }
 
@@ -1413,6 +1421,7 @@ struct Search {
 
     int count(int c, bool level1) {
         SupportArgs a = support_args(c, level1);
+        CPM_CHECK(launch_ranges(ctx, a, grid()));
         if (periodic) return launch_support_periodic(ctx, a, grid());
         return launch_support_count(ctx, a, tile_words, grid());
     }
@@ -1450,7 +1459,7 @@ struct Search {
         }
         add(w.ctl); add(w.dbs); add(w.fcol); add(w.pairbits); add(w.tmask); add(w.part); add(w.partl);
         add(w.stamps);
-        add(w.pat); add(w.pat_per); add(w.filter);
+        add(w.pat); add(w.pat_per); add(w.filter); add(w.ranges);
         k.caps[0] = w.caps.tasks;
         k.caps[1] = w.caps.ext;
         k.caps[2] = w.caps.patterns;
@@ -1463,6 +1472,7 @@ struct Search {
         k.parts_tiles = parts_tiles;
         k.periodic = periodic;
         k.prune = prune;
+        k.ranges = use_ranges;
         return k;
     }
 
@@ -1522,6 +1532,7 @@ struct Search {
         }
         if (periodic) CPM_CHECK(w.pat_per.ensure((w.caps.patterns + 1) * 8));
         CPM_CHECK(w.part.ensure((size_t)3 * grid_tasks() * 8));
+        CPM_CHECK(w.ranges.ensure((size_t)2 * (w.caps.ext + 1) * 4));
         CPM_CHECK(w.partl.ensure((size_t)grid() * 8));
         CPM_CHECK(w.ctl.ensure(kCtlBytes + (2 * in.size() + 1) * 4));
         CPM_CHECK(w.stamps.ensure((size_t)grid() * 4 * 8));
@@ -1647,6 +1658,8 @@ struct Search {
             D.ts = in[i].d_ts;
             total_items += b->n_items;
         }
+        use_ranges = false;
+        for (const DbDev &D : dbs) use_ranges = use_ranges || (n_db > 1 && D.tiles < te);
         CPM_CHECK(w.dbs.ensure(std::max<uint32_t>(n_db, 1) * sizeof(DbDev)));
         qput(w.dbs.p, dbs.data(), n_db * sizeof(DbDev));
 

@@ -64,6 +64,8 @@ struct LevelView {
     const uint64_t *rin;   // retained rows (SupportArgs)
     uint64_t *rout;
     uint64_t slots;
+    cptr<uint32_t> rtask, rtiles;  // per work range (SupportArgs::ranges), or null
+    uint64_t rcap;
 };
 
 // false when the launch has nothing to do (the same answer in every block)
@@ -88,7 +90,24 @@ __device__ __forceinline__ bool level_view(const SupportArgs &a, LevelView &v) {
     v.src = v.rin ? to_const(a.L.src) : nullptr;
     v.rout = a.rout;
     v.slots = a.slots;
+    v.rtask = a.ranges ? to_const(a.ranges) : nullptr;
+    v.rtiles = a.ranges ? to_const(a.ranges + a.rcap) : nullptr;
+    v.rcap = a.ranges ? a.rcap : 0;
     return true;
+}
+
+// The work range of item `it` reaches no database tile `tile` (nothing to
+// count), from the range table; false when there is no table entry.
+__device__ __forceinline__ bool range_empty(const LevelView &v, uint64_t it, uint32_t tile) {
+    const uint64_t g = it / v.te;
+    return g < v.rcap && tile >= v.rtiles[g];
+}
+
+// the first task of item `it`'s range: from the range table, else by binary
+// search
+__device__ __forceinline__ uint32_t range_task(const LevelView &v, uint64_t it, uint32_t r0) {
+    const uint64_t g = it / v.te;
+    return g < v.rcap ? v.rtask[g] : find_task(v.ext_off, v.n_tasks, r0);
 }
 
 // Work item `it`: entries [r0, r1) on tile `tile`.
@@ -254,9 +273,10 @@ void k_support_count(SupportArgs a) {
     unsigned long long bytes = 0, cands = 0;
     for (uint64_t it = blockIdx.x, k = 0; it < v.total; it += gridDim.x, ++k) {
         const Range r = range_of(v, it, k);
+        if (range_empty(v, it, r.tile)) continue;  // the same answer in every lane
         const uint32_t ne = r.r1 - r.r0;  // <= 64 (the group is at most 64)
         const uint32_t colv = lane < ne ? col_of(v.fcol, v.ext[r.r0 + lane]) : 0u;
-        uint32_t t = find_task(v.ext_off, v.n_tasks, r.r0);
+        uint32_t t = range_task(v, it, r.r0);
         if (KEEP && short_rows<QPL>(a, v, r, t, colv, emap, &ebytes, bytes, cands)) continue;
         // The AND of the first d - 1 prefix rows of the last task built here
         // (its parent's itemset), reused while the next tasks are its
@@ -504,7 +524,8 @@ __global__ __launch_bounds__(256) void k_support_periodic(SupportArgs a) {
     unsigned long long bytes = 0, cands = 0;
     for (uint64_t it = blockIdx.x, k = 0; it < lv.total; it += gridDim.x, ++k) {
         const Range r = range_of(lv, it, k);
-        uint32_t t = find_task(lv.ext_off, lv.n_tasks, r.r0);
+        if (range_empty(lv, it, r.tile)) continue;
+        uint32_t t = range_task(lv, it, r.r0);
         for (uint32_t e0 = r.r0; e0 < r.r1;) {
             while (lv.ext_off[t + 1] <= e0) ++t;  // empty tasks
             const uint32_t e1 = min(r.r1, lv.ext_off[t + 1]);
@@ -588,6 +609,49 @@ __global__ __launch_bounds__(256) void k_periodic_finalize(SupportArgs a) {
     }
 }
 
+// ---- range table ------------------------------------------------------------------
+
+// One wave per work range: its first task by a 64-ary search over the task
+// offsets (each step narrows the interval 64-fold with one load per lane;
+// three steps for 2^18 tasks), then the tiles of its tasks' databases, 64
+// tasks per load.
+__global__ __launch_bounds__(256) void k_ranges(SupportArgs a) {
+    LevelView v;
+    if (!level_view(a, v) || !a.ranges) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t n_r = ((uint64_t)v.n_ext + v.G - 1) / v.G;
+    const uint64_t n = n_r < a.rcap ? n_r : a.rcap;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    for (uint64_t g = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < n; g += waves) {
+        const uint32_t r0 = (uint32_t)(g * v.G);
+        const uint32_t r1 = min(v.n_ext, r0 + v.G);
+        // the largest t < n_tasks with ext_off[t] <= r0 lies in [lo, hi)
+        // (ext_off[0] = 0 <= r0 < ext_off[n_tasks] = n_ext)
+        uint32_t lo = 0, hi = v.n_tasks;
+        while (hi - lo > 1) {
+            const uint32_t step = (hi - lo + 63) / 64;
+            const uint32_t p = lo + lane * step;
+            const unsigned long long le = __ballot(p < hi && v.ext_off[p] <= r0);  // a prefix of lanes, lane 0 set
+            const uint32_t k = 63u - (uint32_t)__builtin_clzll(le);
+            lo += k * step;
+            hi = min(hi, lo + step);
+        }
+        uint32_t tiles = 0;
+        for (uint32_t base = lo;; base += 64) {
+            const uint32_t t = base + lane;
+            const bool in = t < v.n_tasks && v.ext_off[t] < r1;
+            if (in && v.ext_off[t + 1] > v.ext_off[t]) tiles = max(tiles, v.dbs[v.dbi[t]].tiles);
+            if (__ballot(in) != ~0ull) break;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) tiles = max(tiles, (uint32_t)__shfl_xor((int)tiles, off, 64));
+        if (lane == 0) {
+            a.ranges[g] = lo;
+            a.ranges[a.rcap + g] = tiles;
+        }
+    }
+}
+
 // ---- operator: child rows of one prefix -----------------------------------------
 
 __global__ __launch_bounds__(256) void k_intersect(const uint64_t *__restrict__ prefix,
@@ -650,6 +714,13 @@ int launch_support_periodic(cpm_ctx *ctx, const SupportArgs &a, uint32_t grid) {
                        ctx->stream, a);
     CPM_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_periodic_finalize, dim3(grid), dim3(256), 0, ctx->stream, a);
+    CPM_HIP(hipGetLastError());
+    return CPM_OK;
+}
+
+int launch_ranges(cpm_ctx *ctx, const SupportArgs &a, uint32_t grid) {
+    if (!a.ranges || a.te == 0) return CPM_OK;
+    hipLaunchKernelGGL(k_ranges, dim3(std::max<uint32_t>(1, grid / 8)), dim3(256), 0, ctx->stream, a);
     CPM_HIP(hipGetLastError());
     return CPM_OK;
 }

@@ -103,6 +103,14 @@ struct SupportArgs {
     const uint64_t *rin = nullptr;
     uint64_t *rout = nullptr;
     uint64_t slots = 0;
+    // Per work range r (entries [r G, r G + G)), filled by launch_ranges
+    // before the count: ranges[r] = its first task, ranges[rcap + r] = the
+    // most tiles any of its tasks' databases has, so a work item on a tile
+    // no database of its range reaches is skipped with one load.  Null (or
+    // r >= rcap): the kernel finds the task by binary search and walks the
+    // range's tasks.
+    uint32_t *ranges = nullptr;
+    uint64_t rcap = 0;
 };
 
 // Entries per work range for a level of n_ext entries: at most G, and small
@@ -125,6 +133,9 @@ inline uint32_t tiles_eff_of(uint32_t tiles) { return tiles >= 8 ? (tiles + 7) /
 int launch_support_count(cpm_ctx *ctx, const SupportArgs &a, uint32_t tile_words, uint32_t grid);
 // support and periodicity of every entry (partials, then an in-order fold)
 int launch_support_periodic(cpm_ctx *ctx, const SupportArgs &a, uint32_t grid);
+// a.ranges for the level a describes (sizes read from device memory, so it
+// can be replayed from a graph); nothing when a.ranges is null
+int launch_ranges(cpm_ctx *ctx, const SupportArgs &a, uint32_t grid);
 
 // child_j = prefix & ext_j (rows of `stride` words), sup[j] = popcount.
 // out may be null (count only).
