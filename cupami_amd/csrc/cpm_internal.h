@@ -303,7 +303,8 @@ __device__ __forceinline__ cptr<T> to_const(const T *p) {
 // ---- wave sums and scans on DPP ------------------------------------------------
 // Data-parallel moves between lanes of a row of 16 inside one VALU
 // instruction: no LDS round trip and no wait between steps (a __shfl is an
-// LDS permute with a wait after it).  Every lane of the wave must be active.
+// LDS permute with a wait after it).  Every lane of a row the result is read
+// from must be active (of the wave, for the wave-wide ones).
 
 // Row sums: after the two quad swaps and the half-row and row mirrors every
 // lane of a row of 16 holds the sum of its row.
@@ -322,7 +323,18 @@ __device__ __forceinline__ uint32_t dpp_wave_sum(uint32_t v) {
            (uint32_t)__builtin_amdgcn_readlane((int)v, 32) + (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
 }
 
-__device__ __forceinline__ unsigned long long dpp_wave_sum64(unsigned long long v) {
+// Inclusive scan inside each row of 16 lanes (lanes shifted in from outside
+// the row read 0).
+__device__ __forceinline__ uint32_t dpp_row_scan_incl(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);  // row_shr:8
+    return x;
+}
+
+// Row sums of 64-bit values (each half moved, then a 64-bit add).
+__device__ __forceinline__ unsigned long long dpp_row_sum64(unsigned long long v) {
     auto step = [&](auto ctrl) {
         constexpr int c = decltype(ctrl)::value;
         const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)v, c, 0xf, 0xf, false);
@@ -333,6 +345,11 @@ __device__ __forceinline__ unsigned long long dpp_wave_sum64(unsigned long long 
     step(std::integral_constant<int, 0x4e>{});
     step(std::integral_constant<int, 0x141>{});
     step(std::integral_constant<int, 0x140>{});
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long dpp_wave_sum64(unsigned long long v) {
+    v = dpp_row_sum64(v);
     unsigned long long t = 0;
 #pragma unroll
     for (int r = 0; r < 64; r += 16)
@@ -346,10 +363,7 @@ __device__ __forceinline__ unsigned long long dpp_wave_sum64(unsigned long long 
 // 0 and 2 added into rows 1 and 3, then the last lane of row 1 into rows 2
 // and 3 (rows outside a move's row mask take 0).
 __device__ __forceinline__ uint32_t dpp_wave_scan_incl(uint32_t x) {
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);  // row_shr:1
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);  // row_shr:2
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);  // row_shr:4
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);  // row_shr:8
+    x = dpp_row_scan_incl(x);
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false); // row_bcast:15
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false); // row_bcast:31
     return x;

@@ -237,6 +237,53 @@ __device__ __forceinline__ uint32_t small_child_count(const uint32_t *wf, uint32
     return c;
 }
 
+// Tasks of at most kTinyTask entries are planned four to a wave: each row of
+// 16 lanes (a DPP row) takes one task, so a wave walks a quarter of the
+// dependent load chains it would walk one task at a time (the deep levels of
+// a batch are hundreds of thousands of tasks of a few entries).
+constexpr uint32_t kTinyTask = 16;
+
+// small_child_count for the task of one row of 16 lanes: wf, rw, rm are the
+// row's 16-entry slices, nk <= 16.  The row's lanes call it together (rows
+// of a wave may take other branches).
+__device__ __forceinline__ uint32_t tiny_child_count(const uint32_t *wf, uint32_t *rw, uint32_t *rm, uint32_t nk,
+                                                     const uint32_t *table, uint32_t pair_w32, uint32_t *trow) {
+    const uint32_t lane = threadIdx.x & 63, lg = lane & 15, sh = lane & 48;
+    const uint32_t f = lg < nk ? wf[lg] : 0u;
+    const uint32_t q = f >> 5;
+    const bool head = lg < nk && (lg == 0 || (wf[lg - 1] >> 5) != q);
+    const uint32_t heads = (uint32_t)(__ballot(head) >> sh) & 0xffffu;
+    const uint32_t nrun = (uint32_t)__popc(heads);
+    const uint32_t run = (uint32_t)__popc(heads & ((2u << lg) - 1)) - 1u;  // lg < nk: its run
+    if (head) {
+        rw[run] = q;
+        rm[run] = 0;
+    }
+    wave_sync();
+    if (lg < nk) atomicOr(&rm[run], 1u << (f & 31));
+    wave_sync();
+    if (trow)
+        for (uint32_t w = lg; w < pair_w32; w += 16) {
+            uint32_t lo = 0, hi = nrun;  // the run of word w, if any
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (rw[mid] < w) lo = mid + 1; else hi = mid;
+            }
+            trow[w] = lo < nrun && rw[lo] == w ? rm[lo] : 0u;
+        }
+    uint32_t c = 0;
+    if (lg + 1 < nk) {
+        const uint32_t *prow = table + (uint64_t)f * pair_w32;
+        for (uint32_t r = run; r < nrun; ++r) {
+            uint32_t m = rm[r] & prow[rw[r]];
+            if (r == run) m &= above(f);
+            c += __popc(m);
+        }
+    }
+    wave_sync();  // the runs are read before the row's next task rewrites them
+    return c;
+}
+
 // sum of part[0 .. b), in every lane of the block
 __device__ __forceinline__ unsigned long long sum_before(const unsigned long long *part, uint32_t b,
                                                          unsigned long long *red) {
@@ -324,11 +371,53 @@ __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
     for (uint32_t w = tid; w < kMaskWords; w += blockDim.x) lmask[w] = 0;
     __syncthreads();
     unsigned long long s_pat = 0, s_child = 0, s_cext = 0;  // this lane's share of the block sums
+    // tiny tasks, one row of 16 lanes each: the same steps as a small task
+    // with the row's share of the ballot and a row scan
+    {
+        const uint32_t grp = lane >> 4, lg = lane & 15, sh = lane & 48;
+        for (uint64_t tb = lo + 4ull * wave; tb < hi; tb += 16) {
+            const uint64_t tt = tb + grp;
+            const uint32_t t = (uint32_t)(tt < hi ? tt : lo);
+            const uint32_t a = p.L.ext_off[t], b = p.L.ext_off[t + 1];
+            const bool tiny = tt < hi && b - a <= kTinyTask;
+            const DbDev &D = p.dbs[p.L.db[t]];
+            const uint32_t e = a + lg;
+            const bool k = tiny && e < b && keep_entry(p, D, e);
+            const uint32_t bal = (uint32_t)(__ballot(k) >> sh) & 0xffffu;
+            const uint32_t nk = (uint32_t)__popc(bal);
+            uint32_t *const wfr = &wf[wave][sh], *const rwr = &rw[wave][sh], *const rmr = &rm[wave][sh];
+            wave_sync();  // the previous tasks' reads of wf are done
+            if (k) {
+                const uint32_t pos = (uint32_t)__popc(bal & ((1u << lg) - 1));
+                p.kept[a + pos] = e;
+                wfr[pos] = p.L.ext[e] - D.fbase;
+            }
+            wave_sync();
+            const uint32_t nc = (tiny && p.emit_children && nk >= 2) ? nk - 1 : 0;
+            uint32_t c = 0;
+            if (nc && p.pairbits) {
+                uint32_t *trow = p.tmask ? p.tmask + (uint64_t)t * p.tmask_w : nullptr;
+                c = tiny_child_count(wfr, rwr, rmr, nk, p.pairbits + D.pair_off, D.pair_w32, trow);
+                if (lg < nc) p.ccount[a + lg] = c;
+            } else if (lg < nc) {
+                c = nk - 1 - lg;
+            }
+            const uint32_t cext = dpp_row_sum(c);
+            if (tiny && lg == 0) {
+                p.cnt_pat[t] = nk;
+                p.cnt_child[t] = nc;
+                p.cnt_cext[t] = cext;
+                s_pat += nk;
+                s_child += nc;
+                s_cext += cext;
+            }
+        }
+    }
     // small tasks, one wave each: keep flags by ballot, child counts from the
     // pair table bit by bit
     for (uint32_t t = (uint32_t)lo + wave; t < hi; t += 4) {
         const uint32_t a = p.L.ext_off[t], b = p.L.ext_off[t + 1];
-        if (b - a > kSmallTask) continue;
+        if (b - a > kSmallTask || b - a <= kTinyTask) continue;
         const DbDev &D = p.dbs[p.L.db[t]];
         const uint32_t e = a + lane;
         const bool k = e < b && keep_entry(p, D, e);
@@ -576,10 +665,94 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
     const bool fill = p.tmask && p.pairbits;  // k_child_fill writes the child entries
     for (uint32_t w = tid; w < kMaskWords; w += blockDim.x) lmask[w] = 0;
     __syncthreads();
+    // tiny tasks, one row of 16 lanes each: lane j writes kept entry j and
+    // its child; the prefix's d <= 64 items are held four per lane
+    {
+        const uint32_t grp = lane >> 4, lg = lane & 15, sh = lane & 48;
+        for (uint64_t tb = 4ull * (blockIdx.x * 4 + wave); tb < n_tasks; tb += 16ull * gridDim.x) {
+            const uint64_t tt = tb + grp;
+            const uint32_t t = (uint32_t)(tt < n_tasks ? tt : tb);
+            const uint32_t nk_t = p.cnt_pat[t];
+            if (!(tt < n_tasks && nk_t <= kTinyTask)) continue;  // the row's lanes together
+            const uint32_t nk = nk_t;
+            const uint32_t dbi = p.L.db[t];
+            const DbDev &D = p.dbs[dbi];
+            const uint32_t a = p.L.ext_off[t];
+            const uint64_t pb = pat0 + p.off_pat[t];
+            uint32_t sv[4];  // prefix item lg + 16 m
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const uint32_t j = lg + 16 * m;
+                sv[m] = j < d ? p.L.prefix[(uint64_t)t * kMaxDepth + j] : 0u;
+            }
+            unsigned long long psum = 0;  // sum of the prefix items' filter terms
+            if (stage) {
+                unsigned long long v = 0;
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    if (lg + 16 * m < d) v += item_term(sv[m]);
+                psum = dpp_row_sum64(v);
+            }
+            uint32_t f = 0, e = 0;
+            if (lg < nk) {
+                e = p.kept[a + lg];
+                f = p.L.ext[e];
+                p.pat_parent[pb + lg] = p.L.pid[t];
+                p.pat_item[pb + lg] = p.fcol ? p.fcol[f] : f;
+                p.pat_sup[pb + lg] = p.L.sup[e];
+                p.pat_db[pb + lg] = dbi;
+                if (p.pat_per) p.pat_per[pb + lg] = p.L.per[e];
+                if (stage) filter_insert(p.filter, fmask, set_hash(psum + item_term(f)));
+            }
+            if (!(children && nk >= 2)) continue;
+            uint32_t *const wfr = &wf[wave][sh];
+            if (!fill) {
+                wave_sync();  // the previous task's reads of wf are done
+                if (lg < nk) wfr[lg] = f - D.fbase;
+                wave_sync();
+            }
+            const uint32_t *table = p.pairbits ? p.pairbits + D.pair_off : nullptr;
+            const bool live = lg + 1 < nk;
+            const uint32_t fl = f - D.fbase;
+            const uint32_t *prow = table && live ? table + (uint64_t)fl * D.pair_w32 : nullptr;
+            uint32_t cj = 0;
+            if (live) cj = table ? p.ccount[a + lg] : nk - 1 - lg;
+            const uint32_t off = dpp_row_scan_incl(cj) - cj;
+            // child c's prefix row: this task's prefix and item f of the row's
+            // lane c (d + 1 <= 64 when children are emitted)
+            const uint32_t cb = p.off_child[t];
+            for (uint32_t c = 0; c + 1 < nk; ++c) {
+                const uint32_t fc = (uint32_t)__shfl((int)f, (int)(sh + c), 64);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const uint32_t j = lg + 16 * m;
+                    if (j <= d) p.N.prefix[(uint64_t)(cb + c) * kMaxDepth + j] = j < d ? sv[m] : fc;
+                }
+            }
+            if (live) {
+                const uint32_t ct = cb + lg;
+                p.N.par[ct] = t;
+                p.N.src[ct] = e;
+                p.N.pid[ct] = (uint32_t)(pb + lg);
+                p.N.db[ct] = dbi;
+                uint32_t w = p.off_cext[t] + off;
+                odst[ct] = w;
+                if (!fill) {
+                    for (uint32_t l = lg + 1; l < nk; ++l) {
+                        const uint32_t g = wfr[l];
+                        if (table && !pair_bit(prow, g)) continue;
+                        xdst[w] = D.fbase + g;
+                        if (!stage) p.N.sup[w] = 0;
+                        ++w;
+                    }
+                }
+            }
+        }
+    }
     // small tasks, one wave each: lane j writes kept entry j and its child
     for (uint32_t t = blockIdx.x * 4 + wave; t < n_tasks; t += gridDim.x * 4) {
         const uint32_t nk = p.cnt_pat[t];
-        if (nk > kSmallTask) continue;
+        if (nk > kSmallTask || nk <= kTinyTask) continue;
         const uint32_t dbi = p.L.db[t];
         const DbDev &D = p.dbs[dbi];
         const uint32_t a = p.L.ext_off[t];
