@@ -742,7 +742,8 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
                         const uint32_t g = wfr[l];
                         if (table && !pair_bit(prow, g)) continue;
                         xdst[w] = D.fbase + g;
-                        if (!stage) p.N.sup[w] = 0;
+                        if (stage) p.pos[w] = ct;
+                        else p.N.sup[w] = 0;
                         ++w;
                     }
                 }
@@ -809,7 +810,8 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
                     const uint32_t g = wf[wave][l];
                     if (table && !pair_bit(prow, g)) continue;
                     xdst[w] = D.fbase + g;
-                    if (!stage) p.N.sup[w] = 0;
+                    if (stage) p.pos[w] = ct;
+                    else p.N.sup[w] = 0;
                     ++w;
                 }
             }
@@ -905,7 +907,8 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
                             if (q == (fl >> 5)) m &= above(fl);
                             while (m) {
                                 xdst[w] = D.fbase + q * 32 + (uint32_t)__builtin_ctz(m);
-                                if (!stage) p.N.sup[w] = 0;
+                                if (stage) p.pos[w] = ct;
+                                else p.N.sup[w] = 0;
                                 ++w;
                                 m &= m - 1;
                             }
@@ -914,7 +917,8 @@ __global__ __launch_bounds__(256) void k_plan_write(PlanArgs p) {
                 } else {
                     for (uint32_t l = j + 1; l < nk; ++l) {
                         xdst[w] = p.L.ext[p.kept[a + l]];
-                        if (!stage) p.N.sup[w] = 0;
+                        if (stage) p.pos[w] = ct;
+                        else p.N.sup[w] = 0;
                         ++w;
                     }
                 }
@@ -948,7 +952,7 @@ __device__ __forceinline__ void child_fill_groups(const PlanArgs &p, uint32_t n_
     const uint32_t lane = threadIdx.x & 63, sub = lane % S, g = lane / S;
     const uint32_t groups = gridDim.x * (blockDim.x >> 6) * G;
     for (uint32_t ct = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * G + g; ct < n_child; ct += groups) {
-        const uint32_t t = upper_task(p.off_child, n_tasks, ct);
+        const uint32_t t = min(p.N.par[ct], n_tasks - 1);  // written by plan/write
         const DbDev &D = p.dbs[p.L.db[t]];
         const uint32_t fl = p.N.prefix[(uint64_t)ct * kMaxDepth + d] - D.fbase;
         const uint32_t *mask = p.tmask + (uint64_t)t * p.tmask_w;
@@ -971,7 +975,8 @@ __device__ __forceinline__ void child_fill_groups(const PlanArgs &p, uint32_t n_
             uint32_t pos = w + x - c;
             while (m) {
                 xdst[pos] = D.fbase + q * 32 + (uint32_t)__builtin_ctz(m);
-                if (!stage) p.N.sup[pos] = 0;
+                if (stage) p.pos[pos] = ct;  // the entry's child, for the subset test
+                else p.N.sup[pos] = 0;
                 ++pos;
                 m &= m - 1;
             }
@@ -1005,7 +1010,9 @@ __global__ __launch_bounds__(256) void k_prune_flags(PlanArgs p) {
     chunk_of(n_cand, blockIdx.x, gridDim.x, &lo, &hi);
     unsigned long long c = 0;
     for (uint64_t w = lo + threadIdx.x; w < hi; w += blockDim.x) {
-        const uint32_t ct = upper_task(p.off2, n_child, w);
+        // the entry's child, left in pos by whichever kernel wrote the entry
+        // (clamped: an index out of range could only come from a missed write)
+        const uint32_t ct = min(p.pos[w], n_child - 1);
         const uint32_t *pre = p.N.prefix + (uint64_t)ct * kMaxDepth;
         const uint32_t x = p.ext2[w];
         // the subsets without the last prefix item or without x are kept by

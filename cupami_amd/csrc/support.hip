@@ -720,7 +720,7 @@ int launch_support_periodic(cpm_ctx *ctx, const SupportArgs &a, uint32_t grid) {
 
 int launch_ranges(cpm_ctx *ctx, const SupportArgs &a, uint32_t grid) {
     if (!a.ranges || a.te == 0) return CPM_OK;
-    hipLaunchKernelGGL(k_ranges, dim3(std::max<uint32_t>(1, grid / 8)), dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_ranges, dim3(std::max<uint32_t>(1, grid)), dim3(256), 0, ctx->stream, a);
     CPM_HIP(hipGetLastError());
     return CPM_OK;
 }
