@@ -376,16 +376,40 @@ struct GramDb {
     uint32_t n_items, pad;
     uint64_t out_off;
 };
+// One workgroup's share of the pair matrix: a 256 x 256 block of database
+// db over the K (word) range [k_begin, k_end).  A block cut into several K
+// slices stores each slice's sums in its own partial slot (256 x 256 int32)
+// and k_gram's reduce adds them; a block of one slice writes its outputs
+// (slot = kGramDirect).
+constexpr uint32_t kGramDirect = 0xffffffffu;
 struct GramJob {
-    uint32_t db, bi, bj, pad;
+    uint32_t db, bi, bj, slot;
     uint64_t k_begin, k_end;
 };
-// Work table for a batch of databases; *out_elems = sum of n_items^2.
-int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<GramJob> &jobs,
-              uint64_t *out_elems);
-// Run the jobs (device tables) into `out`, zeroing it first when `clear` (the
-// kernel adds its K slices); n_ops feeds the statistics.
-int launch_gram(cpm_ctx *ctx, const GramJob *d_jobs, uint64_t n_jobs, const GramDb *d_dbs,
-                int32_t *d_out, uint64_t out_elems, uint64_t n_ops, bool clear);
+// a block of several slices: partial slots [slot0, slot0 + nslots)
+struct GramBlock {
+    uint32_t db, bi, bj, slot0, nslots, pad;
+};
+struct GramPlan {
+    std::vector<GramJob> jobs;
+    std::vector<GramBlock> blocks;
+    uint64_t slots = 0;      // partial slots (256 x 256 int32 each)
+    uint64_t out_elems = 0;  // sum of n_items^2
+};
+// Work table for a batch of databases.
+int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, GramPlan &plan);
+// Device copies of a plan's tables and its partial slots.
+struct GramDev {
+    const GramJob *jobs = nullptr;
+    uint64_t n_jobs = 0;
+    const GramBlock *blocks = nullptr;
+    uint64_t n_blocks = 0;
+    int32_t *partials = nullptr;
+};
+// Run the jobs into `out` (the upper triangle of each database's matrix;
+// the rest is left as it is, zeroed first when `clear`); n_ops feeds the
+// statistics.
+int launch_gram(cpm_ctx *ctx, const GramDev &g, const GramDb *d_dbs, int32_t *d_out, uint64_t out_elems,
+                uint64_t n_ops, bool clear);
 
 }  // namespace cpm

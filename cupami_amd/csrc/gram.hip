@@ -26,8 +26,11 @@
 // next K tile is loaded into registers and stashed into the other buffer
 // while the current one is multiplied, one barrier per tile.  On diagonal
 // blocks the six pieces with work are spread 1.5 per SIMD (two waves share
-// two of them, each taking every other K group).  Slices are summed with
-// integer atomics.  One launch serves any number of databases:
+// two of them, each taking every other K group, and their halves are added
+// through LDS at the end).  A block cut into several K slices stores each
+// slice into a partial slot and a second launch adds the slots (plain
+// stores and one pass instead of per-element atomics from every slice); a
+// block of one slice writes its outputs.  One launch serves any number of databases:
 // the job table lists each workgroup's (database, block row, block column,
 // K range).
 #include "cpm_internal.h"
@@ -74,7 +77,7 @@ __device__ __forceinline__ int plane_scale(int p) {
 }
 
 __global__ __launch_bounds__(kThreads) void k_gram(const GramJob *__restrict__ jobs, const GramDb *__restrict__ dbs,
-                                                   int32_t *__restrict__ out) {
+                                                   int32_t *__restrict__ out, int32_t *__restrict__ partials) {
     __shared__ uint32_t sA[2][kBT][kLdsW];  // two buffers of A and B tiles: 144 KB
     __shared__ uint32_t sB[2][kBT][kLdsW];
     const uint32_t tid = threadIdx.x, lane = tid & 63;
@@ -206,20 +209,74 @@ __global__ __launch_bounds__(kThreads) void k_gram(const GramJob *__restrict__ j
         __syncthreads();
         cur ^= 1;
     }
+    // On a diagonal block waves 6 and 7 hold the odd K groups of the pieces
+    // waves 4 and 5 hold the even ones: those halves are added through LDS
+    // (the tiles are free once the K loop's last barrier is passed), so one
+    // wave stores each piece.
+    const bool diag = bi == bj;
+    if (diag) {
+        float *xch = reinterpret_cast<float *>(&sA[0][0][0]);  // 2 waves x 128 x 64 sums: 64 KB
+        static_assert(2 * kTX * kTY * 16 * 64 * sizeof(float) <= sizeof(sA), "exchange fits the A tiles");
+        auto at = [&](uint32_t w, int x, int y, int i) { return ((w * kTX * kTY + x * kTY + y) * 16 + i) * 64 + lane; };
+        if (wave >= 6)
+#pragma unroll
+            for (int x = 0; x < kTX; ++x)
+#pragma unroll
+                for (int y = 0; y < kTY; ++y)
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) xch[at(wave - 6, x, y, i)] = acc[x][y][i];
+        __syncthreads();
+        if (wave == 4 || wave == 5)
+#pragma unroll
+            for (int x = 0; x < kTX; ++x)
+#pragma unroll
+                for (int y = 0; y < kTY; ++y)
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) acc[x][y][i] += xch[at(wave - 4, x, y, i)];
+    }
+    if (diag && wave >= 6) return;
     // D tile layout of the 32 x 32 form: element i of lane l is row
-    // 8 (i / 4) + 4 (l / 32) + i % 4, column l % 32
+    // 8 (i / 4) + 4 (l / 32) + i % 4, column l % 32.  A block of one K slice
+    // writes its upper-triangle outputs; a slice of a longer block stores
+    // the whole piece into its partial slot (coalesced rows), for the reduce.
+    int32_t *const slot = job.slot == kGramDirect ? nullptr : partials + (uint64_t)job.slot * kBT * kBT;
 #pragma unroll
     for (int x = 0; x < kTX; ++x)
 #pragma unroll
         for (int y = 0; y < kTY; ++y) {
-            const uint32_t col = bj * kBT + wc * kWaveCols + y * 32 + r32;
+            const uint32_t cl = wc * kWaveCols + y * 32 + r32;
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-                const uint32_t row = bi * kBT + wr * kWaveRows + x * 32 + 8 * (i / 4) + 4 * h + (i % 4);
+                const uint32_t rl = wr * kWaveRows + x * 32 + 8 * (i / 4) + 4 * h + (i % 4);
                 const int v = (int)acc[x][y][i];
-                if (row < n && col < n && row <= col && v) atomicAdd(res + (uint64_t)row * ld + col, v);
+                if (slot) {
+                    slot[rl * kBT + cl] = v;
+                } else {
+                    const uint32_t row = bi * kBT + rl, col = bj * kBT + cl;
+                    if (row < n && col < n && row <= col) res[(uint64_t)row * ld + col] = v;
+                }
             }
         }
+}
+
+// The upper-triangle outputs of the blocks cut into K slices: the sum of
+// their slots.
+__global__ __launch_bounds__(256) void k_gram_reduce(const GramBlock *__restrict__ blocks, uint64_t n_blocks,
+                                                     const GramDb *__restrict__ dbs,
+                                                     const int32_t *__restrict__ partials, int32_t *__restrict__ out) {
+    constexpr uint64_t kE = (uint64_t)kBT * kBT;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_blocks * kE;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const GramBlock b = blocks[i / kE];
+        const uint32_t e = (uint32_t)(i % kE), rl = e / kBT, cl = e % kBT;
+        const uint32_t n = dbs[b.db].n_items;
+        const uint32_t row = b.bi * kBT + rl, col = b.bj * kBT + cl;
+        if (row >= n || col >= n || row > col) continue;
+        const int32_t *p = partials + (uint64_t)b.slot0 * kE + e;
+        int32_t sum = 0;
+        for (uint32_t k = 0; k < b.nslots; ++k) sum += p[k * kE];
+        out[dbs[b.db].out_off + (uint64_t)row * n + col] = sum;
+    }
 }
 
 }  // namespace
@@ -236,9 +293,11 @@ constexpr int diagonal_active_waves() {
 }
 
 
-int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<GramJob> &jobs,
-              uint64_t *out_elems) {
+int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, GramPlan &plan) {
+    std::vector<GramJob> &jobs = plan.jobs;
     jobs.clear();
+    plan.blocks.clear();
+    plan.slots = 0;
     // work of a block: its K tiles x the share of its waves with work
     // (a diagonal block's six pieces are spread 1.5 per SIMD, so it takes
     // 6/8 of a full block's time)
@@ -252,7 +311,7 @@ int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<Gr
         const double kt = (double)((d.words + kKW - 1) / kKW);
         total += kt * ((double)nb * (nb - 1) / 2 + diag * nb);
     }
-    *out_elems = out;
+    plan.out_elems = out;
     // One workgroup per CU at a time (the accumulators take half the
     // register file): cut each block's K range in proportion to its work so
     // that the jobs fill one round of CUs evenly.
@@ -272,22 +331,38 @@ int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, std::vector<Gr
                 uint64_t slices = per_job > 0 ? (uint64_t)(work / per_job) : 1;
                 slices = std::max<uint64_t>(std::min<uint64_t>(std::max<uint64_t>(slices, 1), ktiles), exact);
                 const uint64_t slice_words = (ktiles + slices - 1) / slices * kKW;
-                for (uint64_t k0 = 0; k0 < d.words; k0 += slice_words)
-                    jobs.push_back({i, bi, bj, 0, k0, std::min<uint64_t>(d.words, k0 + slice_words)});
+                const uint64_t n_slices = (d.words + slice_words - 1) / slice_words;
+                const uint64_t slot0 = plan.slots;
+                if (n_slices > 1) {
+                    plan.blocks.push_back({i, bi, bj, (uint32_t)slot0, (uint32_t)n_slices, 0});
+                    plan.slots += n_slices;
+                }
+                uint64_t s = 0;
+                for (uint64_t k0 = 0; k0 < d.words; k0 += slice_words, ++s)
+                    jobs.push_back({i, bi, bj, n_slices > 1 ? (uint32_t)(slot0 + s) : kGramDirect, k0,
+                                    std::min<uint64_t>(d.words, k0 + slice_words)});
             }
     }
-    if (jobs.size() >= (1ull << 31)) return fail(CPM_ERR_INVALID, "pair block grid too large");
+    if (jobs.size() >= (1ull << 31) || plan.slots >= kGramDirect)
+        return fail(CPM_ERR_INVALID, "pair block grid too large");
     return CPM_OK;
 }
 
-int launch_gram(cpm_ctx *ctx, const GramJob *d_jobs, uint64_t n_jobs, const GramDb *d_dbs,
-                int32_t *d_out, uint64_t out_elems, uint64_t n_ops, bool clear) {
+int launch_gram(cpm_ctx *ctx, const GramDev &g, const GramDb *d_dbs, int32_t *d_out, uint64_t out_elems,
+                uint64_t n_ops, bool clear) {
     if (clear) CPM_HIP(hipMemsetAsync(d_out, 0, out_elems * sizeof(int32_t), ctx->stream));
-    if (n_jobs == 0) return CPM_OK;
-    Timed timer(ctx, 5);
-    hipLaunchKernelGGL(k_gram, dim3((unsigned)n_jobs), dim3(kThreads), 0, ctx->stream, d_jobs, d_dbs,
-                       d_out);
+    if (g.n_jobs == 0) return CPM_OK;
+    if (g.n_blocks && !g.partials) return fail(CPM_ERR_INVALID, "no partial slots for sliced pair blocks");
+    Timed timer(ctx, 5);  // both launches
+    hipLaunchKernelGGL(k_gram, dim3((unsigned)g.n_jobs), dim3(kThreads), 0, ctx->stream, g.jobs, d_dbs,
+                       d_out, g.partials);
     CPM_HIP(hipGetLastError());
+    if (g.n_blocks) {
+        const uint64_t items = g.n_blocks * (uint64_t)kBT * kBT;
+        hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)std::min<uint64_t>((items + 255) / 256, 8192)), dim3(256),
+                           0, ctx->stream, g.blocks, g.n_blocks, d_dbs, g.partials, d_out);
+        CPM_HIP(hipGetLastError());
+    }
     CPM_CHECK(timer.finish());
     ctx->stats[4] += 1;
     ctx->stats[6] += (double)n_ops;
@@ -319,19 +394,30 @@ extern "C" int cpm_pair_gram(cpm_ctx *ctx, const cpm_bits *bits, const uint32_t 
     dbs[0].cols = dc.as<uint32_t>();
     dbs[0].n_items = n;
     dbs[0].out_off = 0;
-    std::vector<GramJob> jobs;
-    uint64_t elems = 0;
-    CPM_CHECK(gram_plan(ctx, dbs, jobs, &elems));
+    GramPlan plan;
+    CPM_CHECK(gram_plan(ctx, dbs, plan));
+    const uint64_t elems = plan.out_elems;
+    DBuf dblocks, dparts;
     CPM_CHECK(dout.ensure(elems * 4));
-    CPM_CHECK(djobs.ensure(std::max<size_t>(jobs.size(), 1) * sizeof(GramJob)));
+    CPM_CHECK(djobs.ensure(std::max<size_t>(plan.jobs.size(), 1) * sizeof(GramJob)));
+    CPM_CHECK(dblocks.ensure(std::max<size_t>(plan.blocks.size(), 1) * sizeof(GramBlock)));
+    CPM_CHECK(dparts.ensure(std::max<uint64_t>(plan.slots, 1) * kBT * kBT * 4));
     CPM_CHECK(ddb.ensure(sizeof(GramDb)));
-    if (!jobs.empty())
-        CPM_HIP(hipMemcpyAsync(djobs.p, jobs.data(), jobs.size() * sizeof(GramJob),
+    if (!plan.jobs.empty())
+        CPM_HIP(hipMemcpyAsync(djobs.p, plan.jobs.data(), plan.jobs.size() * sizeof(GramJob),
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (!plan.blocks.empty())
+        CPM_HIP(hipMemcpyAsync(dblocks.p, plan.blocks.data(), plan.blocks.size() * sizeof(GramBlock),
                                hipMemcpyHostToDevice, ctx->stream));
     CPM_HIP(hipMemcpyAsync(ddb.p, dbs.data(), sizeof(GramDb), hipMemcpyHostToDevice, ctx->stream));
     const uint64_t ops = (uint64_t)bits->n_transactions * n * (uint64_t)(n + 1);
-    CPM_CHECK(launch_gram(ctx, djobs.as<GramJob>(), jobs.size(), ddb.as<GramDb>(), dout.as<int32_t>(), elems,
-                          ops, true));
+    GramDev g;
+    g.jobs = djobs.as<GramJob>();
+    g.n_jobs = plan.jobs.size();
+    g.blocks = dblocks.as<GramBlock>();
+    g.n_blocks = plan.blocks.size();
+    g.partials = dparts.as<int32_t>();
+    CPM_CHECK(launch_gram(ctx, g, ddb.as<GramDb>(), dout.as<int32_t>(), elems, ops, true));
     CPM_CHECK(copy_to_host(ctx, gram, dout.p, (size_t)n * n * 4));
     return CPM_OK;
 }

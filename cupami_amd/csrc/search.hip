@@ -1386,6 +1386,7 @@ struct GraphKey {
 struct SearchWs {
     LevelSet set[2];
     DBuf ctl, dbs, fcol, pairbits, tmask, gram, goff, gjobs, gdbs, l2, part, partl, scatter, ts, stamps;
+    DBuf gblocks, gparts;  // the pair matrix's sliced blocks and their partial slots
     DBuf pat;             // pattern forest: parent, item, support and database columns, pat_pitch() bytes apart
     DBuf filter;          // subset-test Bloom filter (filter_words() words)
     DBuf ranges;          // per work range of the level being counted: first task, most tiles
@@ -1968,7 +1969,8 @@ struct Search {
         bool gram_ok = use_gram;
         for (uint32_t i = 0; i < n_db && gram_ok; ++i)
             if (nf[i] > 16384 || dbs[i].n >= (1ull << 31)) gram_ok = false;
-        std::vector<GramJob> jobs;
+        GramPlan gplan;
+        GramDev gdev;
         uint64_t gram_elems = 0, gram_ops = 0;
         if (gram_ok) {
             std::vector<GramDb> gdb(n_db);
@@ -1986,15 +1988,24 @@ struct Search {
                 off += (uint64_t)gdb[i].n_items * gdb[i].n_items;
                 gram_ops += dbs[i].n * (uint64_t)gdb[i].n_items * (gdb[i].n_items + 1);
             }
-            CPM_CHECK(gram_plan(ctx, gdb, jobs, &gram_elems));
+            CPM_CHECK(gram_plan(ctx, gdb, gplan));
+            gram_elems = gplan.out_elems;
             CPM_CHECK(w.gram.ensure(std::max<uint64_t>(gram_elems, 1) * 4));
             CPM_CHECK(w.goff.ensure(n_db * 8));
-            CPM_CHECK(w.gjobs.ensure(std::max<size_t>(jobs.size(), 1) * sizeof(GramJob)));
+            CPM_CHECK(w.gjobs.ensure(std::max<size_t>(gplan.jobs.size(), 1) * sizeof(GramJob)));
+            CPM_CHECK(w.gblocks.ensure(std::max<size_t>(gplan.blocks.size(), 1) * sizeof(GramBlock)));
+            CPM_CHECK(w.gparts.ensure(std::max<uint64_t>(gplan.slots, 1) * 256 * 256 * 4));
             CPM_CHECK(w.gdbs.ensure(n_db * sizeof(GramDb)));
             qput(w.goff.p, goff.data(), n_db * 8);
-            qput(w.gjobs.p, jobs.data(), jobs.size() * sizeof(GramJob));
+            qput(w.gjobs.p, gplan.jobs.data(), gplan.jobs.size() * sizeof(GramJob));
+            qput(w.gblocks.p, gplan.blocks.data(), gplan.blocks.size() * sizeof(GramBlock));
             qput(w.gdbs.p, gdb.data(), n_db * sizeof(GramDb));
-            qzero(w.gram.p, gram_elems * 4);  // the Gram adds its K slices into it
+            gdev.jobs = w.gjobs.as<GramJob>();
+            gdev.n_jobs = gplan.jobs.size();
+            gdev.blocks = w.gblocks.as<GramBlock>();
+            gdev.n_blocks = gplan.blocks.size();
+            gdev.partials = w.gparts.as<int32_t>();
+            qzero(w.gram.p, gram_elems * 4);  // the Gram writes the upper triangles
         }
         tmask_w = 0;
         if (pair_words) {
@@ -2015,8 +2026,7 @@ struct Search {
         const unsigned g2 = (unsigned)std::min<uint64_t>(t2, 8192);
         if (gram_ok && periodic) {
             // level 2 = the frequent pairs (Gram supports), then their periods
-            CPM_CHECK(launch_gram(ctx, w.gjobs.as<GramJob>(), jobs.size(), w.gdbs.as<GramDb>(),
-                                  w.gram.as<int32_t>(), gram_elems, gram_ops, false));
+            CPM_CHECK(launch_gram(ctx, gdev, w.gdbs.as<GramDb>(), w.gram.as<int32_t>(), gram_elems, gram_ops, false));
             uint32_t *cnt = S1.cnt[0].as<uint32_t>();
             hipLaunchKernelGGL(k_level2_frequent_count, dim3(g2), dim3(256), 0, st, w.l2.as<L2Db>(), n_db,
                                (uint32_t)t2, w.dbs.as<DbDev>(), w.gram.as<int32_t>(), w.goff.as<uint64_t>(), cnt);
@@ -2032,8 +2042,8 @@ struct Search {
                                S1.ptrs(sz(1)), w.dbs.as<DbDev>());
             CPM_HIP(hipGetLastError());
             if (gram_ok) {
-                CPM_CHECK(launch_gram(ctx, w.gjobs.as<GramJob>(), jobs.size(), w.gdbs.as<GramDb>(),
-                                      w.gram.as<int32_t>(), gram_elems, gram_ops, false));
+                CPM_CHECK(launch_gram(ctx, gdev, w.gdbs.as<GramDb>(), w.gram.as<int32_t>(), gram_elems, gram_ops,
+                                      false));
                 hipLaunchKernelGGL(k_gram_to_level2, dim3(g2), dim3(256), 0, st, S1.ptrs(sz(1)),
                                    w.dbs.as<DbDev>(), w.gram.as<int32_t>(), w.goff.as<uint64_t>());
                 CPM_HIP(hipGetLastError());
