@@ -271,13 +271,14 @@ void k_support_count(SupportArgs a) {
     static_assert(16 * 128 * QPL < 65536, "packed row counts");
     const unsigned long long t_start = wall_clock64();
     unsigned long long bytes = 0, cands = 0;
-    for (uint64_t it = blockIdx.x, k = 0; it < v.total; it += gridDim.x, ++k) {
+    // work item `it`, the k-th this block takes
+    auto item = [&](uint64_t it, uint64_t k) {
         const Range r = range_of(v, it, k);
-        if (range_empty(v, it, r.tile)) continue;  // the same answer in every lane
+        if (range_empty(v, it, r.tile)) return;  // the same answer in every lane
         const uint32_t ne = r.r1 - r.r0;  // <= 64 (the group is at most 64)
         const uint32_t colv = lane < ne ? col_of(v.fcol, v.ext[r.r0 + lane]) : 0u;
         uint32_t t = range_task(v, it, r.r0);
-        if (KEEP && short_rows<QPL>(a, v, r, t, colv, emap, &ebytes, bytes, cands)) continue;
+        if (KEEP && short_rows<QPL>(a, v, r, t, colv, emap, &ebytes, bytes, cands)) return;
         // The AND of the first d - 1 prefix rows of the last task built here
         // (its parent's itemset), reused while the next tasks are its
         // siblings: children of one task are stored next to each other.
@@ -443,7 +444,9 @@ void k_support_count(SupportArgs a) {
             if (c) atomicAdd(sup + r.r0 + tid, c);
         }
         __syncthreads();
-    }
+    };
+    // items are dealt statically: block b takes b, b + grid, ...
+    for (uint64_t it = blockIdx.x, k = 0; it < v.total; it += gridDim.x, ++k) item(it, k);
     launch_epilogue(a, t_start, bytes, cands);
 }
 
@@ -524,8 +527,7 @@ __global__ __launch_bounds__(256) void k_support_periodic(SupportArgs a) {
     unsigned long long bytes = 0, cands = 0;
     for (uint64_t it = blockIdx.x, k = 0; it < lv.total; it += gridDim.x, ++k) {
         const Range r = range_of(lv, it, k);
-        if (range_empty(lv, it, r.tile)) continue;
-        uint32_t t = range_task(lv, it, r.r0);
+        uint32_t t = find_task(lv.ext_off, lv.n_tasks, r.r0);  // periodic searches are one database: no range table
         for (uint32_t e0 = r.r0; e0 < r.r1;) {
             while (lv.ext_off[t + 1] <= e0) ++t;  // empty tasks
             const uint32_t e1 = min(r.r1, lv.ext_off[t + 1]);
