@@ -377,11 +377,15 @@ struct GramDb {
     uint64_t out_off;
 };
 // One workgroup's share of the pair matrix: a 256 x 256 block of database
-// db over the K (word) range [k_begin, k_end).  A block cut into several K
-// slices stores each slice's sums in its own partial slot (256 x 256 int32)
-// and k_gram's reduce adds them; a block of one slice writes its outputs
-// (slot = kGramDirect).
+// db over the K (word) range [k_begin, k_end).  A block of one slice writes
+// its outputs (slot = kGramDirect).  A block cut into several K slices either
+// stores each slice's sums in its own partial slot (256 x 256 int32), which
+// k_gram's reduce adds, or adds them into the zeroed output with atomics
+// (slot = kGramAtomic): slots pay off for blocks of many outputs, atomics
+// for small matrices cut into many slices (a reduce thread would sum one
+// output over every slice).
 constexpr uint32_t kGramDirect = 0xffffffffu;
+constexpr uint32_t kGramAtomic = 0xfffffffeu;
 struct GramJob {
     uint32_t db, bi, bj, slot;
     uint64_t k_begin, k_end;

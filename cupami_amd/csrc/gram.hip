@@ -27,10 +27,11 @@
 // while the current one is multiplied, one barrier per tile.  On diagonal
 // blocks the six pieces with work are spread 1.5 per SIMD (two waves share
 // two of them, each taking every other K group, and their halves are added
-// through LDS at the end).  A block cut into several K slices stores each
-// slice into a partial slot and a second launch adds the slots (plain
-// stores and one pass instead of per-element atomics from every slice); a
-// block of one slice writes its outputs.  One launch serves any number of databases:
+// through LDS at the end).  A block of one slice writes its outputs; a block
+// of many outputs cut into K slices stores each slice into a partial slot
+// and a second launch adds the slots (plain stores and one pass instead of
+// per-element atomics from every slice); a small matrix cut into many
+// slices adds them with atomics.  One launch serves any number of databases:
 // the job table lists each workgroup's (database, block row, block column,
 // K range).
 #include "cpm_internal.h"
@@ -237,9 +238,11 @@ __global__ __launch_bounds__(kThreads) void k_gram(const GramJob *__restrict__ j
     if (diag && wave >= 6) return;
     // D tile layout of the 32 x 32 form: element i of lane l is row
     // 8 (i / 4) + 4 (l / 32) + i % 4, column l % 32.  A block of one K slice
-    // writes its upper-triangle outputs; a slice of a longer block stores
-    // the whole piece into its partial slot (coalesced rows), for the reduce.
-    int32_t *const slot = job.slot == kGramDirect ? nullptr : partials + (uint64_t)job.slot * kBT * kBT;
+    // writes its upper-triangle outputs and a slice of a block adding with
+    // atomics adds them; a slice of a block with partial slots stores the
+    // whole piece into its slot (coalesced rows), for the reduce.
+    int32_t *const slot = job.slot < kGramAtomic ? partials + (uint64_t)job.slot * kBT * kBT : nullptr;
+    const bool atomic = job.slot == kGramAtomic;
 #pragma unroll
     for (int x = 0; x < kTX; ++x)
 #pragma unroll
@@ -253,7 +256,10 @@ __global__ __launch_bounds__(kThreads) void k_gram(const GramJob *__restrict__ j
                     slot[rl * kBT + cl] = v;
                 } else {
                     const uint32_t row = bi * kBT + rl, col = bj * kBT + cl;
-                    if (row < n && col < n && row <= col) res[(uint64_t)row * ld + col] = v;
+                    if (row < n && col < n && row <= col) {
+                        if (!atomic) res[(uint64_t)row * ld + col] = v;
+                        else if (v) atomicAdd(res + (uint64_t)row * ld + col, v);
+                    }
                 }
             }
         }
@@ -333,17 +339,20 @@ int gram_plan(const cpm_ctx *ctx, const std::vector<GramDb> &dbs, GramPlan &plan
                 const uint64_t slice_words = (ktiles + slices - 1) / slices * kKW;
                 const uint64_t n_slices = (d.words + slice_words - 1) / slice_words;
                 const uint64_t slot0 = plan.slots;
-                if (n_slices > 1) {
+                // measured: slots win at 1000 items in ~28 slices, atomics at
+                // ~120 items in 256 slices
+                const bool slots = n_slices > 1 && d.n_items >= 2 * kBT && n_slices <= 64;
+                if (slots) {
                     plan.blocks.push_back({i, bi, bj, (uint32_t)slot0, (uint32_t)n_slices, 0});
                     plan.slots += n_slices;
                 }
                 uint64_t s = 0;
                 for (uint64_t k0 = 0; k0 < d.words; k0 += slice_words, ++s)
-                    jobs.push_back({i, bi, bj, n_slices > 1 ? (uint32_t)(slot0 + s) : kGramDirect, k0,
-                                    std::min<uint64_t>(d.words, k0 + slice_words)});
+                    jobs.push_back({i, bi, bj, slots ? (uint32_t)(slot0 + s) : n_slices > 1 ? kGramAtomic : kGramDirect,
+                                    k0, std::min<uint64_t>(d.words, k0 + slice_words)});
             }
     }
-    if (jobs.size() >= (1ull << 31) || plan.slots >= kGramDirect)
+    if (jobs.size() >= (1ull << 31) || plan.slots >= kGramAtomic)
         return fail(CPM_ERR_INVALID, "pair block grid too large");
     return CPM_OK;
 }
