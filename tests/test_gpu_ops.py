@@ -158,7 +158,11 @@ def test_intersect_extensions(ctx):
         assert np.array_equal(idx, keep) and np.array_equal(rows, expect_rows[keep])
 
 
-@pytest.mark.parametrize("n,items", [(64, 3), (1000, 17), (70_000, 65), (300_001, 130), (5000, 257)])
+# (30_000, 520) and (200_000, 600): blocks of 512+ items cut into K slices,
+# summed through partial slots and the reduce launch; the others add their
+# slices with atomics or write single-slice blocks directly
+@pytest.mark.parametrize("n,items", [(64, 3), (1000, 17), (70_000, 65), (300_001, 130), (5000, 257),
+                                     (30_000, 520), (200_000, 600)])
 def test_pair_gram_equals_popcount(ctx, n, items):
     rows, cols = random_store(n + items, n, items, 0.35)
     bits = ctx.build_bits(rows, cols, n, items)
@@ -225,6 +229,23 @@ def test_pfeclat_mine_random(ctx, seed):
     for ms, per in [(1, 400), (3, 60), (5, 30), (20, 200)]:
         expect = oracle.pfeclat(host, ts, n, end, ms, per)
         assert ctx.pfeclat_mine(bits, ts, end, ms, per).to_dict() == expect, (ms, per)
+
+
+def test_pfeclat_large_result_streamed(ctx):
+    """A periodic search of 2^19 - 1 patterns: the result streams to the host
+    chunk by chunk (and the first attempt overflows its pattern capacity);
+    the forest equals the oracle's."""
+    rng = np.random.default_rng(19)
+    n, items = 3000, 19
+    dense = rng.random((n, items)) < 0.95
+    rows, cols = np.nonzero(dense)
+    ts = np.arange(1, n + 1, dtype=np.int64)
+    bits = ctx.build_bits(rows.astype(np.uint64), cols.astype(np.uint32), n, items)
+    host = oracle.bits_build(rows.astype(np.uint64), cols.astype(np.uint32), n, items)
+    expect = oracle.pfeclat(host, ts, n, n, 1, n)
+    assert len(expect) == 2 ** items - 1
+    got = ctx.pfeclat_mine(bits, ts, n, 1, n).to_dict()
+    assert got == expect
 
 
 def test_pfeclat_mine_device_timestamps(ctx):
