@@ -1458,6 +1458,8 @@ struct Search {
     uint64_t copied = 0;             // patterns queued for copying into res
 
     Search(cpm_ctx *c, SearchWs &ws) : ctx(c), w(ws) {}
+    Search(const Search &) = delete;
+    Search &operator=(const Search &) = delete;
     ~Search() { drop_stream(); }
 
     // the streamed result of an attempt that will not be used (copies waited for)
