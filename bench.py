@@ -675,6 +675,15 @@ def run_roofs(args):
             best = max(best, ops / (ms / 1e3) / 1e12)
         out[name] = {"tops": round(best, 1), "published_dense_tops": 10100.0}
         log(f"bench: roof {name}: {best:.1f} TOP/s")
+    # a phase boundary inside one launch (grid-wide barrier with an
+    # agent-scope release) against a launch boundary in a graph
+    for blocks in (256, 1024, 2048):
+        ctx.roof_phases(blocks, 8, 4096)
+        runs = [ctx.roof_phases(blocks, 64, 4096) for _ in range(3)]
+        ub, ul = min(r[0] for r in runs), min(r[1] for r in runs)
+        out[f"phase_{blocks}_blocks"] = {"us_per_barrier": round(ub, 2), "us_per_launch": round(ul, 2),
+                                         "dirty_bytes_per_block": 4096}
+        log(f"bench: roof phases, {blocks} workgroups: barrier {ub:.2f} us, graph launch {ul:.2f} us")
     props = torch.cuda.get_device_properties(0)
     return {"metric": "measured roofs (AND+popcount GB/s by residency; FP4 MFMA TOP/s)",
             "value": out["mall_16mib_per_xcd"]["gb_s"], "unit": "GB/s", "n_gpus": 1, "steps": 3, "warmup": 1,

@@ -105,6 +105,8 @@ SIGNATURES = {
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "cpm_roof_mfma": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint32, _vp, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_double)]),
+    "cpm_roof_phases": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "cpm_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "cpm_stats_reset": (ctypes.c_int, [_vp]),
     "cpm_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -206,6 +208,13 @@ class Context:
         check(load().cpm_roof_stream(self.handle, _vp(int(d_buf)), int(nbytes), int(reps), int(bool(xcd_sliced)),
                                      _vp(int(d_sink)), ctypes.byref(ms), ctypes.byref(nb)))
         return ms.value, nb.value
+
+    def roof_phases(self, blocks, rounds, dirty_bytes):
+        """(us per grid-wide barrier in one launch, us per launch in a graph)."""
+        ub, ul = ctypes.c_double(), ctypes.c_double()
+        check(load().cpm_roof_phases(self.handle, int(blocks), int(rounds), int(dirty_bytes),
+                                     ctypes.byref(ub), ctypes.byref(ul)))
+        return ub.value, ul.value
 
     def roof_mfma(self, iters, d_sink, shape=16):
         """FP4 MFMA loop microbenchmark; returns (ms, ops)."""

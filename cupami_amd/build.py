@@ -10,7 +10,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libcupami_hip.so"
-SOURCES = ["core.hip", "support.hip", "gram.hip", "search.hip", "sort.hip", "fptree.hip", "fpgrowth.hip", "ingest.hip", "output.hip", "roofs.hip", "textin.hip"]
+SOURCES = ["core.hip", "support.hip", "gram.hip", "search.hip", "sort.hip", "fptree.hip", "fpgrowth.hip", "ingest.hip", "output.hip", "roofs.hip", "phases.hip", "textin.hip"]
 
 # -packed-fp32-ops: keep the compiler from forming v_pk_*_f32 instructions.
 FLAGS = [

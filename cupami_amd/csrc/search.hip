@@ -593,16 +593,6 @@ __global__ __launch_bounds__(256) void k_plan_offsets(PlanArgs p) {
     }
 }
 
-// largest t < n with off[t] <= i
-__device__ __forceinline__ uint32_t upper_task(const uint32_t *__restrict__ off, uint32_t n, uint64_t i) {
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= i) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
 // subset test (cuApriori.py:487-490) ----------------------------------------------
 //
 // The itemsets of the level being planned go into a blocked Bloom filter:

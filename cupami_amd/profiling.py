@@ -186,12 +186,19 @@ def roofs_summary(result, command):
     lines = ["# Measured roofs of this device", "",
              "AND + popcount over the two halves of a buffer, best of three timed launches; XCD-sliced buffers "
              "give each XCD its own eighth (the support kernel's tile == XCD mapping), so 2-4 MiB per XCD stay "
-             "in L2 and 16 MiB per XCD in the Infinity Cache.  FP4 MFMA loops of independent accumulators.",
+             "in L2 and 16 MiB per XCD in the Infinity Cache.  FP4 MFMA loops of independent accumulators.  "
+             "Phase boundaries: 64 rounds of 4 KB of stores per workgroup and a grid-wide barrier in one "
+             "cooperative launch, against the same rounds as 64 launches of one hipGraph (best of three).",
              "", "```", command, "```", ""]
+    sources = ["cupami_amd/csrc/roofs.hip", "cupami_amd/csrc/phases.hip"]
     f = {"device": result.get("device", ""), "cu_count": result.get("cu_count", ""),
-         "sources": "cupami_amd/csrc/roofs.hip", "sources_sha256": sources_digest(["cupami_amd/csrc/roofs.hip"])}
+         "sources": " ".join(sources), "sources_sha256": sources_digest(sources)}
     for name, r in roofs.items():
-        f[f"{name}_{'tops' if 'tops' in r else 'gb_s'}"] = r.get("tops", r.get("gb_s"))
+        if "us_per_barrier" in r:
+            f[f"{name}_us_per_barrier"] = r["us_per_barrier"]
+            f[f"{name}_us_per_launch"] = r["us_per_launch"]
+        else:
+            f[f"{name}_{'tops' if 'tops' in r else 'gb_s'}"] = r.get("tops", r.get("gb_s"))
     f["l2_gb_s"] = min(roofs["l2_2mib_per_xcd"]["gb_s"], roofs["l2_4mib_per_xcd"]["gb_s"])
     f["mall_gb_s"] = roofs["mall_16mib_per_xcd"]["gb_s"]
     f["hbm_gb_s"] = max(roofs["hbm_4gib"]["gb_s"], roofs["hbm_8gib_flat"]["gb_s"])

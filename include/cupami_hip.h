@@ -330,6 +330,15 @@ int cpm_set_timing(cpm_ctx *ctx, int on);
 int cpm_roof_stream(cpm_ctx *ctx, const void *d_buf, uint64_t bytes, uint32_t reps, int xcd_sliced,
                     unsigned long long *d_sink, double *ms, double *bytes_read);
 int cpm_roof_mfma(cpm_ctx *ctx, int shape, uint32_t iters, float *d_sink, double *ms, double *ops);
+/* cpm_roof_phases: the cost of a phase boundary inside one launch against a
+ * launch boundary.  `blocks` workgroups of 256 lanes (all resident: the
+ * launch is cooperative; more than fit is an error) run `rounds` (1..4096)
+ * rounds of storing `dirty_bytes` each (a multiple of 16, at most 1 MiB) and
+ * a grid-wide barrier (agent-scope release, arrival on a counter, a bounded
+ * wait); *us_barrier is a round's time.  The same rounds as one launch each
+ * in a hipGraph give *us_launch. */
+int cpm_roof_phases(cpm_ctx *ctx, uint32_t blocks, uint32_t rounds, uint32_t dirty_bytes, double *us_barrier,
+                    double *us_launch);
 
 #ifdef __cplusplus
 }

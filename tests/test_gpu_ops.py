@@ -343,3 +343,16 @@ def test_large_results_views_outlive_their_patterns(ctx):
     parts = last.split(2)
     assert parts[0] == expect_first
     assert parts[1] == oracle.eclat(host, 800)
+
+
+def test_roof_phases(ctx):
+    """The phase-boundary microbenchmark: every wait ends (no time-out
+    error), both costs are positive, and a grid that cannot be resident at
+    once is refused before launch."""
+    import math
+    ub, ul = ctx.roof_phases(64, 4, 4096)
+    assert 0 < ub < 1e5 and 0 < ul < 1e5 and math.isfinite(ub) and math.isfinite(ul)
+    with pytest.raises(ValueError):
+        ctx.roof_phases(1 << 20, 1, 4096)
+    with pytest.raises(ValueError):
+        ctx.roof_phases(64, 1, 100)   # not a multiple of 16 bytes
