@@ -496,21 +496,37 @@ __device__ __forceinline__ PeriodPart pp_wave_reduce(PeriodPart m, uint32_t lane
     return m;
 }
 
-// walk the set bits of one word of an occurrence row, in row order
+// Walk the set bits of one word of an occurrence row, in row order, four at
+// a time: the four timestamp loads are issued together (unconditionally, at
+// row 0 for the missing ones) and folded in order after one wait, instead of
+// one dependent round trip per occurrence.  (Eight at a time took the kernel
+// to 106 VGPRs and was slower.)
 __device__ __forceinline__ void pp_scan_word(PeriodPart &m, uint64_t v, uint64_t row0,
                                              gptr<int64_t> ts, uint64_t n) {
+    constexpr int kBatch = 4;
     while (v) {
-        const uint64_t r = row0 + (uint64_t)__builtin_ctzll(v);
-        v &= v - 1;
-        if (r < n) {
-            const int64_t t = ts[r];
+        uint64_t r[kBatch];
+        bool ok[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const bool any = v != 0;
+            r[j] = row0 + (any ? (uint64_t)__builtin_ctzll(v) : 0);
+            v &= v - 1;  // stays 0 once empty
+            ok[j] = any && r[j] < n;
+        }
+        int64_t t[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) t[j] = ts[ok[j] ? r[j] : 0];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            if (!ok[j]) continue;
             if (m.count == 0) {
-                m.first = t;
+                m.first = t[j];
             } else {
-                const int64_t g = t - m.last;
+                const int64_t g = t[j] - m.last;
                 if (g > m.gap) m.gap = g;
             }
-            m.last = t;
+            m.last = t[j];
             ++m.count;
         }
     }
@@ -683,7 +699,7 @@ __global__ __launch_bounds__(256) void k_intersect(const uint64_t *__restrict__ 
 // kernel holds 8 blocks per CU and runs best at 7 (a full grid waits for
 // slots the previous launch's last waves still hold); the 2048-word kernel
 // holds 4 and runs best oversubscribed at 8 (late blocks take the leftover
-// work); the periodic kernel runs best at 6.
+// work); the periodic kernel (93 VGPRs: five waves per SIMD) runs best at 5.
 uint32_t persistent_grid(const cpm_ctx *ctx, uint32_t per_cu, uint32_t grid) {
     const uint32_t g = (uint32_t)ctx->cu_count * per_cu / 8 * 8;
     return std::max<uint32_t>(8, std::min(grid, g));
@@ -712,7 +728,7 @@ int launch_support_count(cpm_ctx *ctx, const SupportArgs &a, uint32_t tile_words
 
 int launch_support_periodic(cpm_ctx *ctx, const SupportArgs &a, uint32_t grid) {
     if (a.te == 0) return fail(CPM_ERR_INVALID, "zero tiles per work range");
-    hipLaunchKernelGGL(k_support_periodic, dim3(persistent_grid(ctx, 6, grid)), dim3(256), 0,
+    hipLaunchKernelGGL(k_support_periodic, dim3(persistent_grid(ctx, 5, grid)), dim3(256), 0,
                        ctx->stream, a);
     CPM_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_periodic_finalize, dim3(grid), dim3(256), 0, ctx->stream, a);
