@@ -33,6 +33,18 @@ __device__ __forceinline__ uint32_t pop4(uint4 a) {
     return __popc(a.x) + __popc(a.y) + __popc(a.z) + __popc(a.w);
 }
 
+// 16 bytes at byte `off` of a row of `bytes` bytes, as a raw buffer load:
+// zero when off is at or past the end.  Word 3 of the resource is the gfx9
+// family's raw-access setting (32-bit data format; the format fields are
+// unused by untyped loads).
+constexpr int kRawBufferWord3 = 0x00020000;
+__device__ __forceinline__ uint4 row_load(gptr<uint4> row, uint32_t bytes, uint32_t off) {
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(const void *)row, 0, (int)bytes, kRawBufferWord3);
+    const auto x = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+    return make_uint4(x[0], x[1], x[2], x[3]);
+}
+
 __device__ __forceinline__ uint4 and4(uint4 a, uint4 b) {
     return make_uint4(a.x & b.x, a.y & b.y, a.z & b.z, a.w & b.w);
 }
@@ -299,18 +311,25 @@ void k_support_count(SupportArgs a) {
             const gptr<uint4> store = to_global(reinterpret_cast<const uint4 *>(db->store));
             const uint64_t stride_q = db->stride / 2;
             const uint64_t q0 = (uint64_t)r.tile * kQuadsPerTile + tid;
-            // Lane offsets inside a row, clamped into it: every load stays in
-            // bounds and is issued without a branch (a guarded load makes the
-            // compiler wait for it on the spot).  Lanes past the row's end are
-            // zero in P, which zeroes their popcounts.
+            // Row loads are buffer loads: the row is a buffer resource (its
+            // scalar base, num_records = its bytes) and a lane passes a 32-bit
+            // byte offset, so a load costs no 64-bit address arithmetic, and
+            // a lane past the row's end reads zero (an out-of-range buffer
+            // load returns 0) with no branch or clamp of its own; zero prefix
+            // words zero those lanes' popcounts.  (A row is under 2^32 bytes.)
+            // off/valid: the clamped quad index, for the retained-row stores.
+            const uint32_t row_bytes = (uint32_t)(stride_q * sizeof(uint4));
             uint64_t off[QPL];
+            uint32_t boff[QPL];
             bool valid[QPL];
 #pragma unroll
             for (int q = 0; q < QPL; ++q) {
                 const uint64_t o = q0 + (uint64_t)q * 256;
                 valid[q] = o < stride_q;
                 off[q] = valid[q] ? o : stride_q - 1;
+                boff[q] = (uint32_t)((valid[q] ? o : stride_q) * sizeof(uint4));
             }
+            auto at = [&](gptr<uint4> row, int q) { return row_load(row, row_bytes, boff[q]); };
             auto row_of = [&](uint32_t f) { return store + (uint64_t)col_of(v.fcol, f) * stride_q; };
             // entry e's row (e in this work item's range)
             auto entry_row = [&](uint32_t e) {
@@ -327,7 +346,7 @@ void k_support_count(SupportArgs a) {
             if (from_kept) {
                 const gptr<uint4> row = to_global(reinterpret_cast<const uint4 *>(v.rin)) + (uint64_t)src * stride_q;
 #pragma unroll
-                for (int q = 0; q < QPL; ++q) P[q] = row[off[q]];
+                for (int q = 0; q < QPL; ++q) P[q] = at(row, q);
 #pragma unroll
                 for (int q = 0; q < QPL; ++q)
                     if (!valid[q]) P[q] = make_uint4(0, 0, 0, 0);
@@ -346,7 +365,7 @@ void k_support_count(SupportArgs a) {
             if (v.d) {
                 const gptr<uint4> row = row_of(pre[v.d - 1]);
 #pragma unroll
-                for (int q = 0; q < QPL; ++q) last[q] = row[off[q]];
+                for (int q = 0; q < QPL; ++q) last[q] = at(row, q);
                 rows_read += 1;
             }
             if (!same) {
@@ -360,8 +379,8 @@ void k_support_count(SupportArgs a) {
                     uint4 x[QPL], y[QPL];
 #pragma unroll
                     for (int q = 0; q < QPL; ++q) {
-                        x[q] = r0[off[q]];
-                        y[q] = r1[off[q]];
+                        x[q] = at(r0, q);
+                        y[q] = at(r1, q);
                     }
 #pragma unroll
                     for (int q = 0; q < QPL; ++q) PP[q] = and4(PP[q], and4(x[q], y[q]));
@@ -369,7 +388,7 @@ void k_support_count(SupportArgs a) {
                 if (j < np) {
                     const gptr<uint4> r0 = row_of(pre[j]);
 #pragma unroll
-                    for (int q = 0; q < QPL; ++q) PP[q] = and4(PP[q], r0[off[q]]);
+                    for (int q = 0; q < QPL; ++q) PP[q] = and4(PP[q], at(r0, q));
                 }
                 rows_read += np;
             }
@@ -392,7 +411,7 @@ void k_support_count(SupportArgs a) {
                 for (int u = 0; u < NE; ++u) {
                     const gptr<uint4> row = entry_row(e + u);
 #pragma unroll
-                    for (int q = 0; q < QPL; ++q) X[u][q] = row[off[q]];
+                    for (int q = 0; q < QPL; ++q) X[u][q] = at(row, q);
                 }
                 uint32_t c[NE];
 #pragma unroll
