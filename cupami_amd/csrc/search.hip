@@ -292,14 +292,47 @@ __device__ __forceinline__ unsigned long long sum_before(const unsigned long lon
     return block_sum64(v, red);
 }
 
-// first i <= n with off[i] >= x (off[0 .. n] non-decreasing)
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *off, uint32_t n, uint64_t x) {
-    uint32_t lo = 0, hi = n + 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] < x) lo = mid + 1; else hi = mid;
+// For two keys x: the first i <= n with off[i] >= x (off[0 .. n]
+// non-decreasing; n + 1 when there is none), found by a whole 256-lane
+// block in 256-ary rounds.  The answer lies in [lo, hi]; each lane tests one
+// pivot lo + span t / 256 of [lo, hi) per key, and the pivots below the key
+// are counted (ballots summed in LDS).  That leaves 1/256 of the range, and
+// a range of at most 256 has every position tested: two rounds for 65,536
+// children instead of 16 dependent loads of a binary search.  Every lane
+// gets both answers.  cnt: 16 words of LDS.
+__device__ void block_lower_bound2(const uint32_t *off, uint32_t n, uint64_t x0, uint64_t x1, uint32_t *cnt,
+                                   uint32_t &r0, uint32_t &r1) {
+    uint32_t lo[2] = {0, 0}, hi[2] = {n + 1, n + 1};
+    const uint64_t key[2] = {x0, x1};
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t par = 0; hi[0] > lo[0] || hi[1] > lo[1]; par ^= 1) {  // uniform
+        uint32_t *const c = cnt + par * 8;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint64_t span = hi[k] - lo[k];
+            const bool below = span > 0 && off[lo[k] + (uint32_t)((span * threadIdx.x) >> 8)] < key[k];
+            const unsigned long long b = __ballot(below);
+            if (lane == 0) c[k * 4 + wave] = (uint32_t)__popcll(b);
+        }
+        __syncthreads();  // c is written again two rounds on, after the next barrier
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint64_t span = hi[k] - lo[k];
+            if (span == 0) continue;
+            // pivots t < m are below the key: the answer is past pivot m - 1
+            // and at most pivot m
+            const uint32_t m = c[k * 4] + c[k * 4 + 1] + c[k * 4 + 2] + c[k * 4 + 3];
+            if (m == 0) {
+                hi[k] = lo[k];
+            } else {
+                const uint32_t a = lo[k] + (uint32_t)((span * (m - 1)) >> 8) + 1;
+                if (m < 256) hi[k] = lo[k] + (uint32_t)((span * m) >> 8);
+                lo[k] = a;
+            }
+        }
     }
-    return lo;
+    r0 = lo[0];
+    r1 = lo[1];
 }
 
 // Fold the per-block stamps a support launch left (start, end, bytes,
@@ -1035,6 +1068,7 @@ __global__ __launch_bounds__(256) void k_prune_flags(PlanArgs p) {
 // prune/compact: same grid as prune/flags ---------------------------------------------
 __global__ __launch_bounds__(256) void k_prune_compact(PlanArgs p) {
     __shared__ uint32_t lds[8];
+    __shared__ uint32_t cnt[16];
     __shared__ unsigned long long red[4];
     if (!prune_active(p)) return;
     const uint32_t n_child = (uint32_t)p.N.sz->n_tasks;
@@ -1042,6 +1076,11 @@ __global__ __launch_bounds__(256) void k_prune_compact(PlanArgs p) {
     const bool last = blockIdx.x == gridDim.x - 1;
     uint64_t lo, hi;
     chunk_of(n_cand, blockIdx.x, gridDim.x, &lo, &hi);
+    // children whose staged entries start in [lo, hi); the last block also
+    // takes those starting at n_cand, and the end sentinel
+    uint32_t c_lo, c_hi;
+    block_lower_bound2(p.off2, n_child, lo, hi, cnt, c_lo, c_hi);
+    if (last) c_hi = n_child + 1;
     unsigned long long base = sum_before(p.partl, blockIdx.x, red);
     for (uint64_t i0 = lo; i0 < hi; i0 += 256) {
         const uint64_t w = i0 + threadIdx.x;
@@ -1059,10 +1098,6 @@ __global__ __launch_bounds__(256) void k_prune_compact(PlanArgs p) {
         base += tot;
     }
     __syncthreads();  // pos of the chunk is visible to the whole block
-    // children whose staged entries start in [lo, hi); the last block also
-    // takes those starting at n_cand, and the end sentinel
-    const uint32_t c_lo = lower_bound_u32(p.off2, n_child, lo);
-    const uint32_t c_hi = last ? n_child + 1 : lower_bound_u32(p.off2, n_child, hi);
     for (uint32_t ct = c_lo + threadIdx.x; ct < c_hi; ct += blockDim.x) {
         const uint32_t x = p.off2[ct];
         p.N.ext_off[ct] = x < hi ? p.pos[x] : (uint32_t)base;
