@@ -284,13 +284,6 @@ __device__ __forceinline__ uint32_t tiny_child_count(const uint32_t *wf, uint32_
     return c;
 }
 
-// sum of part[0 .. b), in every lane of the block
-__device__ __forceinline__ unsigned long long sum_before(const unsigned long long *part, uint32_t b,
-                                                         unsigned long long *red) {
-    unsigned long long v = 0;
-    for (uint32_t j = threadIdx.x; j < b; j += blockDim.x) v += part[j];
-    return block_sum64(v, red);
-}
 
 // For two keys x: the first i <= n with off[i] >= x (off[0 .. n]
 // non-decreasing; n + 1 when there is none), found by a whole 256-lane
@@ -590,12 +583,16 @@ __device__ void alloc_next(const PlanArgs &p) {
 }
 
 // plan/offsets: same grid as plan/count -----------------------------------------------
+// The three counts (patterns, children, child entries) are scanned together:
+// their chunk-sum loads are in flight at once and their block scans share
+// barriers, so the launch is two rounds of loads rather than six.
 __global__ __launch_bounds__(256) void k_plan_offsets(PlanArgs p) {
-    __shared__ uint32_t lds[8];
-    __shared__ unsigned long long red[4];
+    __shared__ uint32_t lds[3][4];
+    __shared__ unsigned long long red[3][4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool last = blockIdx.x == gridDim.x - 1;
     if (p.ctl->overflow) {
-        if (last && threadIdx.x == 0) p.N.sz->n_tasks = p.N.sz->n_ext = p.N.sz->n_work = p.N.sz->n_cand = 0;
+        if (last && tid == 0) p.N.sz->n_tasks = p.N.sz->n_ext = p.N.sz->n_work = p.N.sz->n_cand = 0;
         return;
     }
     const uint32_t n = (uint32_t)p.L.sz->n_tasks;
@@ -603,25 +600,45 @@ __global__ __launch_bounds__(256) void k_plan_offsets(PlanArgs p) {
     chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
     const uint32_t *cnt[3] = {p.cnt_pat, p.cnt_child, p.cnt_cext};
     uint32_t *off[3] = {p.off_pat, p.off_child, p.off_cext};
-    unsigned long long tot[3];
+    // the sums of the chunks before this block's
+    unsigned long long base[3] = {0, 0, 0};
+    for (uint32_t j = tid; j < blockIdx.x; j += blockDim.x) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) base[a] += p.part[(uint64_t)a * gridDim.x + j];
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        unsigned long long base = sum_before(p.part + (uint64_t)a * gridDim.x, blockIdx.x, red);
-        for (uint64_t i0 = lo; i0 < hi; i0 += 256) {
-            const uint64_t i = i0 + threadIdx.x;
-            const uint32_t v = i < hi ? cnt[a][i] : 0u;
-            uint32_t t;
-            const uint32_t x = block_scan(v, lds, &t);
-            if (i < hi) off[a][i] = (uint32_t)(base + x);
-            base += t;
-        }
-        tot[a] = base;  // in the last block: the total
+        base[a] = dpp_wave_sum64(base[a]);
+        if (lane == 0) red[a][wave] = base[a];
     }
-    if (last && threadIdx.x == 0) {
-        for (int a = 0; a < 3; ++a) off[a][n] = (uint32_t)tot[a];
-        p.L.sz->n_pat = tot[0];
-        p.L.sz->n_child = tot[1];
-        p.L.sz->n_cext = tot[2];
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 3; ++a) base[a] = red[a][0] + red[a][1] + red[a][2] + red[a][3];
+    for (uint64_t i0 = lo; i0 < hi; i0 += 256) {
+        const uint64_t i = i0 + tid;
+        uint32_t v[3], x[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = i < hi ? cnt[a][i] : 0u;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            x[a] = dpp_wave_scan_incl(v[a]);
+            if (lane == 63) lds[a][wave] = x[a];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            uint32_t before = 0;
+            for (uint32_t w = 0; w < wave; ++w) before += lds[a][w];
+            if (i < hi) off[a][i] = (uint32_t)(base[a] + before + x[a] - v[a]);
+            base[a] += lds[a][0] + lds[a][1] + lds[a][2] + lds[a][3];
+        }
+        __syncthreads();  // lds is rewritten by the next round
+    }
+    if (last && tid == 0) {  // base: the totals
+        for (int a = 0; a < 3; ++a) off[a][n] = (uint32_t)base[a];
+        p.L.sz->n_pat = base[0];
+        p.L.sz->n_child = base[1];
+        p.L.sz->n_cext = base[2];
         alloc_next(p);
     }
 }
@@ -1078,10 +1095,14 @@ __global__ __launch_bounds__(256) void k_prune_compact(PlanArgs p) {
     chunk_of(n_cand, blockIdx.x, gridDim.x, &lo, &hi);
     // children whose staged entries start in [lo, hi); the last block also
     // takes those starting at n_cand, and the end sentinel
+    // (the loads of the survivor counts before this chunk are issued first,
+    // so they are in flight during the search's rounds)
+    unsigned long long before = 0;
+    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += blockDim.x) before += p.partl[j];
     uint32_t c_lo, c_hi;
     block_lower_bound2(p.off2, n_child, lo, hi, cnt, c_lo, c_hi);
     if (last) c_hi = n_child + 1;
-    unsigned long long base = sum_before(p.partl, blockIdx.x, red);
+    unsigned long long base = block_sum64(before, red);
     for (uint64_t i0 = lo; i0 < hi; i0 += 256) {
         const uint64_t w = i0 + threadIdx.x;
         const uint32_t f = w < hi ? p.live[w] : 0u;
