@@ -627,12 +627,44 @@ __global__ __launch_bounds__(256) void k_periodic_finalize(SupportArgs a) {
     const uint32_t n_tasks = (uint32_t)a.L.sz->n_tasks;
     if (n_tasks == 0) return;
     const uint64_t n_ext = a.L.sz->n_ext;
+    // A periodic search holds one database (tasks are in database order, so
+    // the first and last tasks' agree): then no entry searches for its task.
+    const bool one_db = a.L.db[0] == a.L.db[n_tasks - 1];
     for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_ext;
          e += (uint64_t)gridDim.x * blockDim.x) {
-        const DbDev *db = a.dbs + a.L.db[find_task(to_const(a.L.ext_off), n_tasks, e)];
+        const DbDev *db = a.dbs + a.L.db[one_db ? 0u : find_task(to_const(a.L.ext_off), n_tasks, e)];
+        const uint32_t tiles = db->tiles;
+        const PeriodPart *pe = a.parts + e * a.parts_tiles;
         PeriodPart m;
         pp_empty(m);
-        for (uint32_t c = 0; c < db->tiles; ++c) m = pp_combine(m, a.parts[e * a.parts_tiles + c]);
+        // the tiles' partials four at a time (their loads in flight
+        // together), folded in tile order
+        constexpr uint32_t kB = 4;
+        for (uint32_t c0 = 0; c0 < tiles; c0 += kB) {
+            int64_t f[kB], l[kB], g[kB];
+            uint32_t n[kB];
+#pragma unroll
+            for (uint32_t i = 0; i < kB; ++i) {
+                const PeriodPart *x = pe + (c0 + i < tiles ? c0 + i : c0);
+                f[i] = x->first;
+                l[i] = x->last;
+                g[i] = x->gap;
+                n[i] = x->count;
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < kB; ++i) {
+                if (c0 + i >= tiles || n[i] == 0) continue;
+                if (m.count == 0) {
+                    m.first = f[i];
+                    m.gap = g[i];
+                } else {
+                    const int64_t x = f[i] - m.last;
+                    m.gap = max(max(m.gap, g[i]), x);
+                }
+                m.last = l[i];
+                m.count += n[i];
+            }
+        }
         int64_t p;
         if (m.count == 0) {
             p = db->end;
