@@ -532,38 +532,72 @@ __device__ __forceinline__ PeriodPart pp_wave_reduce(PeriodPart m, uint32_t lane
     return m;
 }
 
-// Walk the set bits of one word of an occurrence row, in row order, four at
-// a time: the four timestamp loads are issued together (unconditionally, at
-// row 0 for the missing ones) and folded in order after one wait, instead of
-// one dependent round trip per occurrence.  (Eight at a time took the kernel
-// to 106 VGPRs and was slower.)
-__device__ __forceinline__ void pp_scan_word(PeriodPart &m, uint64_t v, uint64_t row0,
+// Fold a lane's 8 words (rows row0 .. row0 + 511, in order) into m, four
+// words at a time.  The timestamps of each word's first and last occurrence
+// are gathered first, eight loads in flight together, which fixes first,
+// last, count and every gap between words (and inside words of two
+// occurrences).  A word of three or more occurrences has inner gaps, each at
+// most its span (last - first: timestamps increase); its occurrences are
+// walked only when that span exceeds the largest gap already known, which
+// then bounds them.
+__device__ __forceinline__ void pp_scan_lane(PeriodPart &m, const uint64_t (&X)[8], uint64_t row0,
                                              gptr<int64_t> ts, uint64_t n) {
-    constexpr int kBatch = 4;
-    while (v) {
-        uint64_t r[kBatch];
-        bool ok[kBatch];
+#pragma unroll 1
+    for (int h = 0; h < 8; h += 4) {
+        uint64_t x[4];
+        uint32_t c[4];
+        int64_t tf[4], tl[4];
 #pragma unroll
-        for (int j = 0; j < kBatch; ++j) {
-            const bool any = v != 0;
-            r[j] = row0 + (any ? (uint64_t)__builtin_ctzll(v) : 0);
-            v &= v - 1;  // stays 0 once empty
-            ok[j] = any && r[j] < n;
+        for (int v = 0; v < 4; ++v) {
+            const uint64_t r = row0 + 64ull * (h + v);  // rows at and past n hold no occurrence
+            const uint64_t xv = h == 0 ? X[v] : X[4 + v];
+            x[v] = r >= n ? 0ull : n - r >= 64 ? xv : xv & ((1ull << (n - r)) - 1);
+            c[v] = (uint32_t)__popcll(x[v]);
+            const uint64_t rf = x[v] ? r + (uint64_t)__builtin_ctzll(x[v]) : 0;
+            const uint64_t rl = x[v] ? r + 63 - (uint64_t)__builtin_clzll(x[v]) : 0;
+            tf[v] = ts[rf];
+            tl[v] = ts[rl];
         }
-        int64_t t[kBatch];
 #pragma unroll
-        for (int j = 0; j < kBatch; ++j) t[j] = ts[ok[j] ? r[j] : 0];
-#pragma unroll
-        for (int j = 0; j < kBatch; ++j) {
-            if (!ok[j]) continue;
+        for (int v = 0; v < 4; ++v) {
+            if (!c[v]) continue;
             if (m.count == 0) {
-                m.first = t[j];
+                m.first = tf[v];
             } else {
-                const int64_t g = t[j] - m.last;
+                const int64_t g = tf[v] - m.last;
                 if (g > m.gap) m.gap = g;
             }
-            m.last = t[j];
-            ++m.count;
+            if (c[v] == 2 && tl[v] - tf[v] > m.gap) m.gap = tl[v] - tf[v];
+            m.last = tl[v];
+            m.count += c[v];
+        }
+#pragma unroll 1
+        for (int v = 0; v < 4; ++v) {
+            if (c[v] < 3 || tl[v] - tf[v] <= m.gap) continue;
+            // the word's inner gaps: its occurrences after the first, four
+            // timestamp loads at a time
+            uint64_t w = x[v] & (x[v] - 1);
+            int64_t prev = tf[v];
+            const uint64_t r = row0 + 64ull * (h + v);
+            while (w) {
+                uint64_t rr[4];
+                bool ok[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    ok[j] = w != 0;
+                    rr[j] = r + (ok[j] ? (uint64_t)__builtin_ctzll(w) : 0);
+                    w &= w - 1;
+                }
+                int64_t t[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) t[j] = ts[rr[j]];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (!ok[j]) continue;
+                    if (t[j] - prev > m.gap) m.gap = t[j] - prev;
+                    prev = t[j];
+                }
+            }
         }
     }
 }
@@ -617,8 +651,7 @@ __global__ __launch_bounds__(256) void k_support_periodic(SupportArgs a) {
                         uint64_t X[8];
 #pragma unroll
                         for (int v = 0; v < 8; ++v) X[v] = P[v] & row[v];
-#pragma unroll
-                        for (int v = 0; v < 8; ++v) pp_scan_word(m, X[v], (w0 + v) * 64, ts, n);
+                        pp_scan_lane(m, X, w0 * 64, ts, n);
                     }
                     m = pp_wave_reduce(m, lane);
                     if (lane == 0) wpart[wave][e - eb] = m;
@@ -767,7 +800,7 @@ __global__ __launch_bounds__(256) void k_intersect(const uint64_t *__restrict__ 
 // kernel holds 8 blocks per CU and runs best at 7 (a full grid waits for
 // slots the previous launch's last waves still hold); the 2048-word kernel
 // holds 4 and runs best oversubscribed at 8 (late blocks take the leftover
-// work); the periodic kernel (93 VGPRs: five waves per SIMD) runs best at 5.
+// work); the periodic kernel (119 VGPRs: four waves per SIMD) runs best at 4.
 uint32_t persistent_grid(const cpm_ctx *ctx, uint32_t per_cu, uint32_t grid) {
     const uint32_t g = (uint32_t)ctx->cu_count * per_cu / 8 * 8;
     return std::max<uint32_t>(8, std::min(grid, g));
@@ -796,7 +829,7 @@ int launch_support_count(cpm_ctx *ctx, const SupportArgs &a, uint32_t tile_words
 
 int launch_support_periodic(cpm_ctx *ctx, const SupportArgs &a, uint32_t grid) {
     if (a.te == 0) return fail(CPM_ERR_INVALID, "zero tiles per work range");
-    hipLaunchKernelGGL(k_support_periodic, dim3(persistent_grid(ctx, 5, grid)), dim3(256), 0,
+    hipLaunchKernelGGL(k_support_periodic, dim3(persistent_grid(ctx, 4, grid)), dim3(256), 0,
                        ctx->stream, a);
     CPM_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_periodic_finalize, dim3(grid), dim3(256), 0, ctx->stream, a);
