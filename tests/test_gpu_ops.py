@@ -193,6 +193,31 @@ def test_periodic_eval(ctx, n):
         assert np.array_equal(s, es) and np.array_equal(p, ep)
 
 
+@pytest.mark.parametrize("seed", [3, 4])
+def test_periodic_eval_inner_gaps(ctx, seed):
+    """Dense rows whose largest gap lies between two occurrences of one
+    64-row word: steps of 1 with a few jumps of up to 900.  The kernel walks
+    a word's inner occurrences only when the word's span exceeds the largest
+    gap already known; the periods must still be the oracle's."""
+    n, items = 20_000, 6
+    rng = np.random.default_rng(seed)
+    dense = rng.random((n, items)) < np.array([0.9, 0.7, 0.5, 0.3, 0.1, 0.02])
+    rows, cols = np.nonzero(dense)
+    steps = np.ones(n, np.int64)
+    steps[rng.choice(n, 12, replace=False)] = rng.integers(50, 900, 12)
+    ts = np.cumsum(steps)
+    end = int(ts[-1]) + 3
+    bits = ctx.build_bits(rows.astype(np.uint64), cols.astype(np.uint32), n, items)
+    host = oracle.bits_build(rows.astype(np.uint64), cols.astype(np.uint32), n, items)
+    for width in (1, 2, 3):
+        cw = np.array([c for c in combinations(range(items), width)], np.uint32)
+        s, p = ctx.periodic_eval(bits, ts, end, cw)
+        es, ep = oracle.periodic_eval(host, ts, n, end, cw)
+        assert np.array_equal(s, es) and np.array_equal(p, ep), width
+    for ms, per in [(1, 10_000), (200, 1000)]:
+        assert ctx.pfeclat_mine(bits, ts, end, ms, per).to_dict() == oracle.pfeclat(host, ts, n, end, ms, per)
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_eclat_mine_random(ctx, seed):
     rng = random.Random(seed)
