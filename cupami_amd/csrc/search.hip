@@ -488,6 +488,11 @@ __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
         big[atomicAdd(&n_big, 1u)] = (uint32_t)(t0 + tid);
     __syncthreads();
     const uint32_t nb = n_big;
+    // a large task's kept items (local indexes), when there are at most
+    // kBigKept of them: the mask, child-count and clear passes then read them
+    // here instead of through kept -> ext, two dependent loads each
+    constexpr uint32_t kBigKept = 1024;
+    __shared__ uint32_t kf[kBigKept];
     for (uint32_t bi = 0; bi < nb; ++bi) {
         const uint32_t t = big[bi];
         const uint32_t a = p.L.ext_off[t], b = p.L.ext_off[t + 1];
@@ -495,26 +500,35 @@ __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
         uint32_t nk = 0;
         for (uint32_t i0 = a; i0 < b; i0 += 256) {
             const uint32_t e = i0 + tid;
+            const uint32_t x = e < b ? p.L.ext[e] : 0u;  // loaded beside the support
             const uint32_t f = (e < b && keep_entry(p, D, e)) ? 1u : 0u;
             uint32_t tot;
             const uint32_t pos = block_scan(f, lds, &tot);
-            if (f) p.kept[a + nk + pos] = e;
+            if (f) {
+                p.kept[a + nk + pos] = e;
+                if (nk + pos < kBigKept) kf[nk + pos] = x - D.fbase;
+            }
             nk += tot;
         }
         __syncthreads();
+        const bool cached = nk <= kBigKept;
+        auto item = [&](uint32_t j) { return cached ? kf[j] : p.L.ext[p.kept[a + j]] - D.fbase; };
         uint32_t cext = 0;
         if (p.emit_children && nk >= 2) {
             uint32_t c = 0;
             if (p.pairbits) {
-                mask_set(lmask, p.kept, p.L.ext, a, nk, D.fbase);
+                for (uint32_t j = tid; j < nk; j += blockDim.x) {
+                    const uint32_t f = item(j);
+                    atomicOr(&lmask[f >> 5], 1u << (f & 31));
+                }
                 __syncthreads();
                 if (p.tmask)
                     for (uint32_t q = tid; q < D.pair_w32; q += blockDim.x)
                         p.tmask[(uint64_t)t * p.tmask_w + q] = lmask[q];
-                const uint32_t wlast = (p.L.ext[p.kept[a + nk - 1]] - D.fbase) >> 5;
+                const uint32_t wlast = item(nk - 1) >> 5;
                 const uint32_t *table = p.pairbits + D.pair_off;
                 for (uint32_t j = tid; j + 1 < nk; j += 256) {
-                    const uint32_t f = p.L.ext[p.kept[a + j]] - D.fbase;
+                    const uint32_t f = item(j);
                     const uint32_t cj = child_count(lmask, table + (uint64_t)f * D.pair_w32, f, wlast);
                     p.ccount[a + j] = cj;
                     c += cj;
@@ -523,7 +537,8 @@ __global__ __launch_bounds__(256) void k_plan_count(PlanArgs p) {
                 for (uint32_t j = tid; j + 1 < nk; j += 256) c += nk - 1 - j;
             }
             (void)block_scan(c, lds, &cext);
-            if (p.pairbits) mask_clear(lmask, p.kept, p.L.ext, a, nk, D.fbase);
+            if (p.pairbits)
+                for (uint32_t j = tid; j < nk; j += blockDim.x) lmask[item(j) >> 5] = 0;
         }
         const uint32_t nc = (p.emit_children && nk >= 2) ? nk - 1 : 0;
         if (tid == 0) {
