@@ -1325,15 +1325,23 @@ __global__ __launch_bounds__(256) void k_level2_frequent_write(const L2Db *l2, u
 }
 
 // level-2 supports from the per-database Gram matrices
-__global__ void k_gram_to_level2(LevelPtrs L, const DbDev *dbs, const int32_t *gram, const uint64_t *goff) {
+// (with `pairbits`, also the pair tables of the frequent pairs: what
+// k_pair_table would write from these supports, in the same pass)
+__global__ void k_gram_to_level2(LevelPtrs L, const DbDev *dbs, const int32_t *gram, const uint64_t *goff,
+                                 uint32_t *pairbits) {
     const uint32_t n = (uint32_t)L.sz->n_tasks;
     for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
         const uint32_t i = L.db[t];
         const DbDev &D = dbs[i];
         const uint32_t a = L.prefix[(uint64_t)t * kMaxDepth] - D.fbase;
         const int32_t *g = gram + goff[i] + (uint64_t)a * D.nf;
-        for (uint32_t e = L.ext_off[t] + threadIdx.x; e < L.ext_off[t + 1]; e += blockDim.x)
-            L.sup[e] = (uint32_t)g[L.ext[e] - D.fbase];
+        uint32_t *row = pairbits && D.pair_w32 ? pairbits + D.pair_off + (uint64_t)a * D.pair_w32 : nullptr;
+        for (uint32_t e = L.ext_off[t] + threadIdx.x; e < L.ext_off[t + 1]; e += blockDim.x) {
+            const uint32_t b = L.ext[e] - D.fbase;
+            const uint32_t sup = (uint32_t)g[b];
+            L.sup[e] = sup;
+            if (row && sup >= D.minsup) atomicOr(row + (b >> 5), 1u << (b & 31));
+        }
     }
 }
 
@@ -2089,6 +2097,7 @@ struct Search {
         CPM_CHECK(flush());
 
         const unsigned g2 = (unsigned)std::min<uint64_t>(t2, 8192);
+        bool pairs_done = false;  // the pair tables were written with the level-2 supports
         if (gram_ok && periodic) {
             // level 2 = the frequent pairs (Gram supports), then their periods
             CPM_CHECK(launch_gram(ctx, gdev, w.gdbs.as<GramDb>(), w.gram.as<int32_t>(), gram_elems, gram_ops, false));
@@ -2110,13 +2119,15 @@ struct Search {
                 CPM_CHECK(launch_gram(ctx, gdev, w.gdbs.as<GramDb>(), w.gram.as<int32_t>(), gram_elems, gram_ops,
                                       false));
                 hipLaunchKernelGGL(k_gram_to_level2, dim3(g2), dim3(256), 0, st, S1.ptrs(sz(1)),
-                                   w.dbs.as<DbDev>(), w.gram.as<int32_t>(), w.goff.as<uint64_t>());
+                                   w.dbs.as<DbDev>(), w.gram.as<int32_t>(), w.goff.as<uint64_t>(),
+                                   pair_words ? w.pairbits.as<uint32_t>() : nullptr);
                 CPM_HIP(hipGetLastError());
+                pairs_done = true;
             } else {
                 CPM_CHECK(count(1, false));
             }
         }
-        if (pair_words) {
+        if (pair_words && !pairs_done) {
             PlanArgs p = plan_args(1, 1);
             hipLaunchKernelGGL(k_pair_table, dim3((unsigned)std::min<uint64_t>(t2, 8192)), dim3(256), 0, st, p,
                                w.pairbits.as<uint32_t>());
